@@ -1,0 +1,126 @@
+/*
+ * slimefinder: MI355X (gfx950) slime-chunk cluster search, C ABI.
+ *
+ * Drop-in for the reference's native search libraries: SlimeCore.cpp (CPU)
+ * and SlimeCoreGPU.cu (CUDA "v1"). Every function keeps the reference name,
+ * argument meaning, result layout and error convention; the comment on each
+ * cites the reference declaration it replaces. Plain C types only; the
+ * library owns all device memory and the caller owns every host buffer.
+ *
+ * Searches are synchronous and run on the current HIP device. The control
+ * functions are process-global and may be called from another thread while a
+ * search runs, as in the reference.
+ */
+#ifndef SLIMEFINDER_H
+#define SLIMEFINDER_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Result row, 24 bytes: SlimeCore.cpp:182-189, SlimeCoreGPU.cu:198-205.
+ * center_x/center_z are block coordinates cx*16+8 (wrapped to int32). With
+ * precise_afk == 0, obs_count is 0 and afk_x/afk_z equal the center. */
+typedef struct {
+    int32_t size;
+    int32_t center_x;
+    int32_t center_z;
+    int32_t obs_count;
+    int32_t afk_x;
+    int32_t afk_z;
+} ExtChunkResult;
+
+/* ---- search (returns the total number of centers with min_size <= count
+ * <= max_size; fills the best min(total, max_results) rows ranked by size
+ * desc, center_x asc, center_z asc; -1 on invalid arguments or device error) */
+
+/* SlimeCore.cpp:643-651. `threads` is accepted and ignored (the GPU runs it). */
+int64_t search_slime_clusters_centered(int64_t seed, int64_t rd_max, int32_t min_size,
+                                       int32_t max_size, int64_t rd_min, int32_t search_center_x,
+                                       int32_t search_center_z, ExtChunkResult* results_buffer,
+                                       int32_t max_results, int32_t threads, int32_t precise_afk);
+
+/* SlimeCore.cpp:634-641: search around chunk (0, 0). */
+int64_t search_slime_clusters(int64_t seed, int64_t rd_max, int32_t min_size, int32_t max_size,
+                              int64_t rd_min, ExtChunkResult* results_buffer, int32_t max_results,
+                              int32_t threads, int32_t precise_afk);
+
+/* SlimeCoreGPU.cu:3927-3935 (GPU twin: no `threads`, rd_max <= (INT32_MAX-19)/2). */
+int64_t search_slime_clusters_gpu_v1_centered(int64_t seed, int64_t rd_max, int32_t min_size,
+                                              int32_t max_size, int64_t rd_min,
+                                              int32_t search_center_x, int32_t search_center_z,
+                                              ExtChunkResult* results_buffer, int32_t max_results,
+                                              int32_t precise_afk);
+
+/* SlimeCoreGPU.cu:3918-3925. */
+int64_t search_slime_clusters_gpu_v1(int64_t seed, int64_t rd_max, int32_t min_size,
+                                     int32_t max_size, int64_t rd_min,
+                                     ExtChunkResult* results_buffer, int32_t max_results,
+                                     int32_t precise_afk);
+
+/* SlimeCoreGPU.cu:3937-3955: legacy aliases. */
+int64_t search_slime_clusters_gpu_v34(int64_t seed, int64_t rd_max, int32_t min_size,
+                                      int32_t max_size, int64_t rd_min,
+                                      ExtChunkResult* results_buffer, int32_t max_results,
+                                      int32_t precise_afk);
+int64_t search_slime_clusters_gpu_v34_centered(int64_t seed, int64_t rd_max, int32_t min_size,
+                                               int32_t max_size, int64_t rd_min,
+                                               int32_t search_center_x, int32_t search_center_z,
+                                               ExtChunkResult* results_buffer,
+                                               int32_t max_results, int32_t precise_afk);
+
+/* ---- control plane: SlimeCore.cpp:341-371, SlimeCoreGPU.cu:3085-3121 ---- */
+
+void request_cancel(void);  /* also clears pause */
+void request_pause(void);
+void resume_search(void);
+void reset_cancel(void);    /* also zeroes progress and the counters below */
+int32_t get_progress(void); /* 1..79 while scanning, 90 when done */
+int64_t get_processed_centers(void);
+int64_t get_gpu_scan_work_ns(void);
+void set_y_scan_config(int32_t enabled, int32_t platform_y, int32_t y_min, int32_t y_max,
+                       int32_t y_step);
+
+/* SlimeCoreGPU.cu:3142-3161 (host-side single-chunk test). */
+bool is_slime_chunk_c(int64_t seed, int64_t cx, int64_t cz);
+
+/* SlimeCoreGPU.cu:3035-3083, 3118-3121, under HIP semantics. */
+int32_t get_cuda_device_count(void);
+int32_t get_cuda_device_name(char* buffer, int32_t buffer_size);
+void cleanup_gpu_resources(void);
+int32_t get_gpu_v1_shape(void);     /* last tile shape id, -1 before any scan */
+int32_t get_gpu_v1_algorithm(void); /* last scan kernel id, -1 before any scan */
+int32_t get_gpu_v1_rng(void);
+void set_gpu_v1_rng_override(int32_t value);
+int32_t get_gpu_v34_shape(void);
+int32_t get_gpu_v34_rng(void);
+void set_gpu_v34_rng_override(int32_t value);
+
+/* ---- extension with no counterpart in the reference ---------------------
+ * Rectangle scan with the score histogram, for callers that partition a
+ * region themselves (and for size-independent parity checks). Centers
+ * [x0, x0+w) x [z0, z0+h); hist222 (optional) receives the count of in-range
+ * hits per exact score 0..221. Same return convention as the searches. */
+int64_t slime_scan_rect(int64_t seed, int32_t x0, int32_t z0, int64_t w, int64_t h,
+                        int32_t min_size, int32_t max_size, ExtChunkResult* results_buffer,
+                        int32_t max_results, int64_t* hist222);
+
+/* Extension with no counterpart in the reference: device time of the scan
+ * kernel launches since the last reset, measured with HIP events on the
+ * engine's stream (for benchmarks). reset != 0 zeroes the counters after the
+ * read. */
+void slime_kernel_stats(int64_t* launches, int64_t* kernel_ns, int32_t reset);
+
+/* Extension with no counterpart in the reference: compute units, peak shader
+ * clock (kHz) and HBM size of the current device; 0 on success, -1 if no
+ * device is usable. */
+int32_t slime_device_info(int32_t* compute_units, int32_t* clock_khz, int64_t* hbm_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SLIMEFINDER_H */

@@ -1,0 +1,125 @@
+"""ctypes bindings for the CPU checker (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module. It loads:
+  * liboracle.so - the scalar C restatement in this folder;
+  * _ref/libslimecore_ref[_avx2].so - the reference's SlimeCore.cpp compiled
+    unmodified by `make -C oracle ref` (present only where it was built).
+"""
+import ctypes as C
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+class ChunkRow(C.Structure):
+    """Result row layout shared by the reference ABI (SlimeCore.cpp:182-189)."""
+    _fields_ = [("size", C.c_int32), ("center_x", C.c_int32), ("center_z", C.c_int32),
+                ("obs_count", C.c_int32), ("afk_x", C.c_int32), ("afk_z", C.c_int32)]
+
+
+def build(ref=False):
+    """Compile the C restatement (and the reference, when its tree exists)."""
+    targets = ["all"]
+    if ref and os.path.isdir(os.environ.get("REF_DIR", "/root/reference")):
+        targets.append("ref")
+    subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
+
+
+_oracle = None
+
+
+def oracle_lib():
+    global _oracle
+    if _oracle is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        lib = C.CDLL(path)
+        i32, i64, u64 = C.c_int32, C.c_int64, C.c_uint64
+        lib.oracle_is_slime.argtypes = [i64, i64, i64]
+        lib.oracle_is_slime.restype = i32
+        lib.oracle_first_draw.argtypes = [i64, i64, i64]
+        lib.oracle_first_draw.restype = i32
+        lib.oracle_slime_from_state0.argtypes = [u64]
+        lib.oracle_slime_from_state0.restype = i32
+        lib.oracle_java_next_int.argtypes = [i64]
+        lib.oracle_java_next_int.restype = i32
+        lib.oracle_x_term.argtypes = [i64]
+        lib.oracle_x_term.restype = u64
+        lib.oracle_z_term.argtypes = [i64]
+        lib.oracle_z_term.restype = u64
+        lib.oracle_disk_count.argtypes = [i64, i64, i64]
+        lib.oracle_disk_count.restype = i32
+        lib.oracle_count_rejected_hits.argtypes = []
+        lib.oracle_count_rejected_hits.restype = i64
+        lib.oracle_search_hist.argtypes = [i64, i64, i32, i32, i64, i32, i32,
+                                           C.POINTER(ChunkRow), i32, C.POINTER(C.c_int64)]
+        lib.oracle_search_hist.restype = i64
+        lib.oracle_search_rect.argtypes = [i64, i64, i64, i64, i64, i32, i32,
+                                           C.POINTER(ChunkRow), i32, C.POINTER(C.c_int64)]
+        lib.oracle_search_rect.restype = i64
+        _oracle = lib
+    return _oracle
+
+
+def ref_path(avx2=True):
+    name = "libslimecore_ref_avx2.so" if avx2 else "libslimecore_ref.so"
+    return os.path.join(HERE, "_ref", name)
+
+
+def ref_lib(avx2=None):
+    """The compiled reference, or None where it was not built."""
+    if avx2 is None:
+        avx2 = cpu_has_avx2()
+    path = ref_path(avx2)
+    if not os.path.exists(path):
+        return None
+    lib = C.CDLL(path)
+    i32, i64 = C.c_int32, C.c_int64
+    lib.search_slime_clusters_centered.argtypes = [i64, i64, i32, i32, i64, i32, i32,
+                                                   C.POINTER(ChunkRow), i32, i32, i32]
+    lib.search_slime_clusters_centered.restype = i64
+    lib.reset_cancel.argtypes = []
+    return lib
+
+
+def cpu_has_avx2():
+    try:
+        with open("/proc/cpuinfo") as f:
+            return " avx2" in f.read()
+    except OSError:
+        return False
+
+
+def rows_to_tuples(buf, n):
+    return [(buf[i].size, buf[i].center_x, buf[i].center_z, buf[i].obs_count,
+             buf[i].afk_x, buf[i].afk_z) for i in range(n)]
+
+
+def oracle_search(seed, rd_max, min_size, max_size=221, rd_min=0, cx=0, cz=0,
+                  max_results=65536, hist=False):
+    """Returns (total, rows[, hist]) from the C restatement."""
+    lib = oracle_lib()
+    buf = (ChunkRow * max(1, max_results))()
+    h = (C.c_int64 * 222)()
+    total = lib.oracle_search_hist(seed, rd_max, min_size, max_size, rd_min, cx, cz,
+                                   buf, max_results, h)
+    rows = rows_to_tuples(buf, min(max(total, 0), max_results)) if total >= 0 else []
+    if hist:
+        return total, rows, list(h)
+    return total, rows
+
+
+def ref_search(lib, seed, rd_max, min_size, max_size=221, rd_min=0, cx=0, cz=0,
+               max_results=65536, threads=None):
+    """Returns (total, rows) from the compiled reference."""
+    if threads is None:
+        threads = os.cpu_count() or 1
+    buf = (ChunkRow * max(1, max_results))()
+    lib.reset_cancel()
+    total = lib.search_slime_clusters_centered(seed, rd_max, min_size, max_size, rd_min,
+                                               cx, cz, buf, max_results, threads, 0)
+    rows = rows_to_tuples(buf, min(max(total, 0), max_results)) if total >= 0 else []
+    return total, rows

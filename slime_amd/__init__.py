@@ -1,0 +1,154 @@
+"""slime_amd: MI355X-native slime-chunk cluster search.
+
+Python side of the drop-in: ctypes bindings to libslimefinder.so with the
+reference's function names and argument meaning (SlimeCore.cpp:340-651,
+SlimeCoreGPU.cu:3034-3955, called from Python via ctypes exactly as the
+reference's GUI does: xp_scan.py:537-551). There is no Python or CPU
+fallback: importing a function fails loudly if the native library is absent.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libslimefinder.so")
+
+__all__ = [
+    "ExtChunkResult", "lib", "build", "search_slime_clusters_centered",
+    "search_slime_clusters", "search_slime_clusters_gpu_v1_centered", "scan_rect",
+    "EXPORTS",
+]
+
+# Every symbol include/slimefinder.h declares.
+EXPORTS = (
+    "search_slime_clusters_centered", "search_slime_clusters",
+    "search_slime_clusters_gpu_v1_centered", "search_slime_clusters_gpu_v1",
+    "search_slime_clusters_gpu_v34", "search_slime_clusters_gpu_v34_centered",
+    "request_cancel", "request_pause", "resume_search", "reset_cancel", "get_progress",
+    "get_processed_centers", "get_gpu_scan_work_ns", "set_y_scan_config", "is_slime_chunk_c",
+    "get_cuda_device_count", "get_cuda_device_name", "cleanup_gpu_resources",
+    "get_gpu_v1_shape", "get_gpu_v1_algorithm", "get_gpu_v1_rng", "set_gpu_v1_rng_override",
+    "get_gpu_v34_shape", "get_gpu_v34_rng", "set_gpu_v34_rng_override", "slime_scan_rect",
+    "slime_kernel_stats", "slime_device_info",
+)
+
+
+class ExtChunkResult(C.Structure):
+    """24-byte result row (SlimeCore.cpp:182-189)."""
+    _fields_ = [("size", C.c_int32), ("center_x", C.c_int32), ("center_z", C.c_int32),
+                ("obs_count", C.c_int32), ("afk_x", C.c_int32), ("afk_z", C.c_int32)]
+
+
+def build(jobs=8):
+    """Compiles libslimefinder.so for gfx950 with hipcc (no GPU needed)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "csrc"), f"-j{jobs}"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    """The loaded native library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with slime_amd.build() "
+                           "(there is no fallback path)")
+    L = C.CDLL(LIB_PATH)
+    i32, i64, P = C.c_int32, C.c_int64, C.POINTER(ExtChunkResult)
+    L.search_slime_clusters_centered.argtypes = [i64, i64, i32, i32, i64, i32, i32, P, i32, i32, i32]
+    L.search_slime_clusters_centered.restype = i64
+    L.search_slime_clusters.argtypes = [i64, i64, i32, i32, i64, P, i32, i32, i32]
+    L.search_slime_clusters.restype = i64
+    for name in ("search_slime_clusters_gpu_v1_centered", "search_slime_clusters_gpu_v34_centered"):
+        getattr(L, name).argtypes = [i64, i64, i32, i32, i64, i32, i32, P, i32, i32]
+        getattr(L, name).restype = i64
+    for name in ("search_slime_clusters_gpu_v1", "search_slime_clusters_gpu_v34"):
+        getattr(L, name).argtypes = [i64, i64, i32, i32, i64, P, i32, i32]
+        getattr(L, name).restype = i64
+    L.slime_scan_rect.argtypes = [i64, i32, i32, i64, i64, i32, i32, P, i32, C.POINTER(C.c_int64)]
+    L.slime_scan_rect.restype = i64
+    for name in ("get_progress", "get_cuda_device_count", "get_gpu_v1_shape",
+                 "get_gpu_v1_algorithm", "get_gpu_v1_rng", "get_gpu_v34_shape", "get_gpu_v34_rng"):
+        getattr(L, name).restype = i32
+        getattr(L, name).argtypes = []
+    for name in ("get_processed_centers", "get_gpu_scan_work_ns"):
+        getattr(L, name).restype = i64
+        getattr(L, name).argtypes = []
+    for name in ("request_cancel", "request_pause", "resume_search", "reset_cancel",
+                 "cleanup_gpu_resources"):
+        getattr(L, name).restype = None
+        getattr(L, name).argtypes = []
+    L.set_y_scan_config.argtypes = [i32, i32, i32, i32, i32]
+    L.set_y_scan_config.restype = None
+    L.is_slime_chunk_c.argtypes = [i64, i64, i64]
+    L.is_slime_chunk_c.restype = C.c_bool
+    L.get_cuda_device_name.argtypes = [C.c_char_p, i32]
+    L.get_cuda_device_name.restype = i32
+    L.set_gpu_v1_rng_override.argtypes = [i32]
+    L.set_gpu_v34_rng_override.argtypes = [i32]
+    L.slime_kernel_stats.argtypes = [C.POINTER(i64), C.POINTER(i64), i32]
+    L.slime_kernel_stats.restype = None
+    L.slime_device_info.argtypes = [C.POINTER(i32), C.POINTER(i32), C.POINTER(i64)]
+    L.slime_device_info.restype = i32
+    _lib = L
+    return L
+
+
+def _rows(buf, total, k):
+    n = min(max(total, 0), k)
+    return [(buf[i].size, buf[i].center_x, buf[i].center_z, buf[i].obs_count,
+             buf[i].afk_x, buf[i].afk_z) for i in range(n)]
+
+
+def search_slime_clusters_centered(seed, rd_max, min_size, max_size=221, rd_min=0,
+                                   search_center_x=0, search_center_z=0, max_results=65536,
+                                   threads=0, precise_afk=0):
+    """Mirror of SlimeCore.cpp:643-651. Returns (total, rows); total -1 on bad args."""
+    buf = (ExtChunkResult * max(1, max_results))()
+    total = lib().search_slime_clusters_centered(seed, rd_max, min_size, max_size, rd_min,
+                                                 search_center_x, search_center_z, buf,
+                                                 max_results, threads, precise_afk)
+    return total, _rows(buf, total, max_results)
+
+
+def search_slime_clusters(seed, rd_max, min_size, max_size=221, rd_min=0, max_results=65536,
+                          threads=0, precise_afk=0):
+    """Mirror of SlimeCore.cpp:634-641 (search around chunk 0,0)."""
+    return search_slime_clusters_centered(seed, rd_max, min_size, max_size, rd_min, 0, 0,
+                                          max_results, threads, precise_afk)
+
+
+def search_slime_clusters_gpu_v1_centered(seed, rd_max, min_size, max_size=221, rd_min=0,
+                                          search_center_x=0, search_center_z=0,
+                                          max_results=65536, precise_afk=0):
+    """Mirror of SlimeCoreGPU.cu:3927-3935."""
+    buf = (ExtChunkResult * max(1, max_results))()
+    total = lib().search_slime_clusters_gpu_v1_centered(seed, rd_max, min_size, max_size, rd_min,
+                                                        search_center_x, search_center_z, buf,
+                                                        max_results, precise_afk)
+    return total, _rows(buf, total, max_results)
+
+
+def device_info():
+    """(compute_units, clock_khz, hbm_bytes) of the current HIP device, or None."""
+    cu, clk, mem = C.c_int32(), C.c_int32(), C.c_int64()
+    if lib().slime_device_info(C.byref(cu), C.byref(clk), C.byref(mem)) != 0:
+        return None
+    return cu.value, clk.value, mem.value
+
+
+def kernel_stats(reset=False):
+    """(launches, kernel_ns) of the scan kernels since the last reset."""
+    n, ns = C.c_int64(), C.c_int64()
+    lib().slime_kernel_stats(C.byref(n), C.byref(ns), 1 if reset else 0)
+    return n.value, ns.value
+
+
+def scan_rect(seed, x0, z0, w, h, min_size, max_size=221, max_results=65536):
+    """Rectangle scan with the score histogram: (total, rows, hist[222])."""
+    buf = (ExtChunkResult * max(1, max_results))()
+    hist = (C.c_int64 * 222)()
+    total = lib().slime_scan_rect(seed, x0, z0, w, h, min_size, max_size, buf, max_results, hist)
+    return total, _rows(buf, total, max_results), list(hist)

@@ -1,0 +1,235 @@
+// C ABI of libslimefinder.so: the reference's search and control exports.
+//
+// Argument validation, return conventions and control semantics follow
+// SlimeCore.cpp:340-651 (CPU names) and SlimeCoreGPU.cu:3034-3955 (GPU
+// names); the only difference between the two families is the rd_max limit
+// ((INT32_MAX-17)/2 vs (INT32_MAX-19)/2) and the CPU `threads` argument.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstring>
+
+#include "../../include/slimefinder.h"
+#include "engine.h"
+#include "slime_math.h"
+
+using slime::control;
+
+static_assert(sizeof(ExtChunkResult) == sizeof(slime::ResultRow), "row layout");
+
+namespace {
+
+int32_t g_last_shape = -1;
+int32_t g_last_algorithm = -1;
+int32_t g_rng_override = -1;
+
+struct YScan {
+    int32_t enabled = 0, platform_y = -64, y_min = -64, y_max = 64, y_step = 4;
+} g_yscan;
+
+// SlimeCore.cpp:378-384 / SlimeCoreGPU.cu:3169-3175.
+bool valid_args(int64_t rd_max, int64_t rd_max_limit, int32_t min_size, int32_t max_size,
+                int64_t rd_min, int32_t cx, int32_t cz, const ExtChunkResult* out,
+                int32_t max_results) {
+    if (!out || max_results <= 0 || rd_max < 0 || rd_max > rd_max_limit || rd_min < 0 ||
+        rd_min > rd_max || min_size < 0 || min_size > 221 || max_size < min_size)
+        return false;
+    if ((int64_t)cx - rd_max - 8 < INT32_MIN || (int64_t)cx + rd_max + 8 > INT32_MAX ||
+        (int64_t)cz - rd_max - 8 < INT32_MIN || (int64_t)cz + rd_max + 8 > INT32_MAX)
+        return false;
+    return true;
+}
+
+int64_t search_common(int64_t seed, int64_t rd_max, int32_t min_size, int32_t max_size,
+                      int64_t rd_min, int32_t cx, int32_t cz, ExtChunkResult* out,
+                      int32_t max_results, int64_t rd_max_limit) {
+    if (!valid_args(rd_max, rd_max_limit, min_size, max_size, rd_min, cx, cz, out, max_results))
+        return -1;
+    control().progress.store(1, std::memory_order_relaxed);
+    slime::SearchSpec spec;
+    spec.seed = seed;
+    spec.x0 = (int32_t)((int64_t)cx - rd_max);
+    spec.z0 = (int32_t)((int64_t)cz - rd_max);
+    spec.w = 2 * rd_max + 1;
+    spec.h = 2 * rd_max + 1;
+    spec.min_size = min_size;
+    spec.max_size = max_size;
+    spec.rd_min = rd_min;
+    spec.acx = cx;
+    spec.acz = cz;
+    g_last_shape = 0;
+    g_last_algorithm = 0;
+    return slime::run_search(spec, reinterpret_cast<slime::ResultRow*>(out), max_results);
+}
+
+constexpr int64_t kCpuRdLimit = (INT32_MAX - 17LL) / 2LL;
+constexpr int64_t kGpuRdLimit = (INT32_MAX - 19LL) / 2LL;
+
+}  // namespace
+
+extern "C" {
+
+int64_t search_slime_clusters_centered(int64_t seed, int64_t rd_max, int32_t min_size,
+                                       int32_t max_size, int64_t rd_min, int32_t search_center_x,
+                                       int32_t search_center_z, ExtChunkResult* results_buffer,
+                                       int32_t max_results, int32_t threads, int32_t precise_afk) {
+    (void)threads;
+    (void)precise_afk;
+    return search_common(seed, rd_max, min_size, max_size, rd_min, search_center_x,
+                         search_center_z, results_buffer, max_results, kCpuRdLimit);
+}
+
+int64_t search_slime_clusters(int64_t seed, int64_t rd_max, int32_t min_size, int32_t max_size,
+                              int64_t rd_min, ExtChunkResult* results_buffer, int32_t max_results,
+                              int32_t threads, int32_t precise_afk) {
+    return search_slime_clusters_centered(seed, rd_max, min_size, max_size, rd_min, 0, 0,
+                                          results_buffer, max_results, threads, precise_afk);
+}
+
+int64_t search_slime_clusters_gpu_v1_centered(int64_t seed, int64_t rd_max, int32_t min_size,
+                                              int32_t max_size, int64_t rd_min,
+                                              int32_t search_center_x, int32_t search_center_z,
+                                              ExtChunkResult* results_buffer, int32_t max_results,
+                                              int32_t precise_afk) {
+    (void)precise_afk;
+    return search_common(seed, rd_max, min_size, max_size, rd_min, search_center_x,
+                         search_center_z, results_buffer, max_results, kGpuRdLimit);
+}
+
+int64_t search_slime_clusters_gpu_v1(int64_t seed, int64_t rd_max, int32_t min_size,
+                                     int32_t max_size, int64_t rd_min,
+                                     ExtChunkResult* results_buffer, int32_t max_results,
+                                     int32_t precise_afk) {
+    return search_slime_clusters_gpu_v1_centered(seed, rd_max, min_size, max_size, rd_min, 0, 0,
+                                                 results_buffer, max_results, precise_afk);
+}
+
+int64_t search_slime_clusters_gpu_v34(int64_t seed, int64_t rd_max, int32_t min_size,
+                                      int32_t max_size, int64_t rd_min,
+                                      ExtChunkResult* results_buffer, int32_t max_results,
+                                      int32_t precise_afk) {
+    return search_slime_clusters_gpu_v1(seed, rd_max, min_size, max_size, rd_min, results_buffer,
+                                        max_results, precise_afk);
+}
+
+int64_t search_slime_clusters_gpu_v34_centered(int64_t seed, int64_t rd_max, int32_t min_size,
+                                               int32_t max_size, int64_t rd_min,
+                                               int32_t search_center_x, int32_t search_center_z,
+                                               ExtChunkResult* results_buffer,
+                                               int32_t max_results, int32_t precise_afk) {
+    return search_slime_clusters_gpu_v1_centered(seed, rd_max, min_size, max_size, rd_min,
+                                                 search_center_x, search_center_z,
+                                                 results_buffer, max_results, precise_afk);
+}
+
+int64_t slime_scan_rect(int64_t seed, int32_t x0, int32_t z0, int64_t w, int64_t h,
+                        int32_t min_size, int32_t max_size, ExtChunkResult* results_buffer,
+                        int32_t max_results, int64_t* hist222) {
+    if (!results_buffer || max_results <= 0 || w <= 0 || h <= 0 || w > INT32_MAX ||
+        h > INT32_MAX || min_size < 0 || min_size > 221 || max_size < min_size)
+        return -1;
+    slime::SearchSpec spec;
+    spec.seed = seed;
+    spec.x0 = x0;
+    spec.z0 = z0;
+    spec.w = w;
+    spec.h = h;
+    spec.min_size = min_size;
+    spec.max_size = max_size;
+    return slime::run_search(spec, reinterpret_cast<slime::ResultRow*>(results_buffer),
+                             max_results, hist222);
+}
+
+void slime_kernel_stats(int64_t* launches, int64_t* kernel_ns, int32_t reset) {
+    slime::kernel_stats(launches, kernel_ns, reset != 0);
+}
+
+int32_t slime_device_info(int32_t* compute_units, int32_t* clock_khz, int64_t* hbm_bytes) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return -1;
+    }
+    if (compute_units) *compute_units = prop.multiProcessorCount;
+    if (clock_khz) *clock_khz = prop.clockRate;
+    if (hbm_bytes) *hbm_bytes = (int64_t)prop.totalGlobalMem;
+    return 0;
+}
+
+void request_cancel(void) {
+    control().cancel.store(1, std::memory_order_relaxed);
+    control().pause.store(0, std::memory_order_relaxed);
+}
+
+void request_pause(void) { control().pause.store(1, std::memory_order_relaxed); }
+
+void resume_search(void) { control().pause.store(0, std::memory_order_relaxed); }
+
+void reset_cancel(void) {
+    control().cancel.store(0, std::memory_order_relaxed);
+    control().pause.store(0, std::memory_order_relaxed);
+    control().progress.store(0, std::memory_order_relaxed);
+    control().processed_centers.store(0, std::memory_order_relaxed);
+    control().scan_work_ns.store(0, std::memory_order_relaxed);
+}
+
+int32_t get_progress(void) { return control().progress.load(std::memory_order_relaxed); }
+
+int64_t get_processed_centers(void) {
+    return control().processed_centers.load(std::memory_order_relaxed);
+}
+
+int64_t get_gpu_scan_work_ns(void) { return control().scan_work_ns.load(std::memory_order_relaxed); }
+
+void set_y_scan_config(int32_t enabled, int32_t platform_y, int32_t y_min, int32_t y_max,
+                       int32_t y_step) {
+    g_yscan.enabled = enabled ? 1 : 0;
+    g_yscan.platform_y = platform_y;
+    g_yscan.y_min = y_min;
+    g_yscan.y_max = y_max;
+    g_yscan.y_step = y_step > 0 ? y_step : 4;
+}
+
+bool is_slime_chunk_c(int64_t seed, int64_t cx, int64_t cz) {
+    return slime::is_slime(seed, (int32_t)cx, (int32_t)cz);
+}
+
+int32_t get_cuda_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+int32_t get_cuda_device_name(char* buffer, int32_t buffer_size) {
+    if (!buffer || buffer_size <= 1) return 0;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        hipGetLastError();
+        buffer[0] = '\0';
+        return 0;
+    }
+    const size_t n = std::min((size_t)buffer_size - 1, std::strlen(prop.name));
+    std::memcpy(buffer, prop.name, n);
+    buffer[n] = '\0';
+    return (int32_t)n;
+}
+
+void cleanup_gpu_resources(void) { slime::release_engine(); }
+
+int32_t get_gpu_v1_shape(void) { return g_last_shape; }
+int32_t get_gpu_v1_algorithm(void) { return g_last_algorithm; }
+int32_t get_gpu_v1_rng(void) { return g_rng_override >= 0 ? g_rng_override : 0; }
+void set_gpu_v1_rng_override(int32_t value) {
+    g_rng_override = (value >= 0 && value <= 2) ? value : -1;
+}
+int32_t get_gpu_v34_shape(void) { return get_gpu_v1_shape(); }
+int32_t get_gpu_v34_rng(void) { return get_gpu_v1_rng(); }
+void set_gpu_v34_rng_override(int32_t value) { set_gpu_v1_rng_override(value); }
+
+}  // extern "C"

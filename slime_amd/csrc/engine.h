@@ -1,0 +1,67 @@
+// Host-side search engine behind the C ABI.
+#pragma once
+
+#include <stdint.h>
+
+#include <atomic>
+#include <vector>
+
+#include "scan.h"
+
+namespace slime {
+
+// Result row of the reference ABI (SlimeCore.cpp:182-189), 24 bytes.
+struct ResultRow {
+    int32_t size;
+    int32_t center_x;  // block coordinates cx*16+8
+    int32_t center_z;
+    int32_t obs_count;
+    int32_t afk_x;
+    int32_t afk_z;
+};
+static_assert(sizeof(ResultRow) == 24, "ABI row is 24 bytes");
+
+// One search: a square of radius rd_max around (center_x, center_z), or any
+// rectangle of centers, with the reference's per-center filters.
+struct SearchSpec {
+    int64_t seed = 0;
+    int32_t x0 = 0, z0 = 0;  // first center
+    int64_t w = 0, h = 0;    // centers
+    int32_t min_size = 0, max_size = 221;
+    int64_t rd_min = 0;
+    int32_t acx = 0, acz = 0;  // annulus center (search center)
+};
+
+// Process-global control flags of the reference ABI (SlimeCore.cpp:16-27,
+// SlimeCoreGPU.cu:3085-3116).
+struct Control {
+    std::atomic<int> cancel{0};
+    std::atomic<int> pause{0};
+    std::atomic<int> progress{0};
+    std::atomic<int64_t> processed_centers{0};
+    std::atomic<int64_t> scan_work_ns{0};
+};
+Control& control();
+
+// Runs one search to completion on the current device. Returns the total
+// number of in-range hits and writes up to max_results ranked rows into out
+// (block coordinates, precise_afk == 0 columns); hist_out (optional) gets the
+// per-score counts. Returns -1 on a device error.
+int64_t run_search(const SearchSpec& spec, ResultRow* out, int32_t max_results,
+                   int64_t* hist_out = nullptr);
+
+// Device time of the scan kernel launches (HIP events on the engine's stream).
+void kernel_stats(int64_t* launches, int64_t* kernel_ns, bool reset);
+
+// Releases every device buffer the engine holds.
+void release_engine();
+
+// Ranking order of the reference: size desc, then center_x asc, center_z asc
+// compared on 64-bit block coordinates (SlimeCore.cpp:191-206).
+inline bool ranks_before(const Hit& a, const Hit& b) {
+    if (a.size != b.size) return a.size > b.size;
+    if (a.cx != b.cx) return a.cx < b.cx;
+    return a.cz < b.cz;
+}
+
+}  // namespace slime

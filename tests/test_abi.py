@@ -1,0 +1,84 @@
+"""The C ABI library on the CPU: it loads, exports every declared symbol, and
+its host-side paths (argument validation, control plane, the single-chunk
+test) behave like the reference without touching a GPU."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from conftest import ROOT, load_golden
+
+HEADER = os.path.join(ROOT, "include", "slimefinder.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:int64_t|int32_t|void|bool)\s+(\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol(native):
+    names = declared_functions()
+    assert len(names) >= 26
+    for name in names:
+        assert hasattr(native, name), name
+
+
+def test_python_mirror_lists_every_export():
+    import slime_amd
+    assert sorted(slime_amd.EXPORTS) == declared_functions()
+
+
+def test_invalid_arguments_return_minus_one(native):
+    import slime_amd
+    cases = [c for c in load_golden("small_searches.json") if c["total"] == -1]
+    assert len(cases) == 10
+    for c in cases:
+        k = c["max_results"]
+        buf = (slime_amd.ExtChunkResult * max(1, k))()
+        got = native.search_slime_clusters_centered(c["seed"], c["rd_max"], c["min_size"],
+                                                    c["max_size"], c["rd_min"], c["cx"], c["cz"],
+                                                    buf if k > 0 else None, k, 1, 0)
+        assert got == -1, c
+    buf = (slime_amd.ExtChunkResult * 4)()
+    # null buffer, and the GPU names' tighter rd_max limit (SlimeCoreGPU.cu:3169-3175)
+    assert native.search_slime_clusters_centered(1, 10, 40, 221, 0, 0, 0, None, 4, 1, 0) == -1
+    lim = ((1 << 31) - 1 - 19) // 2
+    assert native.search_slime_clusters_gpu_v1_centered(1, lim + 1, 40, 221, 0, 0, 0, buf, 4, 0) == -1
+    assert native.search_slime_clusters_gpu_v1(1, -1, 40, 221, 0, buf, 4, 0) == -1
+    assert native.search_slime_clusters(1, 10, 50, 40, 0, buf, 4, 1, 0) == -1
+    assert native.slime_scan_rect(1, 0, 0, 0, 10, 40, 221, buf, 4, None) == -1
+
+
+def test_is_slime_chunk_matches_reference_bits(native):
+    for c in load_golden("predicate.json"):
+        for i in range(0, 21, 4):
+            for j in range(21):
+                want = bool((c["rows"][i] >> j) & 1)
+                assert native.is_slime_chunk_c(c["seed"], c["cx"] - 10 + i, c["cz"] - 10 + j) == want
+
+
+def test_control_plane(native):
+    native.reset_cancel()
+    assert native.get_progress() == 0
+    assert native.get_processed_centers() == 0
+    native.request_pause()
+    native.resume_search()
+    native.request_cancel()
+    native.reset_cancel()
+    native.set_y_scan_config(1, -64, -64, 64, 0)
+    assert native.get_gpu_v1_algorithm() in (-1, 0, 1, 2, 3)
+    native.set_gpu_v1_rng_override(1)
+    assert native.get_gpu_v1_rng() == 1
+    native.set_gpu_v1_rng_override(9)
+    assert native.get_gpu_v34_rng() == 0
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    import slime_amd
+    monkeypatch.setattr(slime_amd, "_lib", None)
+    monkeypatch.setattr(slime_amd, "LIB_PATH", str(tmp_path / "absent.so"))
+    with pytest.raises(RuntimeError):
+        slime_amd.lib()
