@@ -108,11 +108,22 @@ int64_t slime_scan_rect(int64_t seed, int32_t x0, int32_t z0, int64_t w, int64_t
                         int32_t min_size, int32_t max_size, ExtChunkResult* results_buffer,
                         int32_t max_results, int64_t* hist222);
 
+/* Extension with no counterpart in the reference: the same scan over the
+ * rows of [z0, z0+h) or, when band_rows > 0, only over the strided bands
+ * z0 + b*2^17 + s (s < band_rows), with a forced scan mode: 0 automatic,
+ * 1 plain per-chunk scan, 2 banded scan (returns -1 if the region is narrower
+ * than 2^18 columns). Lets tests and callers pick the kernel explicitly. */
+int64_t slime_scan_bands(int64_t seed, int32_t x0, int32_t z0, int64_t w, int64_t h,
+                         int32_t band_rows, int32_t mode, int32_t min_size, int32_t max_size,
+                         ExtChunkResult* results_buffer, int32_t max_results, int64_t* hist222);
+
 /* Extension with no counterpart in the reference: device time of the scan
  * kernel launches since the last reset, measured with HIP events on the
- * engine's stream (for benchmarks). reset != 0 zeroes the counters after the
- * read. */
-void slime_kernel_stats(int64_t* launches, int64_t* kernel_ns, int32_t reset);
+ * engine's stream, for one kernel kind (0 plain per-chunk scan, 1 banded
+ * scan, 2 both), with the number of candidate centers those launches decided.
+ * reset != 0 zeroes every counter after the read. */
+void slime_kernel_stats(int32_t kind, int64_t* launches, int64_t* kernel_ns, int64_t* centers,
+                        int32_t reset);
 
 /* Extension with no counterpart in the reference: compute units, peak shader
  * clock (kHz) and HBM size of the current device; 0 on success, -1 if no

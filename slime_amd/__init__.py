@@ -28,7 +28,7 @@ EXPORTS = (
     "get_cuda_device_count", "get_cuda_device_name", "cleanup_gpu_resources",
     "get_gpu_v1_shape", "get_gpu_v1_algorithm", "get_gpu_v1_rng", "set_gpu_v1_rng_override",
     "get_gpu_v34_shape", "get_gpu_v34_rng", "set_gpu_v34_rng_override", "slime_scan_rect",
-    "slime_kernel_stats", "slime_device_info",
+    "slime_kernel_stats", "slime_device_info", "slime_scan_bands",
 )
 
 
@@ -88,10 +88,13 @@ def lib():
     L.get_cuda_device_name.restype = i32
     L.set_gpu_v1_rng_override.argtypes = [i32]
     L.set_gpu_v34_rng_override.argtypes = [i32]
-    L.slime_kernel_stats.argtypes = [C.POINTER(i64), C.POINTER(i64), i32]
+    L.slime_kernel_stats.argtypes = [i32, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64), i32]
     L.slime_kernel_stats.restype = None
     L.slime_device_info.argtypes = [C.POINTER(i32), C.POINTER(i32), C.POINTER(i64)]
     L.slime_device_info.restype = i32
+    L.slime_scan_bands.argtypes = [i64, i32, i32, i64, i64, i32, i32, i32, i32, P, i32,
+                                   C.POINTER(C.c_int64)]
+    L.slime_scan_bands.restype = i64
     _lib = L
     return L
 
@@ -139,11 +142,28 @@ def device_info():
     return cu.value, clk.value, mem.value
 
 
-def kernel_stats(reset=False):
-    """(launches, kernel_ns) of the scan kernels since the last reset."""
-    n, ns = C.c_int64(), C.c_int64()
-    lib().slime_kernel_stats(C.byref(n), C.byref(ns), 1 if reset else 0)
-    return n.value, ns.value
+KERNEL_PLAIN, KERNEL_BANDED, KERNEL_ALL = 0, 1, 2
+
+
+def kernel_stats(kind=KERNEL_ALL, reset=False):
+    """(launches, kernel_ns, centers) of the scan kernels since the last reset."""
+    n, ns, c = C.c_int64(), C.c_int64(), C.c_int64()
+    lib().slime_kernel_stats(kind, C.byref(n), C.byref(ns), C.byref(c), 1 if reset else 0)
+    return n.value, ns.value, c.value
+
+
+MODE_AUTO, MODE_PLAIN, MODE_BANDED = 0, 1, 2
+
+
+def scan_bands(seed, x0, z0, w, h, min_size, max_size=221, band_rows=0, mode=MODE_AUTO,
+               max_results=65536):
+    """Scan of [x0,x0+w) x rows of [z0,z0+h) (or the strided bands z0+b*2^17+s,
+    s < band_rows) with a forced kernel: (total, rows, hist[222])."""
+    buf = (ExtChunkResult * max(1, max_results))()
+    hist = (C.c_int64 * 222)()
+    total = lib().slime_scan_bands(seed, x0, z0, w, h, band_rows, mode, min_size, max_size,
+                                   buf, max_results, hist)
+    return total, _rows(buf, total, max_results), list(hist)
 
 
 def scan_rect(seed, x0, z0, w, h, min_size, max_size=221, max_results=65536):

@@ -141,8 +141,30 @@ int64_t slime_scan_rect(int64_t seed, int32_t x0, int32_t z0, int64_t w, int64_t
                              max_results, hist222);
 }
 
-void slime_kernel_stats(int64_t* launches, int64_t* kernel_ns, int32_t reset) {
-    slime::kernel_stats(launches, kernel_ns, reset != 0);
+int64_t slime_scan_bands(int64_t seed, int32_t x0, int32_t z0, int64_t w, int64_t h,
+                         int32_t band_rows, int32_t mode, int32_t min_size, int32_t max_size,
+                         ExtChunkResult* results_buffer, int32_t max_results, int64_t* hist222) {
+    if (!results_buffer || max_results <= 0 || w <= 0 || h <= 0 || w > INT32_MAX ||
+        h > INT32_MAX || band_rows < 0 || mode < 0 || mode > 2 || min_size < 0 ||
+        min_size > 221 || max_size < min_size)
+        return -1;
+    slime::SearchSpec spec;
+    spec.seed = seed;
+    spec.x0 = x0;
+    spec.z0 = z0;
+    spec.w = w;
+    spec.h = h;
+    spec.min_size = min_size;
+    spec.max_size = max_size;
+    spec.band_rows = band_rows;
+    spec.mode = (slime::ScanMode)mode;
+    return slime::run_search(spec, reinterpret_cast<slime::ResultRow*>(results_buffer),
+                             max_results, hist222);
+}
+
+void slime_kernel_stats(int32_t kind, int64_t* launches, int64_t* kernel_ns, int64_t* centers,
+                        int32_t reset) {
+    slime::kernel_stats(kind, launches, kernel_ns, centers, reset != 0);
 }
 
 int32_t slime_device_info(int32_t* compute_units, int32_t* clock_khz, int64_t* hbm_bytes) {
@@ -199,7 +221,7 @@ bool is_slime_chunk_c(int64_t seed, int64_t cx, int64_t cz) {
 int32_t get_cuda_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) {
-        hipGetLastError();
+        (void)hipGetLastError();
         return 0;
     }
     return n;
@@ -210,7 +232,7 @@ int32_t get_cuda_device_name(char* buffer, int32_t buffer_size) {
     int dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
-        hipGetLastError();
+        (void)hipGetLastError();
         buffer[0] = '\0';
         return 0;
     }

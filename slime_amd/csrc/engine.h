@@ -23,6 +23,12 @@ static_assert(sizeof(ResultRow) == 24, "ABI row is 24 bytes");
 
 // One search: a square of radius rd_max around (center_x, center_z), or any
 // rectangle of centers, with the reference's per-center filters.
+enum class ScanMode : int32_t {
+    kAuto = 0,    // banded where the region is at least 2^18 columns wide, plain elsewhere
+    kPlain = 1,   // per-chunk scan everywhere
+    kBanded = 2,  // banded scan required (error if the region is too narrow)
+};
+
 struct SearchSpec {
     int64_t seed = 0;
     int32_t x0 = 0, z0 = 0;  // first center
@@ -30,6 +36,11 @@ struct SearchSpec {
     int32_t min_size = 0, max_size = 221;
     int64_t rd_min = 0;
     int32_t acx = 0, acz = 0;  // annulus center (search center)
+    // 0: the rows [z0, z0+h). Otherwise only the rows z0 + b*2^17 + s with
+    // s < band_rows (and b*2^17 + s < h): a strided set of bands, used to
+    // exercise the banded scan on small inputs.
+    int32_t band_rows = 0;
+    ScanMode mode = ScanMode::kAuto;
 };
 
 // Process-global control flags of the reference ABI (SlimeCore.cpp:16-27,
@@ -50,8 +61,12 @@ Control& control();
 int64_t run_search(const SearchSpec& spec, ResultRow* out, int32_t max_results,
                    int64_t* hist_out = nullptr);
 
-// Device time of the scan kernel launches (HIP events on the engine's stream).
-void kernel_stats(int64_t* launches, int64_t* kernel_ns, bool reset);
+// Bands per banded launch (1..16); a tuning knob.
+void set_max_bands(int nb);
+
+// Device time of the scan kernel launches (HIP events on the engine's stream)
+// of one kind (0 plain, 1 banded, 2 both) and the centers they decided.
+void kernel_stats(int kind, int64_t* launches, int64_t* kernel_ns, int64_t* centers, bool reset);
 
 // Releases every device buffer the engine holds.
 void release_engine();

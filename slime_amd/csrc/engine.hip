@@ -1,26 +1,33 @@
-// Host orchestration of one search: slabs along z, score histogram, emit
-// gate, overflow retry and exact top-K ranking.
+// Host orchestration of one search: a list of kernel launches (banded
+// first-draw slabs where the region is wide enough, plain slabs elsewhere),
+// the score histogram, the emit gate, overflow retry, the exact correction of
+// Java's rejected first draws, and the top-K ranking.
 //
 // Reference behaviour followed (SlimeCoreGPU.cu:3164-3915):
-//   - the region is scanned in z slabs; after each slab the hit counter and
-//     the buffer are read back; an overflowing slab is rerun (:3795-3805);
+//   - the region is scanned in slabs; after each launch the hit counter and
+//     buffer are read back; an overflowing launch is rerun (:3795-3805);
 //   - every in-range hit is counted, but only hits that can still reach the
 //     global top-K are materialised (emit gate, :3731-3740);
 //   - ranking: size desc, center_x asc, center_z asc (:213-220, 3860-3880);
-//   - pause is a sleep-poll between slabs, cancel ranks what was found
+//   - pause is a sleep-poll between launches, cancel ranks what was found
 //     (:3754-3757).
-// The gate here is exact and needs no completion pass: it is raised only to
-// a score s for which at least K hits with score >= s were already counted.
+// The gate here is exact and needs no completion pass: it is raised only to a
+// score s for which at least K hits of score >= s were counted by centers
+// whose count is already final.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <mutex>
 #include <thread>
+#include <unordered_set>
 
 #include "engine.h"
+#include "slime_math.h"
 
 namespace slime {
 
@@ -31,8 +38,20 @@ Control& control() {
 
 namespace {
 
-constexpr uint32_t kHitCap = 1u << 24;  // 16M hits (192 MB) per slab
-constexpr int64_t kTargetCentersPerLaunch = int64_t(1) << 37;
+constexpr uint32_t kHitCap = 1u << 24;                       // hits per launch (192 MB)
+constexpr int64_t kPlainCentersPerLaunch = int64_t(1) << 37;
+constexpr int64_t kBandCentersPerLaunch = int64_t(1) << 39;
+constexpr uint32_t kMismatchCap = 1u << 22;
+constexpr int kBandTileH = 2048;
+// Bands per banded launch (kernel instantiations: 1, 2, 4, ..., 16; up to 14
+// fit three waves per SIMD without spills); the
+// SLIME_MAX_BANDS environment variable overrides it, like the reference's
+// SLIME_GPU_V1_* path switches (SlimeCoreGPU.cu:3267-3354).
+int g_max_bands = [] {
+    const char* v = std::getenv("SLIME_MAX_BANDS");
+    const int n = v ? std::atoi(v) : 0;
+    return n >= 1 && n <= kMaxBands ? n : 12;
+}();
 
 struct Buffers {
     int device = -1;
@@ -43,22 +62,32 @@ struct Buffers {
     unsigned long long* h_counters = nullptr;  // pinned
     Hit* h_hits = nullptr;                     // pinned
     size_t h_hits_cap = 0;
+    uint64_t* d_table = nullptr;
+    int32_t table_n = 0;
+    MismatchChunk* d_mismatch = nullptr;
+    unsigned int* d_mismatch_count = nullptr;
+    CenterFix* d_fixes = nullptr;
+    size_t fixes_cap = 0;
 };
 
 Buffers g_buf;
 std::mutex g_mutex;
-std::atomic<int64_t> g_kernel_launches{0};
-std::atomic<int64_t> g_kernel_ns{0};
+// Per kernel kind (0 plain, 1 banded): launches, device ns, centers decided.
+std::atomic<int64_t> g_kstat[2][3];
 
 void free_buffers() {
-    if (g_buf.device >= 0) hipSetDevice(g_buf.device);
-    if (g_buf.d_hits) hipFree(g_buf.d_hits);
-    if (g_buf.d_counters) hipFree(g_buf.d_counters);
-    if (g_buf.h_counters) hipHostFree(g_buf.h_counters);
-    if (g_buf.h_hits) hipHostFree(g_buf.h_hits);
-    if (g_buf.ev_start) hipEventDestroy(g_buf.ev_start);
-    if (g_buf.ev_stop) hipEventDestroy(g_buf.ev_stop);
-    if (g_buf.stream) hipStreamDestroy(g_buf.stream);
+    if (g_buf.device >= 0) (void)hipSetDevice(g_buf.device);
+    if (g_buf.d_hits) (void)hipFree(g_buf.d_hits);
+    if (g_buf.d_counters) (void)hipFree(g_buf.d_counters);
+    if (g_buf.h_counters) (void)hipHostFree(g_buf.h_counters);
+    if (g_buf.h_hits) (void)hipHostFree(g_buf.h_hits);
+    if (g_buf.d_table) (void)hipFree(g_buf.d_table);
+    if (g_buf.d_mismatch) (void)hipFree(g_buf.d_mismatch);
+    if (g_buf.d_mismatch_count) (void)hipFree(g_buf.d_mismatch_count);
+    if (g_buf.d_fixes) (void)hipFree(g_buf.d_fixes);
+    if (g_buf.ev_start) (void)hipEventDestroy(g_buf.ev_start);
+    if (g_buf.ev_stop) (void)hipEventDestroy(g_buf.ev_stop);
+    if (g_buf.stream) (void)hipStreamDestroy(g_buf.stream);
     g_buf = Buffers{};
 }
 
@@ -68,12 +97,20 @@ bool ensure_buffers() {
     if (g_buf.device == dev && g_buf.d_hits) return true;
     free_buffers();
     g_buf.device = dev;
+    const std::vector<uint64_t>& table = mismatch_table();
+    g_buf.table_n = (int32_t)table.size();
     if (hipStreamCreateWithFlags(&g_buf.stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&g_buf.ev_start) != hipSuccess || hipEventCreate(&g_buf.ev_stop) != hipSuccess ||
+        hipEventCreate(&g_buf.ev_start) != hipSuccess ||
+        hipEventCreate(&g_buf.ev_stop) != hipSuccess ||
         hipMalloc(&g_buf.d_hits, sizeof(Hit) * (size_t)kHitCap) != hipSuccess ||
         hipMalloc(&g_buf.d_counters, sizeof(unsigned long long) * (1 + kHistBins)) != hipSuccess ||
         hipHostMalloc(&g_buf.h_counters, sizeof(unsigned long long) * (1 + kHistBins),
-                      hipHostMallocDefault) != hipSuccess) {
+                      hipHostMallocDefault) != hipSuccess ||
+        hipMalloc(&g_buf.d_table, sizeof(uint64_t) * table.size()) != hipSuccess ||
+        hipMemcpy(g_buf.d_table, table.data(), sizeof(uint64_t) * table.size(),
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc(&g_buf.d_mismatch, sizeof(MismatchChunk) * kMismatchCap) != hipSuccess ||
+        hipMalloc(&g_buf.d_mismatch_count, sizeof(unsigned int)) != hipSuccess) {
         free_buffers();
         return false;
     }
@@ -82,10 +119,10 @@ bool ensure_buffers() {
 
 bool ensure_host_hits(size_t n) {
     if (g_buf.h_hits_cap >= n) return true;
-    if (g_buf.h_hits) hipHostFree(g_buf.h_hits);
+    if (g_buf.h_hits) (void)hipHostFree(g_buf.h_hits);
     g_buf.h_hits = nullptr;
     g_buf.h_hits_cap = 0;
-    size_t want = std::max<size_t>(n, 1 << 16);
+    const size_t want = std::max<size_t>(n, 1 << 16);
     if (hipHostMalloc(&g_buf.h_hits, sizeof(Hit) * want, hipHostMallocDefault) != hipSuccess) {
         g_buf.h_hits = nullptr;
         return false;
@@ -94,43 +131,332 @@ bool ensure_host_hits(size_t n) {
     return true;
 }
 
+bool ensure_fixes(size_t n) {
+    if (g_buf.fixes_cap >= n) return true;
+    if (g_buf.d_fixes) (void)hipFree(g_buf.d_fixes);
+    g_buf.d_fixes = nullptr;
+    g_buf.fixes_cap = 0;
+    const size_t want = std::max<size_t>(n, 1 << 16);
+    if (hipMalloc(&g_buf.d_fixes, sizeof(CenterFix) * want) != hipSuccess) {
+        g_buf.d_fixes = nullptr;
+        return false;
+    }
+    g_buf.fixes_cap = want;
+    return true;
+}
+
+inline uint64_t center_key(int32_t cx, int32_t cz) {
+    return ((uint64_t)(uint32_t)cx << 32) | (uint32_t)cz;
+}
+
 // Largest score s with at least k counted hits of score >= s; `floor` if none.
 int32_t gate_for(const int64_t* hist, int64_t k, int32_t floor) {
     int64_t acc = 0;
     for (int32_t s = kHistBins - 1; s >= 0; --s) {
-        acc += hist[s];
+        acc += std::max<int64_t>(0, hist[s]);
         if (acc >= k) return std::max(s, floor);
     }
     return floor;
 }
 
-// Keeps only the k best candidates (unordered).
-void trim(std::vector<Hit>& cand, size_t k) {
-    if (cand.size() <= k) return;
-    std::nth_element(cand.begin(), cand.begin() + k, cand.end(), ranks_before);
-    cand.resize(k);
-}
+// Score bookkeeping shared by every launch of one search.
+struct Ranker {
+    int32_t min_size = 0, max_size = 221;
+    size_t keep = 0;
+    int64_t hist[kHistBins] = {0};
+    int64_t pending[kHistBins] = {0};  // first-draw scores of centers to be recounted
+    std::vector<Hit> cand;
+    std::unordered_set<uint64_t> recount;  // centers whose scan count is provisional
+    int32_t emit_min = 0;
 
-struct Geometry {
-    int32_t tiles_x;
-    int32_t tile_h;
-    int64_t slab_h;
+    int32_t gate(const unsigned long long* extra) const {
+        int64_t h[kHistBins];
+        for (int s = 0; s < kHistBins; ++s)
+            h[s] = hist[s] - pending[s] + (extra ? (int64_t)extra[s] : 0);
+        return gate_for(h, (int64_t)keep, min_size);
+    }
+    void add(const unsigned long long* slab_hist, const Hit* hits, size_t n) {
+        for (int s = 0; s < kHistBins; ++s) hist[s] += (int64_t)slab_hist[s];
+        for (size_t i = 0; i < n; ++i)
+            if (recount.empty() || !recount.count(center_key(hits[i].cx, hits[i].cz)))
+                cand.push_back(hits[i]);
+        if (cand.size() > std::max<size_t>(2 * keep, 1 << 20)) {
+            std::nth_element(cand.begin(), cand.begin() + keep, cand.end(), ranks_before);
+            cand.resize(keep);
+        }
+        emit_min = std::max(emit_min, gate(nullptr));
+    }
 };
 
-Geometry plan(int64_t w, int64_t h) {
-    Geometry g;
-    g.tiles_x = (int32_t)((w + kTileOut - 1) / kTileOut);
-    // Aim for >= 2048 workgroups per launch; tall tiles amortise the 16-row halo.
-    int64_t want_h = (h * g.tiles_x + 2047) / 2048;
-    int64_t th = std::min<int64_t>(2048, std::max<int64_t>(16, (want_h + 3) / 4 * 4));
-    g.tile_h = (int32_t)th;
-    int64_t slab = kTargetCentersPerLaunch / std::max<int64_t>(1, w);
+// One kernel launch of the plan.
+struct Work {
+    bool banded = false;
+    ScanParams plain{};
+    BandParams band{};
+    int nb = 0;
+    int64_t centers = 0;  // candidate centers this launch decides
+};
+
+enum class RunResult { kOk, kSplit, kError };
+
+RunResult run_work(Work& wk, Ranker& rk) {
+    for (;;) {
+        hipError_t launch_err;
+        if (hipMemsetAsync(g_buf.d_counters, 0, sizeof(unsigned long long) * (1 + kHistBins),
+                           g_buf.stream) != hipSuccess ||
+            hipEventRecord(g_buf.ev_start, g_buf.stream) != hipSuccess)
+            return RunResult::kError;
+        if (wk.banded) {
+            BandParams& p = wk.band;
+            p.emit_min = rk.emit_min;
+            p.hits = g_buf.d_hits;
+            p.hit_cap = kHitCap;
+            p.emitted = g_buf.d_counters;
+            p.hist = g_buf.d_counters + 1;
+            launch_err = launch_scan_band(p, wk.nb, g_buf.stream);
+        } else {
+            ScanParams& p = wk.plain;
+            p.emit_min = rk.emit_min;
+            p.hits = g_buf.d_hits;
+            p.hit_cap = kHitCap;
+            p.emitted = g_buf.d_counters;
+            p.hist = g_buf.d_counters + 1;
+            launch_err = launch_scan_plain(p, g_buf.stream);
+        }
+        if (launch_err != hipSuccess || hipEventRecord(g_buf.ev_stop, g_buf.stream) != hipSuccess ||
+            hipMemcpyAsync(g_buf.h_counters, g_buf.d_counters,
+                           sizeof(unsigned long long) * (1 + kHistBins), hipMemcpyDeviceToHost,
+                           g_buf.stream) != hipSuccess ||
+            hipStreamSynchronize(g_buf.stream) != hipSuccess)
+            return RunResult::kError;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, g_buf.ev_start, g_buf.ev_stop) == hipSuccess) {
+            std::atomic<int64_t>* st = g_kstat[wk.banded ? 1 : 0];
+            st[0].fetch_add(1, std::memory_order_relaxed);
+            st[1].fetch_add((int64_t)((double)ms * 1e6), std::memory_order_relaxed);
+            st[2].fetch_add(wk.centers, std::memory_order_relaxed);
+        }
+        const unsigned long long emitted = g_buf.h_counters[0];
+        const unsigned long long* slab_hist = g_buf.h_counters + 1;
+        if (emitted > kHitCap) {
+            const int32_t next = rk.gate(slab_hist);
+            if (next > rk.emit_min) {
+                rk.emit_min = next;
+                continue;
+            }
+            return RunResult::kSplit;
+        }
+        if (emitted > 0) {
+            if (!ensure_host_hits(emitted) ||
+                hipMemcpyAsync(g_buf.h_hits, g_buf.d_hits, sizeof(Hit) * emitted,
+                               hipMemcpyDeviceToHost, g_buf.stream) != hipSuccess ||
+                hipStreamSynchronize(g_buf.stream) != hipSuccess)
+                return RunResult::kError;
+        }
+        rk.add(slab_hist, g_buf.h_hits, emitted);
+        return RunResult::kOk;
+    }
+}
+
+// Halves a work along z; false if it cannot be split further.
+bool split_work(const Work& wk, Work& a, Work& b) {
+    a = b = wk;
+    if (wk.banded) {
+        const BandParams& p = wk.band;
+        const int32_t rows = p.s_end - p.s_begin;
+        if (rows <= p.tile_h) return false;
+        const int32_t half = (rows / 2 + p.tile_h - 1) / p.tile_h * p.tile_h;
+        a.band.s_end = p.s_begin + half;
+        b.band.s_begin = p.s_begin + half;
+        const int32_t tiles_x = p.tiles_x;
+        a.band.tiles_total = tiles_x * ((a.band.s_end - a.band.s_begin + p.tile_h - 1) / p.tile_h);
+        b.band.tiles_total = tiles_x * ((b.band.s_end - b.band.s_begin + p.tile_h - 1) / p.tile_h);
+        a.centers = wk.centers * half / rows;
+        b.centers = wk.centers - a.centers;
+        return true;
+    }
+    const ScanParams& p = wk.plain;
+    if (p.h <= 4) return false;
+    const int32_t half = (p.h / 2 + 3) / 4 * 4;
+    const int32_t th = std::min(p.tile_h, half);
+    a.plain.h = half;
+    a.plain.tile_h = th;
+    a.plain.tiles_total = p.tiles_x * ((half + th - 1) / th);
+    b.plain.z0 = (int32_t)((uint32_t)p.z0 + (uint32_t)half);
+    b.plain.h = p.h - half;
+    b.plain.tile_h = th;
+    b.plain.tiles_total = p.tiles_x * ((b.plain.h + th - 1) / th);
+    a.centers = (int64_t)p.w * half;
+    b.centers = (int64_t)p.w * b.plain.h;
+    return true;
+}
+
+void fill_filters(const SearchSpec& spec, int32_t& min_size, int32_t& max_size,
+                  int32_t& use_annulus, int64_t& rd_min_sq, int32_t& acx, int32_t& acz) {
+    min_size = spec.min_size;
+    max_size = spec.max_size;
+    use_annulus = spec.rd_min > 0;
+    rd_min_sq = spec.rd_min * spec.rd_min;
+    acx = spec.acx;
+    acz = spec.acz;
+}
+
+// Plain slabs over centers [x0, x0+w) x [z0, z0+h).
+void plan_plain(const SearchSpec& spec, int32_t x0, int64_t w, int32_t z0, int64_t h,
+                std::deque<Work>& out) {
+    const int32_t tiles_x = (int32_t)((w + kTileOut - 1) / kTileOut);
+    const int64_t want_h = (h * tiles_x + 2047) / 2048;  // >= 2048 workgroups per launch
+    const int32_t th = (int32_t)std::min<int64_t>(2048, std::max<int64_t>(16, (want_h + 3) / 4 * 4));
+    int64_t slab = kPlainCentersPerLaunch / std::max<int64_t>(1, w);
     slab = std::max<int64_t>(th, slab / th * th);
-    // keep the grid of one launch within int32 workgroups
-    int64_t max_tiles_z = (int64_t(1) << 30) / g.tiles_x;
-    slab = std::min<int64_t>(slab, std::max<int64_t>(1, max_tiles_z) * th);
-    g.slab_h = slab;
-    return g;
+    slab = std::min<int64_t>(slab, std::max<int64_t>(1, ((int64_t)1 << 30) / tiles_x) * th);
+    for (int64_t z_off = 0; z_off < h; z_off += slab) {
+        Work wk;
+        ScanParams& p = wk.plain;
+        const int64_t rows = std::min<int64_t>(slab, h - z_off);
+        p.seed = spec.seed;
+        p.x0 = x0;
+        p.z0 = (int32_t)((uint32_t)z0 + (uint32_t)z_off);
+        p.w = (int32_t)w;
+        p.h = (int32_t)rows;
+        p.tile_h = th;
+        p.tiles_x = tiles_x;
+        p.tiles_total = (int32_t)(tiles_x * ((rows + th - 1) / th));
+        fill_filters(spec, p.min_size, p.max_size, p.use_annulus, p.rd_min_sq, p.acx, p.acz);
+        wk.centers = rows * w;
+        out.push_back(wk);
+    }
+}
+
+// Plain slabs over every row range of the spec (all rows, or each band).
+void plan_plain_rows(const SearchSpec& spec, int32_t x0, int64_t w, std::deque<Work>& out) {
+    if (spec.band_rows <= 0) {
+        plan_plain(spec, x0, w, spec.z0, spec.h, out);
+        return;
+    }
+    for (int64_t b = 0; b * kPeriod < spec.h; ++b) {
+        const int64_t rows = std::min<int64_t>(spec.band_rows, spec.h - b * kPeriod);
+        plan_plain(spec, x0, w, (int32_t)((uint32_t)spec.z0 + (uint32_t)(b * kPeriod)), rows, out);
+    }
+}
+
+// Banded launches over every center of the spec; needs w >= 2P. Whole 2P
+// column blocks from x0 form the main part; a remainder becomes one block
+// aligned to the right edge whose tiles start at the first slot column that
+// reaches a new center, with the already covered centers masked out.
+void plan_banded(const SearchSpec& spec, std::deque<Work>& out) {
+    const int64_t h = spec.h, W = spec.w;
+    const int64_t nfull = W / (2 * kPeriod), rem = W % (2 * kPeriod);
+    struct XPart {
+        int32_t x0, w, ox_min, slot_lo, nblocks;
+        int64_t centers_per_row;
+    };
+    std::vector<XPart> parts;
+    parts.push_back(XPart{spec.x0, (int32_t)(nfull * 2 * kPeriod), 0, 0, (int32_t)nfull,
+                          nfull * 2 * kPeriod});
+    if (rem > 0)
+        parts.push_back(XPart{(int32_t)((uint32_t)spec.x0 + (uint32_t)(W - 2 * kPeriod)),
+                              2 * kPeriod, (int32_t)(2 * kPeriod - rem),
+                              (int32_t)std::max<int64_t>(0, kPeriod - rem), 1, rem});
+    const int64_t nbz = (h + kPeriod - 1) / kPeriod;
+    const int64_t groups = (nbz + g_max_bands - 1) / g_max_bands;
+    int64_t slot_rows64 = std::min<int64_t>(h, kPeriod);
+    if (spec.band_rows > 0) slot_rows64 = std::min<int64_t>(slot_rows64, spec.band_rows);
+    const int32_t slot_rows = (int32_t)slot_rows64;
+    const int32_t tile_h = std::min(kBandTileH, (slot_rows + 3) / 4 * 4);
+    int64_t b_first = 0;
+    for (int64_t gi = 0; gi < groups; ++gi) {
+        const int64_t nb = (nbz - b_first + (groups - gi) - 1) / (groups - gi);
+        for (const XPart& xp : parts) {
+            const int32_t tiles_per_block = (kPeriod - xp.slot_lo + kBandTileOut - 1) / kBandTileOut;
+            // slot rows per launch: about kBandCentersPerLaunch centers
+            int64_t rows = kBandCentersPerLaunch / std::max<int64_t>(1, nb * xp.centers_per_row);
+            rows = std::max<int64_t>(tile_h, rows / tile_h * tile_h);
+            for (int64_t s = 0; s < slot_rows; s += rows) {
+                Work wk;
+                wk.banded = true;
+                wk.nb = (int)nb;
+                BandParams& p = wk.band;
+                p.seed = spec.seed;
+                p.x0 = xp.x0;
+                p.w = xp.w;
+                p.ox_min = xp.ox_min;
+                p.slot_lo = xp.slot_lo;
+                p.tiles_per_block = tiles_per_block;
+                p.z0 = (int32_t)((uint32_t)spec.z0 + (uint32_t)(b_first * kPeriod));
+                p.h_valid = (int32_t)std::min<int64_t>(h - b_first * kPeriod, nb * kPeriod);
+                p.s_begin = (int32_t)s;
+                p.s_end = (int32_t)std::min<int64_t>(slot_rows, s + rows);
+                p.tile_h = tile_h;
+                p.tiles_x = xp.nblocks * tiles_per_block;
+                p.tiles_total = p.tiles_x * ((p.s_end - p.s_begin + tile_h - 1) / tile_h);
+                fill_filters(spec, p.min_size, p.max_size, p.use_annulus, p.rd_min_sq, p.acx,
+                             p.acz);
+                int64_t c = 0;
+                for (int64_t b = 0; b < nb; ++b) {
+                    const int64_t hi = std::min<int64_t>(p.s_end, p.h_valid - b * kPeriod);
+                    if (hi > p.s_begin) c += (hi - p.s_begin) * xp.centers_per_row;
+                }
+                wk.centers = c;
+                out.push_back(wk);
+            }
+        }
+        b_first += nb;
+    }
+}
+
+// Finds the first-draw mismatches of the banded part and recounts every
+// affected center exactly. Returns false on a device error.
+bool prepare_fixes(const SearchSpec& spec, int64_t wb, Ranker& rk, std::vector<CenterFix>& fixes) {
+    if (hipMemsetAsync(g_buf.d_mismatch_count, 0, sizeof(unsigned int), g_buf.stream) != hipSuccess ||
+        launch_find_mismatches(spec.seed, (int32_t)((int64_t)spec.x0 - 8), wb + 16,
+                               (int32_t)((int64_t)spec.z0 - 8), spec.h + 16, g_buf.d_table,
+                               g_buf.table_n, g_buf.d_mismatch, kMismatchCap,
+                               g_buf.d_mismatch_count, g_buf.stream) != hipSuccess)
+        return false;
+    unsigned int count = 0;
+    if (hipMemcpyAsync(&count, g_buf.d_mismatch_count, sizeof(count), hipMemcpyDeviceToHost,
+                       g_buf.stream) != hipSuccess ||
+        hipStreamSynchronize(g_buf.stream) != hipSuccess || count > kMismatchCap)
+        return false;
+    std::vector<MismatchChunk> mm(count);
+    if (count && (hipMemcpy(mm.data(), g_buf.d_mismatch, sizeof(MismatchChunk) * count,
+                            hipMemcpyDeviceToHost) != hipSuccess))
+        return false;
+    std::sort(mm.begin(), mm.end(), [](const MismatchChunk& a, const MismatchChunk& b) {
+        return a.z != b.z ? a.z < b.z : a.x < b.x;
+    });
+    const int64_t rmin_sq = spec.rd_min * spec.rd_min;
+    fixes.clear();
+    for (const MismatchChunk& m : mm) {
+        for (int dz = -8; dz <= 8; ++dz) {
+            const int hw = disk_half_width(dz < 0 ? -dz : dz);
+            for (int dx = -hw; dx <= hw; ++dx) {
+                const int64_t cx = (int64_t)m.x + dx, cz = (int64_t)m.z + dz;
+                const int64_t ox = cx - spec.x0, oz = cz - spec.z0;
+                if (ox < 0 || ox >= wb || oz < 0 || oz >= spec.h) continue;
+                if (spec.band_rows > 0 && oz % kPeriod >= spec.band_rows) continue;
+                if (spec.rd_min > 0) {
+                    const int64_t ax = cx - spec.acx, az = cz - spec.acz;
+                    if (ax * ax + az * az < rmin_sq) continue;
+                }
+                if (!rk.recount.insert(center_key((int32_t)cx, (int32_t)cz)).second) continue;
+                fixes.push_back(CenterFix{(int32_t)cx, (int32_t)cz, 0, 0});
+            }
+        }
+    }
+    if (fixes.empty()) return true;
+    if (!ensure_fixes(fixes.size()) ||
+        hipMemcpyAsync(g_buf.d_fixes, fixes.data(), sizeof(CenterFix) * fixes.size(),
+                       hipMemcpyHostToDevice, g_buf.stream) != hipSuccess ||
+        launch_recount(spec.seed, g_buf.d_fixes, (int32_t)fixes.size(), g_buf.stream) != hipSuccess ||
+        hipMemcpyAsync(fixes.data(), g_buf.d_fixes, sizeof(CenterFix) * fixes.size(),
+                       hipMemcpyDeviceToHost, g_buf.stream) != hipSuccess ||
+        hipStreamSynchronize(g_buf.stream) != hipSuccess)
+        return false;
+    for (const CenterFix& f : fixes)
+        if (f.first_draw >= spec.min_size && f.first_draw <= spec.max_size) rk.pending[f.first_draw]++;
+    return true;
 }
 
 void wait_while_paused() {
@@ -141,13 +467,20 @@ void wait_while_paused() {
 
 }  // namespace
 
-void kernel_stats(int64_t* launches, int64_t* kernel_ns, bool reset) {
-    if (launches) *launches = g_kernel_launches.load();
-    if (kernel_ns) *kernel_ns = g_kernel_ns.load();
-    if (reset) {
-        g_kernel_launches.store(0);
-        g_kernel_ns.store(0);
+void set_max_bands(int nb) { g_max_bands = std::max(1, std::min(kMaxBands, nb)); }
+
+void kernel_stats(int kind, int64_t* launches, int64_t* kernel_ns, int64_t* centers, bool reset) {
+    int64_t v[3] = {0, 0, 0};
+    for (int k = 0; k < 2; ++k) {
+        if (kind != 2 && kind != k) continue;
+        for (int i = 0; i < 3; ++i) v[i] += g_kstat[k][i].load();
     }
+    if (launches) *launches = v[0];
+    if (kernel_ns) *kernel_ns = v[1];
+    if (centers) *centers = v[2];
+    if (reset)
+        for (int k = 0; k < 2; ++k)
+            for (int i = 0; i < 3; ++i) g_kstat[k][i].store(0);
 }
 
 void release_engine() {
@@ -159,109 +492,77 @@ int64_t run_search(const SearchSpec& spec, ResultRow* out, int32_t max_results,
                    int64_t* hist_out) {
     std::lock_guard<std::mutex> lock(g_mutex);
     Control& ctl = control();
-    if (!ensure_buffers()) {
-        hipGetLastError();
+    auto fail = [&]() -> int64_t {
+        (void)hipGetLastError();
+        ctl.progress.store(0, std::memory_order_relaxed);
         return -1;
+    };
+    if (!ensure_buffers()) return fail();
+
+    Ranker rk;
+    rk.min_size = spec.min_size;
+    rk.max_size = spec.max_size;
+    rk.keep = (size_t)max_results;
+    rk.emit_min = spec.min_size;
+
+    // The banded scan needs pairs of columns 2^17 apart inside the region.
+    const bool banded = spec.w >= 2 * kPeriod && spec.mode != ScanMode::kPlain;
+    if (spec.mode == ScanMode::kBanded && !banded) return fail();
+    std::deque<Work> works;
+    std::vector<CenterFix> fixes;
+    if (banded) {
+        if (!prepare_fixes(spec, spec.w, rk, fixes)) return fail();
+        plan_banded(spec, works);
+    } else {
+        plan_plain_rows(spec, spec.x0, spec.w, works);
     }
-    const int64_t W = spec.w, H = spec.h;
-    const Geometry geo = plan(W, H);
-    int64_t hist[kHistBins] = {0};
-    std::vector<Hit> cand;
-    const size_t keep = (size_t)max_results;
-    int32_t emit_min = spec.min_size;
-    int64_t slab_h = geo.slab_h;
 
-    for (int64_t z_off = 0; z_off < H;) {
+    int64_t total_centers = 0, done_centers = 0;
+    for (const Work& w : works) total_centers += w.centers;
+    bool cancelled = false;
+    while (!works.empty()) {
         wait_while_paused();
-        if (ctl.cancel.load(std::memory_order_relaxed)) break;
-        const int64_t rows = std::min<int64_t>(slab_h, H - z_off);
-        ScanParams p{};
-        p.seed = spec.seed;
-        p.x0 = spec.x0;
-        p.z0 = (int32_t)((uint32_t)spec.z0 + (uint32_t)z_off);
-        p.w = (int32_t)W;
-        p.h = (int32_t)rows;
-        p.tile_h = geo.tile_h;
-        p.tiles_x = geo.tiles_x;
-        p.tiles_total = (int32_t)(geo.tiles_x * ((rows + geo.tile_h - 1) / geo.tile_h));
-        p.min_size = spec.min_size;
-        p.max_size = spec.max_size;
-        p.emit_min = emit_min;
-        p.use_annulus = spec.rd_min > 0;
-        p.rd_min_sq = spec.rd_min * spec.rd_min;
-        p.acx = spec.acx;
-        p.acz = spec.acz;
-        p.hits = g_buf.d_hits;
-        p.hit_cap = kHitCap;
-        p.emitted = g_buf.d_counters;
-        p.hist = g_buf.d_counters + 1;
-
+        if (ctl.cancel.load(std::memory_order_relaxed)) {
+            cancelled = true;
+            break;
+        }
+        Work wk = works.front();
+        works.pop_front();
         const auto t0 = std::chrono::steady_clock::now();
-        if (hipMemsetAsync(g_buf.d_counters, 0, sizeof(unsigned long long) * (1 + kHistBins),
-                           g_buf.stream) != hipSuccess ||
-            hipEventRecord(g_buf.ev_start, g_buf.stream) != hipSuccess ||
-            launch_scan_plain(p, g_buf.stream) != hipSuccess ||
-            hipEventRecord(g_buf.ev_stop, g_buf.stream) != hipSuccess ||
-            hipMemcpyAsync(g_buf.h_counters, g_buf.d_counters,
-                           sizeof(unsigned long long) * (1 + kHistBins), hipMemcpyDeviceToHost,
-                           g_buf.stream) != hipSuccess ||
-            hipStreamSynchronize(g_buf.stream) != hipSuccess) {
-            hipGetLastError();
-            ctl.progress.store(0, std::memory_order_relaxed);
-            return -1;
+        const RunResult r = run_work(wk, rk);
+        if (r == RunResult::kError) return fail();
+        if (r == RunResult::kSplit) {
+            Work a, b;
+            if (!split_work(wk, a, b)) return fail();
+            works.push_front(b);
+            works.push_front(a);
+            continue;
         }
         const auto t1 = std::chrono::steady_clock::now();
-        float kernel_ms = 0.f;
-        if (hipEventElapsedTime(&kernel_ms, g_buf.ev_start, g_buf.ev_stop) == hipSuccess) {
-            g_kernel_launches.fetch_add(1, std::memory_order_relaxed);
-            g_kernel_ns.fetch_add((int64_t)((double)kernel_ms * 1e6), std::memory_order_relaxed);
-        }
-        const unsigned long long emitted = g_buf.h_counters[0];
-        const unsigned long long* slab_hist = g_buf.h_counters + 1;
-
-        if (emitted > kHitCap) {
-            // Raise the gate using this slab's counts, or split the slab.
-            int64_t merged[kHistBins];
-            for (int s = 0; s < kHistBins; ++s) merged[s] = hist[s] + (int64_t)slab_hist[s];
-            const int32_t next = gate_for(merged, (int64_t)keep, spec.min_size);
-            if (next > emit_min) {
-                emit_min = next;
-                continue;
-            }
-            if (rows > geo.tile_h) {
-                slab_h = std::max<int64_t>(geo.tile_h, rows / 2 / geo.tile_h * geo.tile_h);
-                continue;
-            }
-            ctl.progress.store(0, std::memory_order_relaxed);
-            return -1;
-        }
-        for (int s = 0; s < kHistBins; ++s) hist[s] += (int64_t)slab_hist[s];
-        if (emitted > 0) {
-            if (!ensure_host_hits(emitted) ||
-                hipMemcpyAsync(g_buf.h_hits, g_buf.d_hits, sizeof(Hit) * emitted,
-                               hipMemcpyDeviceToHost, g_buf.stream) != hipSuccess ||
-                hipStreamSynchronize(g_buf.stream) != hipSuccess) {
-                hipGetLastError();
-                ctl.progress.store(0, std::memory_order_relaxed);
-                return -1;
-            }
-            cand.insert(cand.end(), g_buf.h_hits, g_buf.h_hits + emitted);
-            if (cand.size() > std::max<size_t>(2 * keep, 1 << 20)) trim(cand, keep);
-        }
-        emit_min = std::max(emit_min, gate_for(hist, (int64_t)keep, spec.min_size));
-        ctl.processed_centers.fetch_add(rows * W, std::memory_order_relaxed);
+        done_centers += wk.centers;
+        ctl.processed_centers.fetch_add(wk.centers, std::memory_order_relaxed);
         ctl.scan_work_ns.fetch_add(
             std::max<int64_t>(1, std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count()),
             std::memory_order_relaxed);
-        z_off += rows;
-        ctl.progress.store(1 + (int32_t)((z_off * 79) / std::max<int64_t>(1, H)),
+        ctl.progress.store(1 + (int32_t)((done_centers * 79) / std::max<int64_t>(1, total_centers)),
                            std::memory_order_relaxed);
+    }
+    // Replace the provisional first-draw counts of recounted centers.
+    if (!cancelled) {
+        for (const CenterFix& f : fixes) {
+            if (f.first_draw >= spec.min_size && f.first_draw <= spec.max_size) rk.hist[f.first_draw]--;
+            if (f.exact >= spec.min_size && f.exact <= spec.max_size) {
+                rk.hist[f.exact]++;
+                rk.cand.push_back(Hit{f.exact, f.cx, f.cz});
+            }
+        }
     }
 
     int64_t total = 0;
-    for (int s = 0; s < kHistBins; ++s) total += hist[s];
-    if (hist_out) std::memcpy(hist_out, hist, sizeof(hist));
-    const size_t n = std::min(cand.size(), keep);
+    for (int s = 0; s < kHistBins; ++s) total += rk.hist[s];
+    if (hist_out) std::memcpy(hist_out, rk.hist, sizeof(rk.hist));
+    std::vector<Hit>& cand = rk.cand;
+    const size_t n = std::min(cand.size(), rk.keep);
     if (n < cand.size())
         std::partial_sort(cand.begin(), cand.begin() + n, cand.end(), ranks_before);
     else

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace slime {
 
 // One emitted center: exact 221-cell count and chunk coordinates.
@@ -38,5 +40,68 @@ struct ScanParams {
 // Plain per-chunk scan: tile = 1008 centers x tile_h centers.
 constexpr int kTileOut = 1008;
 hipError_t launch_scan_plain(const ScanParams& p, hipStream_t stream);
+
+// ---- banded first-draw scan -------------------------------------------
+//
+// Chunks 2^17 apart along x or z share the low 16 bits of the LCG seed sum
+// and have opposite first-draw parity. One lane therefore evaluates a slot
+// (x, s) for 2 x-phases {x, x+P} and nb z-bands {z0 + b*P + s}: a shared
+// low-half computation and one divisibility test per band.
+constexpr int32_t kPeriod = 1 << 17;                    // P
+constexpr int kBandTileOut = 496;                       // centers per banded x-tile
+constexpr int32_t kTilesPerPhase = (kPeriod + kBandTileOut - 1) / kBandTileOut;  // 265
+constexpr int kMaxBands = 14;  // three waves per SIMD without spills
+
+struct BandParams {
+    int64_t seed;
+    int32_t x0;           // first center column of x-block 0, phase 0
+    int32_t w;            // centers at offsets >= w from x0 are outside
+    int32_t ox_min;       // centers at offsets < ox_min from x0 are outside
+    int32_t slot_lo;      // first slot column of the x-tiles (0, or the tail block's)
+    int32_t tiles_per_block;  // x-tiles per block: ceil((P - slot_lo) / 496)
+    int32_t z0;           // center row of band 0, slot 0
+    int32_t h_valid;      // center (b, s) is inside iff b*P + s < h_valid
+    int32_t s_begin;      // first slot row of this launch (multiple of tile_h)
+    int32_t s_end;        // one past the last slot row of this launch
+    int32_t tile_h;       // slot rows per tile, multiple of 4
+    int32_t tiles_x;      // x-blocks * tiles_per_block
+    int32_t tiles_total;
+    int32_t min_size, max_size, emit_min;
+    int32_t use_annulus;
+    int64_t rd_min_sq;
+    int32_t acx, acz;
+    Hit* hits;
+    uint32_t hit_cap;
+    unsigned long long* emitted;
+    unsigned long long* hist;
+};
+hipError_t launch_scan_band(const BandParams& p, int nb, hipStream_t stream);
+
+// ---- first-draw mismatches (Java's nextInt(10) rejection) ------------------
+//
+// The banded scan counts first draws (next(31) % 10 == 0). It differs from
+// Java only where the first draw is rejected (next(31) >= 2^31-8). The seed
+// sums s (mod 2^48) of those chunks form a fixed sorted table; per chunk row
+// the few candidates are inverted to their x and the affected centers are
+// recounted exactly.
+struct MismatchChunk {
+    int32_t x, z;
+};
+struct CenterFix {
+    int32_t cx, cz;
+    int32_t first_draw;  // 221-cell count from first draws (what the scan saw)
+    int32_t exact;       // Java's count
+};
+// Sorted table of scrambled-before seed sums s whose first-draw verdict is
+// wrong (209,712 entries). Built once on the host.
+const std::vector<uint64_t>& mismatch_table();
+// Finds every mismatch chunk with z in [z_first, z_first+rows) and x in
+// [x_first, x_first+cols) (64-bit extents: cols, rows <= 2^32).
+hipError_t launch_find_mismatches(int64_t seed, int32_t x_first, int64_t cols, int32_t z_first,
+                                  int64_t rows, const uint64_t* table, int32_t table_n,
+                                  MismatchChunk* out, uint32_t cap, unsigned int* count,
+                                  hipStream_t stream);
+// Fills first_draw and exact of n centers.
+hipError_t launch_recount(int64_t seed, CenterFix* fixes, int32_t n, hipStream_t stream);
 
 }  // namespace slime

@@ -1,0 +1,169 @@
+"""GPU parity of the banded first-draw scan and its rejection correction.
+
+The banded kernel shares one LCG low half across 2 x-phases and up to 16
+z-bands spaced 2^17 apart, so it only runs on regions at least 2^18 columns
+wide. The checks:
+  * strided band sets (a few rows of each band) against the CPU oracle run on
+    each band's rectangle: totals, whole score histograms and ranked rows;
+  * the banded scan against the plain per-chunk scan (an independent kernel
+    with inline Java redraws) on the same large regions: bit-identical
+    histograms over ~10^11 centers, the size-independent check;
+  * seeds built so that chunks whose first draw Java rejects lie inside the
+    scanned region, both rejected-then-hit and first-draw false positives.
+"""
+import ctypes as C
+import random
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+P = 1 << 17
+MASK48 = (1 << 48) - 1
+JAVA_MUL = 0x5DEECE66D
+SCRAMBLE = 0x5DEECE66D ^ 0x3AD8025F
+INT32_MAX = (1 << 31) - 1
+
+
+def s64(v):
+    v &= (1 << 64) - 1
+    return v - (1 << 64) if v >= (1 << 63) else v
+
+
+def x_term(x):
+    u = x & 0xFFFFFFFF
+    a = (u * u * 4987142) & 0xFFFFFFFF
+    b = (u * 5947611) & 0xFFFFFFFF
+    return (a - (1 << 32) if a >> 31 else a) + (b - (1 << 32) if b >> 31 else b)
+
+
+def z_term(z):
+    u = z & 0xFFFFFFFF
+    sq = (u * u) & 0xFFFFFFFF
+    lin = (u * 389711) & 0xFFFFFFFF
+    return (sq - (1 << 32) if sq >> 31 else sq) * 4392871 + (lin - (1 << 32) if lin >> 31 else lin)
+
+
+def java_verdict(state1):
+    """(first-draw verdict, Java verdict) of a first LCG output."""
+    bits = state1 >> 17
+    fd = bits % 10 == 0
+    st, val = state1, bits % 10
+    while bits - val + 9 > INT32_MAX:
+        st = (st * JAVA_MUL + 11) & MASK48
+        bits = st >> 17
+        val = bits % 10
+    return fd, val == 0
+
+
+def rejected_state(rng, want_java_hit):
+    """A first LCG output in Java's rejection range with the wanted outcome."""
+    while True:
+        state1 = (((INT32_MAX - 7) << 17) + rng.getrandbits(20)) & MASK48
+        fd, java = java_verdict(state1)
+        if fd != java and java == want_java_hit:
+            return state1
+
+
+def seed_with_state(state1, x, z, high16):
+    inv = pow(JAVA_MUL, -1, 1 << 48)
+    s = ((((state1 - 11) * inv) & MASK48) ^ SCRAMBLE)
+    return s64(((s - x_term(x) - z_term(z)) & MASK48) | (high16 << 48))
+
+
+def oracle_bands(oracle, seed, x0, z0, w, h, band_rows, thr, k):
+    """Oracle over the strided bands: (total, hist, top-k rows)."""
+    lib = oracle.oracle_lib()
+    total, hist, rows = 0, [0] * 222, []
+    b = 0
+    while b * P < h:
+        rows_b = min(band_rows, h - b * P)
+        buf = (oracle.ChunkRow * k)()
+        hh = (C.c_int64 * 222)()
+        t = lib.oracle_search_rect(seed, x0, z0 + b * P, w, rows_b, thr, 221, buf, k, hh)
+        assert t >= 0
+        total += t
+        hist = [a + c for a, c in zip(hist, hh)]
+        rows += oracle.rows_to_tuples(buf, min(t, k))
+        b += 1
+    rows.sort(key=lambda r: (-r[0], r[1], r[2]))
+    return total, hist, rows[:k]
+
+
+@pytest.mark.parametrize("case", range(5))
+def test_band_sets_vs_oracle(gpu, oracle, case):
+    """case 1: a remainder wider than P; case 4: a remainder of 700 columns
+    (the right-aligned tail block starts 700 slot columns before its end)."""
+    import slime_amd
+    rng = random.Random(77 + case)
+    seed = s64(rng.getrandbits(64))
+    nb = [1, 3, 5, 16, 2][case]
+    band_rows = [24, 12, 9, 6, 20][case]
+    w = [2 * P, 3 * P + 1000, 2 * P, 4 * P, 2 * P + 700][case]
+    x0 = rng.randint(-10 ** 8, 10 ** 8)
+    z0 = rng.randint(-10 ** 8, 10 ** 8)
+    h = (nb - 1) * P + band_rows
+    thr, k = 36, 500
+    print("case", case, seed, x0, z0, w, h, band_rows, flush=True)
+    total, rows, hist = slime_amd.scan_bands(seed, x0, z0, w, h, thr, 221, band_rows,
+                                             slime_amd.MODE_BANDED, k)
+    want_total, want_hist, want_rows = oracle_bands(oracle, seed, x0, z0, w, h, band_rows, thr, k)
+    assert total == want_total
+    assert hist == want_hist
+    assert rows == want_rows
+
+
+@pytest.mark.parametrize("radius,thr", [(131100, 55), (200000, 58), (400000, 57)])
+def test_banded_equals_plain_large(gpu, radius, thr):
+    """Same square through both kernels: identical totals, histograms, rows."""
+    import slime_amd
+    seed = s64(0x1234_5678_9ABC_DEF0 * (radius + 7))
+    x0 = z0 = -radius
+    w = h = 2 * radius + 1
+    print("square", radius, thr, flush=True)
+    b = slime_amd.scan_bands(seed, x0, z0, w, h, thr, 221, 0, slime_amd.MODE_BANDED, 2000)
+    p = slime_amd.scan_bands(seed, x0, z0, w, h, thr, 221, 0, slime_amd.MODE_PLAIN, 2000)
+    assert b[0] == p[0] and b[2] == p[2] and b[1] == p[1]
+    assert b[0] > 0
+
+
+@pytest.mark.parametrize("java_hit", [True, False])
+def test_rejected_chunks_inside_banded_region(gpu, oracle, java_hit):
+    """Several Java-rejected first draws planted in scanned bands."""
+    import slime_amd
+    rng = random.Random(4242 if java_hit else 2424)
+    x0, z0, w, band_rows, nb = -5 * P - 333, 7 * P + 11, 2 * P, 40, 3
+    h = (nb - 1) * P + band_rows
+    for rep in range(3):
+        state1 = rejected_state(rng, java_hit)
+        b = rng.randrange(nb)
+        x = x0 + rng.randrange(w)
+        z = z0 + b * P + rng.randrange(band_rows)
+        seed = seed_with_state(state1, x, z, rng.getrandbits(16))
+        lib = oracle.oracle_lib()
+        assert lib.oracle_first_draw(seed, x, z) != lib.oracle_is_slime(seed, x, z)
+        thr = max(0, lib.oracle_disk_count(seed, x, z) - 3)
+        print("planted", seed, x, z, thr, flush=True)
+        total, rows, hist = slime_amd.scan_bands(seed, x0, z0, w, h, thr, 221, band_rows,
+                                                 slime_amd.MODE_BANDED, 300)
+        want_total, want_hist, want_rows = oracle_bands(oracle, seed, x0, z0, w, h, band_rows,
+                                                        thr, 300)
+        assert total == want_total
+        assert hist == want_hist
+        assert rows == want_rows
+
+
+def test_full_width_search_hits_recount(gpu, oracle):
+    """A wide search (banded + plain remainder): every reported hit recounts
+    exactly on the CPU and the total equals the plain kernel's."""
+    import slime_amd
+    seed, rd, thr = -7995527694508729151, 300000, 56
+    total, rows = slime_amd.search_slime_clusters_centered(seed, rd, thr, 221, 0, 1000, -2000, 100)
+    assert total > 0
+    lib = oracle.oracle_lib()
+    for r in rows:
+        cx, cz = (r[1] - 8) // 16, (r[2] - 8) // 16
+        assert lib.oracle_disk_count(seed, cx, cz) == r[0]
+    p = slime_amd.scan_bands(seed, 1000 - rd, -2000 - rd, 2 * rd + 1, 2 * rd + 1, thr, 221, 0,
+                             slime_amd.MODE_PLAIN, 100)
+    assert p[0] == total and p[1] == rows
