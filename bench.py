@@ -34,6 +34,10 @@ TOP_K = 65536
 REFERENCE_TPS = 1.409  # published: 14.06 T centers in 9.979 s (README.md:15,19)
 OPS_PER_CENTER = 6      # algorithmic INT32 ops per candidate center
 CPU_SAMPLE_RADIUS = 131072
+# HBM bytes per scan_band_kernel launch (rocprofv3 FETCH_SIZE + WRITE_SIZE of
+# the full-world run, scaled by the same counters on a copy of known size;
+# profiles/scan_band_kernel_full_world_thr60.md): 165.4 KiB x 2.0 + 0.03 KiB x 1.0.
+TRAFFIC_BYTES = 338800
 
 
 def hip_sync():
@@ -41,6 +45,21 @@ def hip_sync():
         C.CDLL("libamdhip64.so").hipDeviceSynchronize()
     except OSError:
         pass
+
+
+def copy_calibration(nbytes):
+    """One device-to-device copy of nbytes: a kernel of known HBM traffic
+    (read nbytes + write nbytes) that scales rocprofv3's FETCH_SIZE/WRITE_SIZE."""
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise SystemExit("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
 
 
 def cpu_baseline(gpu_lib):
@@ -83,6 +102,10 @@ def main():
     ap.add_argument("--radius", type=int, default=FULL_WORLD_RADIUS,
                     help="search radius in chunks (default: the full world)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="also run one device-to-device copy of BYTES (profiler scale)")
+    ap.add_argument("--traffic-kib", type=float, default=None,
+                    help="HBM KiB per launch of the dominant kernel measured by rocprofv3")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -96,6 +119,8 @@ def main():
         return 1
     rd = args.radius
     centers = (2 * rd + 1) ** 2
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
 
     def step():
         return slime_amd.search_slime_clusters_centered(FULL_WORLD_SEED, rd, THRESHOLD, 221, 0,
@@ -140,7 +165,9 @@ def main():
     achieved = d["centers"] * OPS_PER_CENTER / d["kernel_s"] / 1e12
     roofline = {
         "bound": "valu", "unit": "TOP/s", "achieved": achieved, "peak": peak,
-        "frac": achieved / peak, "traffic": None, "kernel": dominant,
+        "frac": achieved / peak,
+        "traffic": args.traffic_kib * 1024 if args.traffic_kib is not None else TRAFFIC_BYTES,
+        "kernel": dominant,
         "ops_per_center": OPS_PER_CENTER, "avg_launch_ms": d["avg_launch_ms"],
         "centers_per_launch": d["centers_per_launch"],
         "peak_basis": f"{cus} CUs x 64 lanes x {clk_khz / 1e6:.2f} GHz (INT32 VALU)",

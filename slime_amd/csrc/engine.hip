@@ -24,7 +24,6 @@
 #include <deque>
 #include <mutex>
 #include <thread>
-#include <unordered_set>
 
 #include "engine.h"
 #include "slime_math.h"
@@ -145,10 +144,6 @@ bool ensure_fixes(size_t n) {
     return true;
 }
 
-inline uint64_t center_key(int32_t cx, int32_t cz) {
-    return ((uint64_t)(uint32_t)cx << 32) | (uint32_t)cz;
-}
-
 // Largest score s with at least k counted hits of score >= s; `floor` if none.
 int32_t gate_for(const int64_t* hist, int64_t k, int32_t floor) {
     int64_t acc = 0;
@@ -159,6 +154,23 @@ int32_t gate_for(const int64_t* hist, int64_t k, int32_t floor) {
     return floor;
 }
 
+// The first-draw mismatch chunks of one search, sorted by (z, x).
+struct MismatchIndex {
+    std::vector<MismatchChunk> mm;
+
+    static bool in_disk(int64_t dx, int64_t dz) { return dx * dx + dz * dz <= 68; }
+
+    // Does the disk of center (cx, cz) contain a mismatch chunk?
+    bool affects(int32_t cx, int32_t cz) const {
+        if (mm.empty()) return false;
+        auto it = std::lower_bound(mm.begin(), mm.end(), (int64_t)cz - 8,
+                                   [](const MismatchChunk& m, int64_t z) { return m.z < z; });
+        for (; it != mm.end() && (int64_t)it->z <= (int64_t)cz + 8; ++it)
+            if (in_disk((int64_t)it->x - cx, (int64_t)it->z - cz)) return true;
+        return false;
+    }
+};
+
 // Score bookkeeping shared by every launch of one search.
 struct Ranker {
     int32_t min_size = 0, max_size = 221;
@@ -166,7 +178,7 @@ struct Ranker {
     int64_t hist[kHistBins] = {0};
     int64_t pending[kHistBins] = {0};  // first-draw scores of centers to be recounted
     std::vector<Hit> cand;
-    std::unordered_set<uint64_t> recount;  // centers whose scan count is provisional
+    const MismatchIndex* recount = nullptr;  // centers whose scan count is provisional
     int32_t emit_min = 0;
 
     int32_t gate(const unsigned long long* extra) const {
@@ -178,8 +190,7 @@ struct Ranker {
     void add(const unsigned long long* slab_hist, const Hit* hits, size_t n) {
         for (int s = 0; s < kHistBins; ++s) hist[s] += (int64_t)slab_hist[s];
         for (size_t i = 0; i < n; ++i)
-            if (recount.empty() || !recount.count(center_key(hits[i].cx, hits[i].cz)))
-                cand.push_back(hits[i]);
+            if (!recount || !recount->affects(hits[i].cx, hits[i].cz)) cand.push_back(hits[i]);
         if (cand.size() > std::max<size_t>(2 * keep, 1 << 20)) {
             std::nth_element(cand.begin(), cand.begin() + keep, cand.end(), ranks_before);
             cand.resize(keep);
@@ -405,11 +416,12 @@ void plan_banded(const SearchSpec& spec, std::deque<Work>& out) {
     }
 }
 
-// Finds the first-draw mismatches of the banded part and recounts every
-// affected center exactly. Returns false on a device error.
-bool prepare_fixes(const SearchSpec& spec, int64_t wb, Ranker& rk, std::vector<CenterFix>& fixes) {
+// Finds the first-draw mismatches of the region and recounts every affected
+// center exactly. Returns false on a device error.
+bool prepare_fixes(const SearchSpec& spec, MismatchIndex& index, Ranker& rk,
+                   std::vector<CenterFix>& fixes) {
     if (hipMemsetAsync(g_buf.d_mismatch_count, 0, sizeof(unsigned int), g_buf.stream) != hipSuccess ||
-        launch_find_mismatches(spec.seed, (int32_t)((int64_t)spec.x0 - 8), wb + 16,
+        launch_find_mismatches(spec.seed, (int32_t)((int64_t)spec.x0 - 8), spec.w + 16,
                                (int32_t)((int64_t)spec.z0 - 8), spec.h + 16, g_buf.d_table,
                                g_buf.table_n, g_buf.d_mismatch, kMismatchCap,
                                g_buf.d_mismatch_count, g_buf.stream) != hipSuccess)
@@ -419,7 +431,8 @@ bool prepare_fixes(const SearchSpec& spec, int64_t wb, Ranker& rk, std::vector<C
                        g_buf.stream) != hipSuccess ||
         hipStreamSynchronize(g_buf.stream) != hipSuccess || count > kMismatchCap)
         return false;
-    std::vector<MismatchChunk> mm(count);
+    std::vector<MismatchChunk>& mm = index.mm;
+    mm.resize(count);
     if (count && (hipMemcpy(mm.data(), g_buf.d_mismatch, sizeof(MismatchChunk) * count,
                             hipMemcpyDeviceToHost) != hipSuccess))
         return false;
@@ -428,23 +441,30 @@ bool prepare_fixes(const SearchSpec& spec, int64_t wb, Ranker& rk, std::vector<C
     });
     const int64_t rmin_sq = spec.rd_min * spec.rd_min;
     fixes.clear();
-    for (const MismatchChunk& m : mm) {
+    for (size_t i = 0; i < mm.size(); ++i) {
+        const MismatchChunk& m = mm[i];
+        // earlier mismatches close enough to share a center already added it
+        size_t first_near = i;
+        while (first_near > 0 && (int64_t)mm[first_near - 1].z >= (int64_t)m.z - 16) --first_near;
         for (int dz = -8; dz <= 8; ++dz) {
             const int hw = disk_half_width(dz < 0 ? -dz : dz);
             for (int dx = -hw; dx <= hw; ++dx) {
                 const int64_t cx = (int64_t)m.x + dx, cz = (int64_t)m.z + dz;
                 const int64_t ox = cx - spec.x0, oz = cz - spec.z0;
-                if (ox < 0 || ox >= wb || oz < 0 || oz >= spec.h) continue;
+                if (ox < 0 || ox >= spec.w || oz < 0 || oz >= spec.h) continue;
                 if (spec.band_rows > 0 && oz % kPeriod >= spec.band_rows) continue;
                 if (spec.rd_min > 0) {
                     const int64_t ax = cx - spec.acx, az = cz - spec.acz;
                     if (ax * ax + az * az < rmin_sq) continue;
                 }
-                if (!rk.recount.insert(center_key((int32_t)cx, (int32_t)cz)).second) continue;
-                fixes.push_back(CenterFix{(int32_t)cx, (int32_t)cz, 0, 0});
+                bool seen = false;
+                for (size_t j = first_near; j < i && !seen; ++j)
+                    seen = MismatchIndex::in_disk((int64_t)mm[j].x - cx, (int64_t)mm[j].z - cz);
+                if (!seen) fixes.push_back(CenterFix{(int32_t)cx, (int32_t)cz, 0, 0});
             }
         }
     }
+    rk.recount = &index;
     if (fixes.empty()) return true;
     if (!ensure_fixes(fixes.size()) ||
         hipMemcpyAsync(g_buf.d_fixes, fixes.data(), sizeof(CenterFix) * fixes.size(),
@@ -510,8 +530,9 @@ int64_t run_search(const SearchSpec& spec, ResultRow* out, int32_t max_results,
     if (spec.mode == ScanMode::kBanded && !banded) return fail();
     std::deque<Work> works;
     std::vector<CenterFix> fixes;
+    MismatchIndex mismatches;
     if (banded) {
-        if (!prepare_fixes(spec, spec.w, rk, fixes)) return fail();
+        if (!prepare_fixes(spec, mismatches, rk, fixes)) return fail();
         plan_banded(spec, works);
     } else {
         plan_plain_rows(spec, spec.x0, spec.w, works);
