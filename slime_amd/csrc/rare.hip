@@ -16,6 +16,7 @@
 //     int32 x, found by lifting one bit at a time, and verified in 48 bits.
 #include <algorithm>
 
+#include "mismatch_table.h"
 #include "scan.h"
 #include "slime_math.h"
 
@@ -23,76 +24,22 @@ namespace slime {
 
 namespace {
 
-// Inverse of the Java LCG multiplier mod 2^64 (Newton iteration, odd input).
-constexpr uint64_t inverse_mod_2_64(uint64_t a) {
-    uint64_t x = a;  // correct to 3 bits for odd a
-    for (int i = 0; i < 6; ++i) x *= 2 - a * x;
-    return x;
-}
-constexpr uint64_t kJavaMulInv = inverse_mod_2_64(kJavaMul);
-static_assert(((kJavaMul * kJavaMulInv) & kMask48) == 1, "LCG inverse");
-
-std::vector<uint64_t> build_table() {
-    std::vector<uint64_t> t;
-    const uint64_t first = ((uint64_t)(0x7FFFFFFFu - 7u)) << 17;  // next(31) >= 2^31-8
-    for (uint64_t st1 = first; st1 <= kMask48; ++st1) {
-        uint32_t bits = (uint32_t)(st1 >> 17);
-        bool first_draw = bits % 10u == 0u;
-        uint64_t st = st1;
-        uint32_t val = bits % 10u;
-        while (bits - val + 9u > 0x7FFFFFFFu) {
-            st = (st * kJavaMul + 11u) & kMask48;
-            bits = (uint32_t)(st >> 17);
-            val = bits % 10u;
-        }
-        if (first_draw == (val == 0u)) continue;
-        const uint64_t st0 = ((st1 - 11u) * kJavaMulInv) & kMask48;
-        t.push_back((st0 ^ kScramble) & kMask48);
-    }
-    std::sort(t.begin(), t.end());
-    return t;
-}
-
-// The x (mod 2^32) with x*x*4987142 + x*5947611 == v (mod 2^32).
-__device__ __forceinline__ uint32_t invert_x_low32(uint32_t v) {
-    uint32_t x = 0;
-#pragma unroll
-    for (int k = 0; k < 32; ++k) {
-        const uint32_t f = x * x * 4987142u + x * 5947611u;
-        x |= ((f ^ v) >> k & 1u) << k;
-    }
-    return x;
-}
-
-__device__ __forceinline__ int32_t lower_bound(const uint64_t* t, int32_t n, uint64_t key) {
-    int32_t lo = 0, hi = n;
-    while (lo < hi) {
-        const int32_t mid = (lo + hi) >> 1;
-        if (t[mid] < key) lo = mid + 1; else hi = mid;
-    }
-    return lo;
-}
-
 __global__ __launch_bounds__(256) void find_mismatches_kernel(
     int64_t seed, int32_t x_first, int64_t cols, int32_t z_first, int64_t rows,
     const uint64_t* table, int32_t n, MismatchChunk* out, uint32_t cap, unsigned int* count) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < rows; i += stride) {
         const int32_t z = (int32_t)((uint32_t)z_first + (uint32_t)i);
-        const uint64_t base = ((uint64_t)seed + z_term(z)) & kMask48;
-        const uint64_t lo = (base - (1ull << 32)) & kMask48;  // xt >= -2^32
-        int32_t k = lower_bound(table, n, lo);
-        for (int32_t c = 0; c < n; ++c, ++k) {
-            if (k >= n) k -= n;  // the window may wrap past 2^48
-            const uint64_t off = (table[k] - lo) & kMask48;
-            if (off >= (1ull << 33)) break;
-            const int64_t d = (int64_t)off - (1ll << 32);  // required xt(x)
-            const int32_t x = (int32_t)invert_x_low32((uint32_t)(uint64_t)d);
-            const int64_t dx = (int64_t)x - (int64_t)x_first;
-            if (dx < 0 || dx >= cols) continue;
-            if ((int64_t)x_term(x) != d) continue;
+        int32_t xs[8];
+        const int found = row_table_hits(seed, z, table, n, x_first, cols, xs, 8);
+        if (found == 0) continue;
+        if (found > 8) {  // never seen (~6 candidates per row, far fewer in range)
+            atomicAdd(count, (unsigned int)(cap + 1));
+            continue;
+        }
+        for (int f = 0; f < found; ++f) {
             const unsigned int slot = atomicAdd(count, 1u);
-            if (slot < cap) out[slot] = MismatchChunk{x, z};
+            if (slot < cap) out[slot] = MismatchChunk{xs[f], z};
         }
     }
 }
@@ -121,7 +68,7 @@ __global__ __launch_bounds__(256) void recount_kernel(int64_t seed, CenterFix* f
 }  // namespace
 
 const std::vector<uint64_t>& mismatch_table() {
-    static const std::vector<uint64_t> table = build_table();
+    static const std::vector<uint64_t> table = build_mismatch_table();
     return table;
 }
 

@@ -43,11 +43,6 @@ constexpr int kPlaneDw = kRowBits / 32;            // 16 dwords per bitplane row
 constexpr int kRing = 24;                          // 20 live rows + 4 being written
 constexpr int kThreads = 256;
 constexpr int kGroupCols = kBandTileOut / 4;       // 124 groups per tile row
-constexpr uint32_t kLowScramble = 0xE4320000u;     // (0x5E434E432 & 0xFFFF) << 16
-constexpr uint32_t kHighScramble = 0x5E434u;       // 0x5E434E432 >> 16
-constexpr uint32_t kElevenHi = 11u << 16;
-constexpr uint32_t kMulInv5 = kJavaMulLo * kInv5;  // M * 5^-1 (mod 2^32)
-constexpr uint32_t kFifthLimit = 858993459u;       // (2^32 - 1) / 5
 
 static_assert(kBandTileOut + 16 == kRowBits, "tile row = centers + halo");
 
@@ -228,23 +223,13 @@ __global__ __launch_bounds__(kThreads, 3) void scan_band_kernel(BandParams p) {
             for (int q = 0; q < L::kPlanes; ++q) acc[q] = 0;
             static_for<0, kChunksPerLane>([&](auto jc) {
                 constexpr int j = kChunksPerLane - 1 - decltype(jc)::value;  // high bits first
-                const uint64_t lsum = (uint64_t)xlo[j] + zlo;
-                const uint32_t carry = (uint32_t)(lsum >> 32);
-                const uint32_t u = (uint32_t)lsum ^ kLowScramble;
-                const uint32_t b0 = xh0[j] + carry, b1 = xh1[j] + carry;
-                const uint32_t w = (uint32_t)(((uint64_t)u * kJavaMulLo + kElevenHi) >> 32);
-                const uint32_t t = b0 + zhi[0] + w;
-                const bool e = (t & 2u) != 0;
-                const uint32_t xa = e ? b1 : b0;  // phase that is even in even bands
-                const uint32_t xb = e ? b0 : b1;
-                const uint32_t bias = u + (w - (t & 1u)) * kInv5;
-                acc[0] = acc[0] + acc[0] + (e ? 1u : 0u);
-                // all products first, so the multiplies of independent bands overlap
+                const BandSlot sl = band_slot(xlo[j], xh0[j], xh1[j], zlo, zhi[0]);
+                acc[0] = acc[0] + acc[0] + (sl.e ? 1u : 0u);
+                // all values first, so the multiplies of independent bands overlap
                 uint32_t v[NB];
                 static_for<0, NB>([&](auto bc) {
                     constexpr int b = decltype(bc)::value;
-                    const uint32_t a = (((b & 1) ? xb : xa) + zhi[b]) ^ kHighScramble;
-                    v[b] = a * kMulInv5 + bias;
+                    v[b] = band_value(sl, b, zhi[b]);
                 });
                 // shift in the complement of each hit: [a*K + bias > limit]
                 static_for<0, NB>([&](auto bc) {
