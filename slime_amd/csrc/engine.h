@@ -24,9 +24,9 @@ static_assert(sizeof(ResultRow) == 24, "ABI row is 24 bytes");
 // One search: a square of radius rd_max around (center_x, center_z), or any
 // rectangle of centers, with the reference's per-center filters.
 enum class ScanMode : int32_t {
-    kAuto = 0,    // banded where the region is at least 2^18 columns wide, plain elsewhere
-    kPlain = 1,   // per-chunk scan everywhere
-    kBanded = 2,  // banded scan required (error if the region is too narrow)
+    kAuto = 0,    // banded scan (two x-phases above 2^17 columns, one below)
+    kPlain = 1,   // independent per-chunk scan everywhere
+    kBanded = 2,  // banded scan (same as kAuto)
 };
 
 struct SearchSpec {

@@ -42,6 +42,7 @@ constexpr int64_t kPlainCentersPerLaunch = int64_t(1) << 37;
 constexpr int64_t kBandCentersPerLaunch = int64_t(1) << 39;
 constexpr uint32_t kMismatchCap = 1u << 22;
 constexpr int kBandTileH = 2048;
+constexpr int kMinBandTileH = 64;
 // Bands per banded launch (kernel instantiations: 1, 2, 4, ..., 16; up to 14
 // fit three waves per SIMD without spills); the
 // SLIME_MAX_BANDS environment variable overrides it, like the reference's
@@ -54,6 +55,7 @@ int g_max_bands = [] {
 
 struct Buffers {
     int device = -1;
+    int cus = 256;  // compute units of the device
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     Hit* d_hits = nullptr;
@@ -96,6 +98,10 @@ bool ensure_buffers() {
     if (g_buf.device == dev && g_buf.d_hits) return true;
     free_buffers();
     g_buf.device = dev;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        cus > 0)
+        g_buf.cus = cus;
     const std::vector<uint64_t>& table = mismatch_table();
     g_buf.table_n = (int32_t)table.size();
     if (hipStreamCreateWithFlags(&g_buf.stream, hipStreamNonBlocking) != hipSuccess ||
@@ -204,7 +210,7 @@ struct Work {
     bool banded = false;
     ScanParams plain{};
     BandParams band{};
-    int nb = 0;
+    int nb = 0, ph = 2;
     int64_t centers = 0;  // candidate centers this launch decides
 };
 
@@ -224,7 +230,7 @@ RunResult run_work(Work& wk, Ranker& rk) {
             p.hit_cap = kHitCap;
             p.emitted = g_buf.d_counters;
             p.hist = g_buf.d_counters + 1;
-            launch_err = launch_scan_band(p, wk.nb, g_buf.stream);
+            launch_err = launch_scan_band(p, wk.nb, wk.ph, g_buf.stream);
         } else {
             ScanParams& p = wk.plain;
             p.emit_min = rk.emit_min;
@@ -351,35 +357,60 @@ void plan_plain_rows(const SearchSpec& spec, int32_t x0, int64_t w, std::deque<W
     }
 }
 
-// Banded launches over every center of the spec; needs w >= 2P. Whole 2P
+// Banded launches over every center of the spec.
+//
+// Along x: a region wider than P pairs columns P apart (two phases): whole 2P
 // column blocks from x0 form the main part; a remainder becomes one block
 // aligned to the right edge whose tiles start at the first slot column that
-// reaches a new center, with the already covered centers masked out.
+// reaches a new center, with the already covered centers masked out; a region
+// between P and 2P wide is one block whose phase-1 columns past the edge are
+// masked out. A region at most P wide is scanned with one phase.
+// Along z: rows P apart form bands, grouped by at most g_max_bands per launch.
+// Tile height: the largest (up to kBandTileH) that still gives every CU a few
+// tiles of the plan; small regions trade halo rows for occupancy.
 void plan_banded(const SearchSpec& spec, std::deque<Work>& out) {
     const int64_t h = spec.h, W = spec.w;
-    const int64_t nfull = W / (2 * kPeriod), rem = W % (2 * kPeriod);
     struct XPart {
-        int32_t x0, w, ox_min, slot_lo, nblocks;
+        int32_t x0, w, ox_min, slot_lo, nblocks, phase_cols, ph;
         int64_t centers_per_row;
     };
     std::vector<XPart> parts;
-    parts.push_back(XPart{spec.x0, (int32_t)(nfull * 2 * kPeriod), 0, 0, (int32_t)nfull,
-                          nfull * 2 * kPeriod});
-    if (rem > 0)
-        parts.push_back(XPart{(int32_t)((uint32_t)spec.x0 + (uint32_t)(W - 2 * kPeriod)),
-                              2 * kPeriod, (int32_t)(2 * kPeriod - rem),
-                              (int32_t)std::max<int64_t>(0, kPeriod - rem), 1, rem});
+    if (W > kPeriod) {
+        const int64_t nfull = W / (2 * kPeriod), rem = W % (2 * kPeriod);
+        if (nfull == 0) {
+            parts.push_back(XPart{spec.x0, (int32_t)W, 0, 0, 1, kPeriod, 2, W});
+        } else {
+            parts.push_back(XPart{spec.x0, (int32_t)(nfull * 2 * kPeriod), 0, 0, (int32_t)nfull,
+                                  kPeriod, 2, nfull * 2 * kPeriod});
+            if (rem > 0)
+                parts.push_back(XPart{(int32_t)((uint32_t)spec.x0 + (uint32_t)(W - 2 * kPeriod)),
+                                      2 * kPeriod, (int32_t)(2 * kPeriod - rem),
+                                      (int32_t)std::max<int64_t>(0, kPeriod - rem), 1, kPeriod, 2,
+                                      rem});
+        }
+    } else {
+        parts.push_back(XPart{spec.x0, (int32_t)W, 0, 0, 1, (int32_t)W, 1, W});
+    }
     const int64_t nbz = (h + kPeriod - 1) / kPeriod;
     const int64_t groups = (nbz + g_max_bands - 1) / g_max_bands;
     int64_t slot_rows64 = std::min<int64_t>(h, kPeriod);
     if (spec.band_rows > 0) slot_rows64 = std::min<int64_t>(slot_rows64, spec.band_rows);
     const int32_t slot_rows = (int32_t)slot_rows64;
-    const int32_t tile_h = std::min(kBandTileH, (slot_rows + 3) / 4 * 4);
+    int64_t col_tiles = 0;  // x-tiles of one slot row over all parts and groups
+    for (const XPart& xp : parts)
+        col_tiles += (int64_t)xp.nblocks * ((xp.phase_cols - xp.slot_lo + kBandTileOut - 1) / kBandTileOut);
+    col_tiles *= groups;
+    const int64_t want_tiles = 4 * (int64_t)g_buf.cus * 3;
+    const int64_t fit_h = (slot_rows * col_tiles + want_tiles - 1) / want_tiles;
+    const int32_t tile_h = (int32_t)std::min<int64_t>(
+        std::min<int64_t>(kBandTileH, (slot_rows + 3) / 4 * 4),
+        std::max<int64_t>(kMinBandTileH, (fit_h + 3) / 4 * 4));
     int64_t b_first = 0;
     for (int64_t gi = 0; gi < groups; ++gi) {
         const int64_t nb = (nbz - b_first + (groups - gi) - 1) / (groups - gi);
         for (const XPart& xp : parts) {
-            const int32_t tiles_per_block = (kPeriod - xp.slot_lo + kBandTileOut - 1) / kBandTileOut;
+            const int32_t tiles_per_block =
+                (xp.phase_cols - xp.slot_lo + kBandTileOut - 1) / kBandTileOut;
             // slot rows per launch: about kBandCentersPerLaunch centers
             int64_t rows = kBandCentersPerLaunch / std::max<int64_t>(1, nb * xp.centers_per_row);
             rows = std::max<int64_t>(tile_h, rows / tile_h * tile_h);
@@ -387,12 +418,14 @@ void plan_banded(const SearchSpec& spec, std::deque<Work>& out) {
                 Work wk;
                 wk.banded = true;
                 wk.nb = (int)nb;
+                wk.ph = xp.ph;
                 BandParams& p = wk.band;
                 p.seed = spec.seed;
                 p.x0 = xp.x0;
                 p.w = xp.w;
                 p.ox_min = xp.ox_min;
                 p.slot_lo = xp.slot_lo;
+                p.phase_cols = xp.phase_cols;
                 p.tiles_per_block = tiles_per_block;
                 p.z0 = (int32_t)((uint32_t)spec.z0 + (uint32_t)(b_first * kPeriod));
                 p.h_valid = (int32_t)std::min<int64_t>(h - b_first * kPeriod, nb * kPeriod);
@@ -401,6 +434,7 @@ void plan_banded(const SearchSpec& spec, std::deque<Work>& out) {
                 p.tile_h = tile_h;
                 p.tiles_x = xp.nblocks * tiles_per_block;
                 p.tiles_total = p.tiles_x * ((p.s_end - p.s_begin + tile_h - 1) / tile_h);
+                p.tag = 0;
                 fill_filters(spec, p.min_size, p.max_size, p.use_annulus, p.rd_min_sq, p.acx,
                              p.acz);
                 int64_t c = 0;
@@ -525,18 +559,17 @@ int64_t run_search(const SearchSpec& spec, ResultRow* out, int32_t max_results,
     rk.keep = (size_t)max_results;
     rk.emit_min = spec.min_size;
 
-    // The banded scan needs pairs of columns 2^17 apart inside the region.
-    const bool banded = spec.w >= 2 * kPeriod && spec.mode != ScanMode::kPlain;
-    if (spec.mode == ScanMode::kBanded && !banded) return fail();
+    const bool banded = spec.mode != ScanMode::kPlain;
+    // Both kernels count first draws; the centers near a Java-rejected chunk
+    // are recounted exactly afterwards.
     std::deque<Work> works;
     std::vector<CenterFix> fixes;
     MismatchIndex mismatches;
-    if (banded) {
-        if (!prepare_fixes(spec, mismatches, rk, fixes)) return fail();
+    if (!prepare_fixes(spec, mismatches, rk, fixes)) return fail();
+    if (banded)
         plan_banded(spec, works);
-    } else {
+    else
         plan_plain_rows(spec, spec.x0, spec.w, works);
-    }
 
     int64_t total_centers = 0, done_centers = 0;
     for (const Work& w : works) total_centers += w.centers;
@@ -574,7 +607,7 @@ int64_t run_search(const SearchSpec& spec, ResultRow* out, int32_t max_results,
             if (f.first_draw >= spec.min_size && f.first_draw <= spec.max_size) rk.hist[f.first_draw]--;
             if (f.exact >= spec.min_size && f.exact <= spec.max_size) {
                 rk.hist[f.exact]++;
-                rk.cand.push_back(Hit{f.exact, f.cx, f.cz});
+                rk.cand.push_back(Hit{f.exact, f.cx, f.cz, 0});
             }
         }
     }

@@ -8,11 +8,13 @@
 
 namespace slime {
 
-// One emitted center: exact 221-cell count and chunk coordinates.
+// One emitted center: exact 221-cell count, chunk coordinates and the tag of
+// the search it belongs to (0 for a single search).
 struct Hit {
     int32_t size;
     int32_t cx;
     int32_t cz;
+    int32_t tag;
 };
 
 constexpr int kHistBins = 222;  // exact counts 0..221
@@ -58,7 +60,8 @@ struct BandParams {
     int32_t w;            // centers at offsets >= w from x0 are outside
     int32_t ox_min;       // centers at offsets < ox_min from x0 are outside
     int32_t slot_lo;      // first slot column of the x-tiles (0, or the tail block's)
-    int32_t tiles_per_block;  // x-tiles per block: ceil((P - slot_lo) / 496)
+    int32_t phase_cols;   // slot columns per phase: P with two phases, w with one
+    int32_t tiles_per_block;  // x-tiles per block: ceil((phase_cols - slot_lo) / 496)
     int32_t z0;           // center row of band 0, slot 0
     int32_t h_valid;      // center (b, s) is inside iff b*P + s < h_valid
     int32_t s_begin;      // first slot row of this launch (multiple of tile_h)
@@ -70,12 +73,14 @@ struct BandParams {
     int32_t use_annulus;
     int64_t rd_min_sq;
     int32_t acx, acz;
+    int32_t tag;          // written into every hit of this job
     Hit* hits;
     uint32_t hit_cap;
     unsigned long long* emitted;
     unsigned long long* hist;
 };
-hipError_t launch_scan_band(const BandParams& p, int nb, hipStream_t stream);
+// One workgroup per tile; nb bands (1..kMaxBands), ph x-phases (1 or 2).
+hipError_t launch_scan_band(const BandParams& p, int nb, int ph, hipStream_t stream);
 
 // ---- first-draw mismatches (Java's nextInt(10) rejection) ------------------
 //

@@ -12,8 +12,9 @@
 //   4. a group whose square reaches min_size gets the exact 221-cell count of
 //      its 16 centers from the ring (SlimeCore.cpp:526-533), which feeds the
 //      score histogram and the hit buffer.
-// The rare first draws that Java rejects (Y >= 0xFFFFFFF0) take the full
-// redraw loop inline, so the ring holds exact slime bits.
+// The ring holds first-draw bits (next(31) % 10 == 0, the slot arithmetic of
+// slime_math.h for one phase and one band); the engine recounts the few
+// centers near a chunk whose first draw Java rejects.
 #include "scan.h"
 #include "slime_math.h"
 
@@ -80,10 +81,13 @@ __global__ __launch_bounds__(kThreads) void scan_plain_kernel(ScanParams p) {
     const uint32_t chunk_x0 = (uint32_t)p.x0 + (uint32_t)off_x - 8u;
     const uint32_t chunk_z0 = (uint32_t)p.z0 + (uint32_t)off_z - 8u;
 
-    uint64_t xv[kWords];
+    uint32_t xlo[kWords], xh[kWords];
 #pragma unroll
-    for (int j = 0; j < kWords; ++j)
-        xv[j] = (uint64_t)p.seed + x_term((int32_t)(chunk_x0 + 64u * j + (uint32_t)lane));
+    for (int j = 0; j < kWords; ++j) {
+        const uint64_t X = (uint64_t)p.seed + x_term((int32_t)(chunk_x0 + 64u * j + (uint32_t)lane));
+        xlo[j] = (uint32_t)X << 16;
+        xh[j] = (uint32_t)(X >> 16);
+    }
 
     const int g = tid;
     const int gbit = 4 * g;
@@ -93,14 +97,14 @@ __global__ __launch_bounds__(kThreads) void scan_plain_kernel(ScanParams p) {
 
     for (int k = 0; k < nbatch; ++k) {
         const int row = 4 * k + wave;
-        const uint64_t zv = z_term((int32_t)(chunk_z0 + (uint32_t)row));
+        const uint64_t Z = z_term((int32_t)(chunk_z0 + (uint32_t)row));
+        const uint32_t zlo = (uint32_t)Z << 16, zhi = (uint32_t)(Z >> 16);
         uint32_t mine = 0;
 #pragma unroll
         for (int j = 0; j < kWords; ++j) {
-            const uint64_t s = xv[j] + zv;
-            const uint32_t y = first_y(s);
-            bool hit = first_draw_hit(y);
-            if (__builtin_expect(y >= kRejectY, 0)) hit = redraw_hit(s);
+            // one phase, one band: a hit needs the even parity and the fifth test
+            const BandSlot sl = band_slot(xlo[j], xh[j], xh[j], zlo, zhi);
+            const bool hit = !sl.e && band_value(sl, 0, zhi) <= kFifthLimit;
             const uint64_t m = __ballot(hit);
             mine = lane == 2 * j ? (uint32_t)m : mine;
             mine = lane == 2 * j + 1 ? (uint32_t)(m >> 32) : mine;
@@ -137,7 +141,7 @@ __global__ __launch_bounds__(kThreads) void scan_plain_kernel(ScanParams p) {
                     atomicAdd(&lhist[c], 1u);
                     if (c >= (uint32_t)p.emit_min) {
                         const unsigned long long idx = atomicAdd(p.emitted, 1ull);
-                        if (idx < p.hit_cap) p.hits[idx] = Hit{(int32_t)c, cx, cz};
+                        if (idx < p.hit_cap) p.hits[idx] = Hit{(int32_t)c, cx, cz, 0};
                     }
                 }
             }

@@ -1,13 +1,14 @@
 """GPU parity of the banded first-draw scan and its rejection correction.
 
-The banded kernel shares one LCG low half across 2 x-phases and up to 16
-z-bands spaced 2^17 apart, so it only runs on regions at least 2^18 columns
-wide. The checks:
+The banded kernel shares one LCG low half across 2 x-phases (1 for regions
+at most 2^17 columns wide) and up to 14 z-bands spaced 2^17 apart. The checks:
   * strided band sets (a few rows of each band) against the CPU oracle run on
-    each band's rectangle: totals, whole score histograms and ranked rows;
-  * the banded scan against the plain per-chunk scan (an independent kernel
-    with inline Java redraws) on the same large regions: bit-identical
-    histograms over ~10^11 centers, the size-independent check;
+    each band's rectangle: totals, whole score histograms and ranked rows,
+    for one phase, two phases with a masked edge, and right-aligned tails;
+  * the banded scan against the plain per-chunk scan (an independent kernel:
+    own tiling, ballot packing and counting) on the same regions:
+    bit-identical histograms over up to ~10^11 centers, the size-independent
+    check;
   * seeds built so that chunks whose first draw Java rejects lie inside the
     scanned region, both rejected-then-hit and first-draw false positives.
 """
@@ -90,16 +91,18 @@ def oracle_bands(oracle, seed, x0, z0, w, h, band_rows, thr, k):
     return total, hist, rows[:k]
 
 
-@pytest.mark.parametrize("case", range(5))
+@pytest.mark.parametrize("case", range(9))
 def test_band_sets_vs_oracle(gpu, oracle, case):
     """case 1: a remainder wider than P; case 4: a remainder of 700 columns
-    (the right-aligned tail block starts 700 slot columns before its end)."""
+    (the right-aligned tail block starts 700 slot columns before its end);
+    cases 5, 6: one phase (narrow and exactly P wide); cases 7, 8: one
+    two-phase block whose phase-1 columns run past the region's edge."""
     import slime_amd
     rng = random.Random(77 + case)
     seed = s64(rng.getrandbits(64))
-    nb = [1, 3, 5, 16, 2][case]
-    band_rows = [24, 12, 9, 6, 20][case]
-    w = [2 * P, 3 * P + 1000, 2 * P, 4 * P, 2 * P + 700][case]
+    nb = [1, 3, 5, 16, 2, 3, 2, 1, 4][case]
+    band_rows = [24, 12, 9, 6, 20, 20, 10, 30, 8][case]
+    w = [2 * P, 3 * P + 1000, 2 * P, 4 * P, 2 * P + 700, 600, P, P + 1, P + P // 2 + 3][case]
     x0 = rng.randint(-10 ** 8, 10 ** 8)
     z0 = rng.randint(-10 ** 8, 10 ** 8)
     h = (nb - 1) * P + band_rows
@@ -113,7 +116,8 @@ def test_band_sets_vs_oracle(gpu, oracle, case):
     assert rows == want_rows
 
 
-@pytest.mark.parametrize("radius,thr", [(131100, 55), (200000, 58), (400000, 57)])
+@pytest.mark.parametrize("radius,thr", [(3000, 45), (20000, 52), (65536, 54), (70000, 54),
+                                        (131100, 55), (200000, 58), (400000, 57)])
 def test_banded_equals_plain_large(gpu, radius, thr):
     """Same square through both kernels: identical totals, histograms, rows."""
     import slime_amd
@@ -154,8 +158,8 @@ def test_rejected_chunks_inside_banded_region(gpu, oracle, java_hit):
 
 
 def test_full_width_search_hits_recount(gpu, oracle):
-    """A wide search (banded + plain remainder): every reported hit recounts
-    exactly on the CPU and the total equals the plain kernel's."""
+    """A wide search (main blocks + right-aligned tail block): every reported
+    hit recounts exactly on the CPU and the total equals the plain kernel's."""
     import slime_amd
     seed, rd, thr = -7995527694508729151, 300000, 56
     total, rows = slime_amd.search_slime_clusters_centered(seed, rd, thr, 221, 0, 1000, -2000, 100)
