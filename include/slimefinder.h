@@ -111,8 +111,8 @@ int64_t slime_scan_rect(int64_t seed, int32_t x0, int32_t z0, int64_t w, int64_t
 /* Extension with no counterpart in the reference: the same scan over the
  * rows of [z0, z0+h) or, when band_rows > 0, only over the strided bands
  * z0 + b*2^17 + s (s < band_rows), with a forced scan mode: 0 automatic,
- * 1 plain per-chunk scan, 2 banded scan (returns -1 if the region is narrower
- * than 2^18 columns). Lets tests and callers pick the kernel explicitly. */
+ * 1 plain per-chunk scan, 2 banded scan (one x-phase up to 2^17 columns,
+ * two above). Lets tests and callers pick the kernel explicitly. */
 int64_t slime_scan_bands(int64_t seed, int32_t x0, int32_t z0, int64_t w, int64_t h,
                          int32_t band_rows, int32_t mode, int32_t min_size, int32_t max_size,
                          ExtChunkResult* results_buffer, int32_t max_results, int64_t* hist222);
@@ -129,6 +129,36 @@ void slime_kernel_stats(int32_t kind, int64_t* launches, int64_t* kernel_ns, int
  * clock (kHz) and HBM size of the current device; 0 on success, -1 if no
  * device is usable. */
 int32_t slime_device_info(int32_t* compute_units, int32_t* clock_khz, int64_t* hbm_bytes);
+
+/* ---- batched searches (extension; the reference runs one search per call)
+ * n independent searches, each with the meaning of
+ * search_slime_clusters_gpu_v1_centered (SlimeCoreGPU.cu:3927-3935), scanned
+ * together by one persistent launch per kernel class fed from a device work
+ * queue. Search i writes up to max_results ranked rows at
+ * results_buffer + i * max_results and its return value (hit count, or -1 for
+ * invalid arguments) to totals[i]. flags bit 0: replay the sweep's launches
+ * from a captured hipGraph. Returns 0, or -1 on bad arrays or a device error.
+ * Results are identical to n separate calls. */
+int32_t slime_search_batch(int32_t n, const int64_t* seeds, const int64_t* rd_max,
+                           const int32_t* min_size, const int32_t* max_size, const int64_t* rd_min,
+                           const int32_t* center_x, const int32_t* center_z,
+                           ExtChunkResult* results_buffer, int32_t max_results, int64_t* totals,
+                           int32_t flags);
+
+/* Extension: figures of the last batch: candidate centers, device time of the
+ * sweep launches, host time of planning + fix-up + upload, of the sweep, and
+ * of the whole call (ns), kernel launches, jobs, searches rerun alone after a
+ * hit-region overflow. Any pointer may be NULL. */
+void slime_batch_stats(int64_t* centers, int64_t* kernel_ns, int64_t* prepare_ns,
+                       int64_t* sweep_ns, int64_t* wall_ns, int32_t* launches, int32_t* jobs,
+                       int32_t* fallbacks);
+
+/* Extension: the autotuner's measured cost table, one entry per kernel class
+ * (bands nb, x-phases ph): device ns per tile row with the whole GPU on that
+ * class, and the fixed per-tile cost in rows. Measures on first use. Returns
+ * the number of classes written (at most cap), or -1 on a device error. */
+int32_t slime_sweep_costs(double* ns_row, double* extra_rows, int32_t* nb, int32_t* ph,
+                          int32_t cap);
 
 #ifdef __cplusplus
 }

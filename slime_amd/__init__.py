@@ -28,7 +28,8 @@ EXPORTS = (
     "get_cuda_device_count", "get_cuda_device_name", "cleanup_gpu_resources",
     "get_gpu_v1_shape", "get_gpu_v1_algorithm", "get_gpu_v1_rng", "set_gpu_v1_rng_override",
     "get_gpu_v34_shape", "get_gpu_v34_rng", "set_gpu_v34_rng_override", "slime_scan_rect",
-    "slime_kernel_stats", "slime_device_info", "slime_scan_bands",
+    "slime_kernel_stats", "slime_device_info", "slime_scan_bands", "slime_search_batch",
+    "slime_batch_stats", "slime_sweep_costs",
 )
 
 
@@ -95,6 +96,14 @@ def lib():
     L.slime_scan_bands.argtypes = [i64, i32, i32, i64, i64, i32, i32, i32, i32, P, i32,
                                    C.POINTER(C.c_int64)]
     L.slime_scan_bands.restype = i64
+    PI64, PI32, PD = C.POINTER(i64), C.POINTER(i32), C.POINTER(C.c_double)
+    L.slime_search_batch.argtypes = [i32, PI64, PI64, PI32, PI32, PI64, PI32, PI32, P, i32, PI64,
+                                     i32]
+    L.slime_search_batch.restype = i32
+    L.slime_batch_stats.argtypes = [PI64, PI64, PI64, PI64, PI64, PI32, PI32, PI32]
+    L.slime_batch_stats.restype = None
+    L.slime_sweep_costs.argtypes = [PD, PD, PI32, PI32, i32]
+    L.slime_sweep_costs.restype = i32
     _lib = L
     return L
 
@@ -172,3 +181,58 @@ def scan_rect(seed, x0, z0, w, h, min_size, max_size=221, max_results=65536):
     hist = (C.c_int64 * 222)()
     total = lib().slime_scan_rect(seed, x0, z0, w, h, min_size, max_size, buf, max_results, hist)
     return total, _rows(buf, total, max_results), list(hist)
+
+
+def search_batch(seeds, rd_max, min_size, max_size=None, rd_min=None, center_x=None,
+                 center_z=None, max_results=1000, use_graph=False):
+    """Many searches in one persistent sweep (slime_search_batch): a list of
+    (total, rows) per search, each identical to search_slime_clusters_gpu_v1_centered."""
+    n = len(seeds)
+    def arr(t, v, default):
+        v = [default] * n if v is None else list(v)
+        return (t * max(1, n))(*v)
+    a_seed = arr(C.c_int64, seeds, 0)
+    a_rd = arr(C.c_int64, rd_max, 0)
+    a_min = arr(C.c_int32, min_size, 0)
+    a_max = arr(C.c_int32, max_size, 221)
+    a_rmin = arr(C.c_int64, rd_min, 0)
+    a_cx = arr(C.c_int32, center_x, 0)
+    a_cz = arr(C.c_int32, center_z, 0)
+    buf = (ExtChunkResult * (max(1, n) * max_results))()
+    tot = (C.c_int64 * max(1, n))()
+    rc = lib().slime_search_batch(n, a_seed, a_rd, a_min, a_max, a_rmin, a_cx, a_cz, buf,
+                                  max_results, tot, 1 if use_graph else 0)
+    if rc != 0:
+        raise RuntimeError("slime_search_batch failed")
+    out = []
+    for i in range(n):
+        t = tot[i]
+        k = min(max(t, 0), max_results)
+        base = i * max_results
+        out.append((t, [(buf[base + j].size, buf[base + j].center_x, buf[base + j].center_z,
+                         buf[base + j].obs_count, buf[base + j].afk_x, buf[base + j].afk_z)
+                        for j in range(k)]))
+    return out
+
+
+def batch_stats():
+    """Figures of the last batch as a dict (slime_batch_stats)."""
+    v64 = [C.c_int64() for _ in range(5)]
+    v32 = [C.c_int32() for _ in range(3)]
+    lib().slime_batch_stats(*[C.byref(x) for x in v64], *[C.byref(x) for x in v32])
+    keys64 = ("centers", "kernel_ns", "prepare_ns", "sweep_ns", "wall_ns")
+    keys32 = ("launches", "jobs", "fallbacks")
+    d = {k: x.value for k, x in zip(keys64, v64)}
+    d.update({k: x.value for k, x in zip(keys32, v32)})
+    return d
+
+
+def sweep_costs():
+    """The autotuner's measured table: [(nb, ph, ns_per_tile_row, extra_rows)]."""
+    cap = 32
+    a, b = (C.c_double * cap)(), (C.c_double * cap)()
+    nb, ph = (C.c_int32 * cap)(), (C.c_int32 * cap)()
+    n = lib().slime_sweep_costs(a, b, nb, ph, cap)
+    if n < 0:
+        raise RuntimeError("slime_sweep_costs failed")
+    return [(nb[i], ph[i], a[i], b[i]) for i in range(n)]

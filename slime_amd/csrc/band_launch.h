@@ -10,11 +10,10 @@ namespace band {
 // stream of the generation with the LDS waits of the counting.
 template <int NB, int PH>
 __global__ __launch_bounds__(kThreads, 3) void scan_band_kernel(BandParams p) {
-    __shared__ uint32_t ring[Ring<NB>::kDwords];
-    __shared__ uint32_t lhist[kHistBins];
-    for (int i = threadIdx.x; i < kHistBins; i += kThreads) lhist[i] = 0;
-    band_tile<NB, PH>(p, p.emit_min, blockIdx.x, ring, lhist);
-    flush_hist(lhist, p.hist);
+    __shared__ TileLds<NB> lds;
+    for (int i = threadIdx.x; i < kHistBins; i += kThreads) lds.lhist[i] = 0;
+    band_tile<NB, PH>(p, p.emit_min, blockIdx.x, lds);
+    flush_hist(lds.lhist, p.hist);
 }
 
 template <int NB, int PH>

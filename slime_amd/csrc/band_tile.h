@@ -28,10 +28,13 @@
 // (reference: x_phase.py:185-252 and benchmarks/algorithm-optimizations.en.md
 // describe the same 2^17 parity relation for 2 X phases and up to 15 Z bands.)
 //
-// Counting follows the plain scan: per band-phase a rolling 20x20 square of
-// each 4x4 group, a 332-cell union-of-disks bound, then the exact 221-cell
-// count. Java's rejected first draws are not handled here; the engine
-// recounts the few affected centers.
+// Counting: per band-phase a rolling 20x20 square of each 4x4 group; the
+// groups that reach min_size are refined by the whole wave together (a bound
+// on the union of the group's 16 disks, 20 lanes per group, then the exact
+// 221-cell counts, 4 lanes per center), so a rare passing group costs the
+// wave a few instructions instead of a long divergent run of one lane.
+// Java's rejected first draws are not handled here; the engine recounts the
+// few affected centers.
 #pragma once
 
 #include <type_traits>
@@ -84,6 +87,13 @@ struct Ring {
     static constexpr int kDwords = kRing * kRowDw;
 };
 
+// LDS of one workgroup: the bitplane ring and the score histogram.
+template <int NB>
+struct TileLds {
+    uint32_t ring[Ring<NB>::kDwords];
+    uint32_t lhist[kHistBins];
+};
+
 // Counting roles: two threads per 4x4 group, each with half of the bands (both
 // phases), except one band: with two phases each thread takes one phase, with
 // one phase a single thread counts.
@@ -96,85 +106,140 @@ struct Roles {
     static constexpr int kOwned = kBands * kPhases;
 };
 
-// Bit mask of the union of the 16 disks of a 4x4 group in group row r
-// (0..19): centers sit at window bits 8..11, so the row spans bits
-// [8 - h, 11 + h] with h the widest disk row any of the 4 center rows reaches.
-__device__ __forceinline__ uint32_t union_row_mask(int r) {
-    int h = -1;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int dz = r - 8 - j;
-        if (dz >= -8 && dz <= 8) h = max(h, disk_half_width(dz < 0 ? -dz : dz));
+// Disk half-widths 8,8,8,7,7,6,5,4,2 for |dz| = 0..8 as nibbles.
+constexpr uint64_t kHalfWidthNibbles = 0x245677888ull;
+constexpr int half_width_of(int adz) { return (int)((kHalfWidthNibbles >> (4 * adz)) & 15u); }
+
+// Half-width of the union of the 16 disks of a 4x4 group in group row r
+// (0..19): the widest disk row any of the group's 4 center rows reaches.
+constexpr int union_half_width(int r) {
+    const int d = r < 19 - r ? r : 19 - r;
+    return d >= 6 ? 8 : d == 0 ? 2 : d >= 4 ? 7 : d + 3;
+}
+
+// Both agree with the disk definition (dx^2 + dz^2 <= 68).
+constexpr int disk_half_width_ref(int adz) {
+    int h = 0;
+    while ((h + 1) * (h + 1) + adz * adz <= 68) ++h;
+    return h;
+}
+constexpr bool check_tables() {
+    for (int adz = 0; adz <= 8; ++adz)
+        if (half_width_of(adz) != disk_half_width_ref(adz)) return false;
+    for (int r = 0; r < 20; ++r) {
+        int h = -1;
+        for (int j = 0; j < 4; ++j) {
+            const int dz = r - 8 - j;
+            if (dz >= -8 && dz <= 8) {
+                const int w = disk_half_width_ref(dz < 0 ? -dz : dz);
+                h = w > h ? w : h;
+            }
+        }
+        if (h != union_half_width(r)) return false;
     }
+    return true;
+}
+static_assert(check_tables(), "disk and union row tables");
+
+// Row r of a group's union of disks: bits [8 - h, 11 + h] of the window.
+__device__ __forceinline__ uint32_t union_row_mask(int r) {
+    const int h = union_half_width(r);
     return ((1u << (4 + 2 * h)) - 1u) << (8 - h);
 }
 
-// The group of band b, phase ph whose 20x20 square passed: a second bound on
-// the union of its 16 disks (332 cells), then the exact 221-cell count of each
-// center, the histogram and the hit buffer.
+// A group of one band-phase, as a lane hands it to its wave:
+// (gbit << 8) | (b << 1) | ph, gbit = 4 * group.
+__device__ __forceinline__ uint32_t group_code(int gbit, int b, int ph) {
+    return ((uint32_t)gbit << 8) | ((uint32_t)b << 1) | (uint32_t)ph;
+}
+
+// Row r (0..19) of the 20-bit window of a coded group: slime chunks of its
+// band-phase starting at group row `first` of the ring.
 template <int NB>
-__device__ __forceinline__ void refine_group(const BandParams& p, int32_t emit_min,
-                                             const uint32_t* ring, int first, int gbit, int b,
-                                             int ph, int64_t ox0, int col, int32_t s_g,
-                                             uint32_t* lhist) {
-    using R = Ring<NB>;
+__device__ __forceinline__ uint32_t group_row(const uint32_t* ring, int first, uint32_t code,
+                                              int r) {
+    const int gbit = (int)(code >> 8), b = (int)((code >> 1) & 15u), ph = (int)(code & 1u);
     const uint32_t flip = ((b ^ ph) & 1) ? 0u : 0xFFFFFFFFu;  // phase mask is E or ~E
-    auto window = [&](int r) {
-        const uint32_t* row = ring + ((first + r) % kRing) * R::kRowDw;
-        return window20(row + (b + 1) * kPlaneDw, gbit) & (window20(row, gbit) ^ flip);
-    };
-    uint32_t bound = 0;
+    const uint32_t* row = ring + ((first + r) % kRing) * Ring<NB>::kRowDw;
+    return window20(row + (b + 1) * kPlaneDw, gbit) & (window20(row, gbit) ^ flip);
+}
+
+// The wave refines the groups its lanes found (todo: a bit per owned
+// band-phase q, coded by code_of(q)). All 64 lanes must be active.
+//  1. Up to three groups at a time: lanes 20s..20s+19 count row r of group s
+//     under the union-of-disks mask; the five bits of each count are summed
+//     per group with ballots (a scalar popcount each).
+//  2. A group whose union count reaches min_size: lane 4c + part sums the
+//     disk rows dz = -8 + part + 4t of center c (4 lanes per center), a quad
+//     reduction gives the exact 221-cell count, and lane 4c emits it.
+template <int NB, typename CodeOf, typename EmitCenter>
+__device__ __forceinline__ void refine_wave(uint32_t todo, const CodeOf& code_of,
+                                            const uint32_t* ring, int first, int32_t min_size,
+                                            const EmitCenter& emit) {
+    const int lane = threadIdx.x & 63;
+    const int slot = lane < 60 ? lane / 20 : 3;
+    const int r = lane - 20 * slot;
+    for (;;) {
+        uint64_t act = __ballot(todo != 0);
+        if (act == 0) break;
+        const uint32_t mine = todo ? code_of(__builtin_ctz(todo)) : 0u;
+        uint32_t code[3] = {0u, 0u, 0u};
+        int nsel = 0;
+        uint64_t taken = 0;
 #pragma unroll
-    for (int r = 0; r < 20; ++r) bound += __popc(window(r) & union_row_mask(r));
-    if (bound < (uint32_t)p.min_size) return;
-    uint32_t cnt[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) cnt[q] = 0;
-#pragma unroll
-    for (int r = 0; r < 20; ++r) {
-        const uint32_t wr = window(r);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int dz = r - 8 - j;
-            if (dz < -8 || dz > 8) continue;
-            const int hw = disk_half_width(dz < 0 ? -dz : dz);
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-                cnt[4 * j + i] += __popc(wr & (((1u << (2 * hw + 1)) - 1u) << (8 + i - hw)));
+        for (int s = 0; s < 3; ++s) {
+            if (act != 0) {
+                const int l = __builtin_ctzll(act);
+                act &= act - 1;
+                taken |= 1ull << l;
+                code[s] = __builtin_amdgcn_readlane(mine, l);
+                nsel = s + 1;
+            }
         }
-    }
+        if ((taken >> lane) & 1ull) todo &= todo - 1;
+        const uint32_t my_code = slot == 0 ? code[0] : slot == 1 ? code[1] : code[2];
+        uint32_t c = 0;
+        if (slot < nsel) c = __popc(group_row<NB>(ring, first, my_code, r) & union_row_mask(r));
+        uint64_t bits[5];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int i = q & 3, jr = q >> 2;
-        const uint32_t c = cnt[q];
-        if (c < (uint32_t)p.min_size || c > (uint32_t)p.max_size) continue;
-        if (col + i >= p.phase_cols || ox0 + i >= p.w || ox0 + i < p.ox_min) continue;
-        const int32_t s_c = s_g + jr;
-        if (s_c >= p.s_end || (int64_t)b * kPeriod + s_c >= p.h_valid) continue;
-        const int32_t cx = (int32_t)((uint32_t)p.x0 + (uint32_t)(ox0 + i));
-        const int32_t cz =
-            (int32_t)((uint32_t)p.z0 + (uint32_t)b * (uint32_t)kPeriod + (uint32_t)s_c);
-        if (p.use_annulus) {
-            const int64_t dx = (int64_t)cx - p.acx, dz = (int64_t)cz - p.acz;
-            if (dx * dx + dz * dz < p.rd_min_sq) continue;
-        }
-        atomicAdd(&lhist[c], 1u);
-        if (c >= (uint32_t)emit_min) {
-            const unsigned long long idx = atomicAdd(p.emitted, 1ull);
-            if (idx < p.hit_cap) p.hits[idx] = Hit{(int32_t)c, cx, cz, p.tag};
+        for (int j = 0; j < 5; ++j) bits[j] = __ballot((c >> j) & 1u);
+        for (int s = 0; s < nsel; ++s) {
+            const uint64_t m = 0xFFFFFull << (20 * s);
+            uint32_t sum = 0;
+#pragma unroll
+            for (int j = 0; j < 5; ++j) sum += (uint32_t)__popcll(bits[j] & m) << j;
+            if (sum < (uint32_t)min_size) continue;
+            // exact counts of the group's 16 centers
+            const int center = lane >> 2, part = lane & 3;
+            const int i = center & 3, jr = center >> 2;
+            uint32_t cnt = 0;
+#pragma unroll
+            for (int t = 0; t < 5; ++t) {
+                const int dz = -8 + part + 4 * t;
+                if (dz <= 8) {
+                    const int hw = half_width_of(dz < 0 ? -dz : dz);
+                    const uint32_t w = group_row<NB>(ring, first, code[s], 8 + jr + dz);
+                    cnt += __popc(w & (((1u << (2 * hw + 1)) - 1u) << (8 + i - hw)));
+                }
+            }
+            cnt += __shfl_xor(cnt, 1);
+            cnt += __shfl_xor(cnt, 2);
+            if (part == 0) emit(code[s], i, jr, cnt);
         }
     }
 }
 
-// Scans tile `tile` of the job into the caller's LDS ring (Ring<NB>::kDwords)
-// and LDS histogram (kHistBins, accumulated). The caller synchronises the
-// workgroup before the call when the ring was in use.
+// Scans tile `tile` of the job with the workgroup's LDS; scores accumulate in
+// lds.lhist. The caller synchronises the workgroup before the call when the
+// LDS was in use.
 template <int NB, int PH>
 __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min, int tile,
-                                          uint32_t* ring, uint32_t* lhist) {
+                                          TileLds<NB>& lds) {
     static_assert(PH == 1 || PH == 2, "one or two x-phases");
     using R = Ring<NB>;
     using C = Roles<NB, PH>;
+    uint32_t* ring = lds.ring;
+    uint32_t* lhist = lds.lhist;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -267,6 +332,9 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
         __syncthreads();
 
         // ---- counting: rolling squares of the owned band-phases -----------
+        const int m = k - 4;
+        const int32_t s_g = tile_s0 + 4 * m;  // first slot row of the groups
+        uint32_t pass = 0;  // owned band-phases whose square reaches min_size
         if (counts) {
             uint32_t c4[C::kOwned];
 #pragma unroll
@@ -289,7 +357,6 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
                     }
                 }
             }
-            uint32_t pass = 0;  // owned band-phases whose square reaches min_size
 #pragma unroll
             for (int q = 0; q < C::kOwned; ++q) {
                 const uint32_t square = (sq[q] & 0xFFFFu) + c4[q] - (sq[q] >> 16);
@@ -297,21 +364,42 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
                 hist[q] = (hist[q] << 8) | c4[q];
                 pass |= (square >= (uint32_t)p.min_size ? 1u : 0u) << q;
             }
-            const int m = k - 4;
-            const int32_t s_g = tile_s0 + 4 * m;  // first slot row of the group
             if (m < 0 || col0 + gbit >= p.phase_cols || s_g >= p.s_end) pass = 0;
-            while (pass) {
-                const int q = __builtin_ctz(pass);
-                pass &= pass - 1;
+            // keep the band-phases with a center inside this job
+#pragma unroll
+            for (int q = 0; q < C::kOwned; ++q) {
                 const int b = b_lo + q / C::kPhases, ph = ph_lo + q % C::kPhases;
-                if (b >= NB) continue;
                 const int64_t ox0 = block_off + (int64_t)ph * kPeriod + col0 + gbit;
-                if (ox0 >= p.w || ox0 + 3 < p.ox_min || (int64_t)b * kPeriod + s_g >= p.h_valid)
-                    continue;
-                refine_group<NB>(p, emit_min, ring, 4 * m, gbit, b, ph, ox0, col0 + gbit, s_g,
-                                 lhist);
+                if (b >= NB || ox0 >= p.w || ox0 + 3 < p.ox_min ||
+                    (int64_t)b * kPeriod + s_g >= p.h_valid)
+                    pass &= ~(1u << q);
             }
         }
+        // ---- refinement by the whole wave ----------------------------------
+        auto code_of = [&](int q) {
+            return group_code(gbit, b_lo + q / C::kPhases, ph_lo + q % C::kPhases);
+        };
+        auto emit = [&](uint32_t code, int i, int jr, uint32_t c) {
+            const int gb = (int)(code >> 8), b = (int)((code >> 1) & 15u), ph = (int)(code & 1u);
+            const int64_t ox = block_off + (int64_t)ph * kPeriod + col0 + gb + i;
+            const int32_t s_c = s_g + jr;
+            if (c < (uint32_t)p.min_size || c > (uint32_t)p.max_size) return;
+            if (col0 + gb + i >= p.phase_cols || ox >= p.w || ox < p.ox_min) return;
+            if (s_c >= p.s_end || (int64_t)b * kPeriod + s_c >= p.h_valid) return;
+            const int32_t cx = (int32_t)((uint32_t)p.x0 + (uint32_t)ox);
+            const int32_t cz =
+                (int32_t)((uint32_t)p.z0 + (uint32_t)b * (uint32_t)kPeriod + (uint32_t)s_c);
+            if (p.use_annulus) {
+                const int64_t dx = (int64_t)cx - p.acx, dz = (int64_t)cz - p.acz;
+                if (dx * dx + dz * dz < p.rd_min_sq) return;
+            }
+            atomicAdd(&lhist[c], 1u);
+            if (c >= (uint32_t)emit_min) {
+                const unsigned long long idx = atomicAdd(p.emitted, 1ull);
+                if (idx < p.hit_cap) p.hits[idx] = Hit{(int32_t)c, cx, cz, p.tag};
+            }
+        };
+        refine_wave<NB>(pass, code_of, ring, 4 * m, p.min_size, emit);
     }
 }
 

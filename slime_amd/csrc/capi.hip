@@ -9,10 +9,12 @@
 #include <algorithm>
 #include <climits>
 #include <cstring>
+#include <vector>
 
 #include "../../include/slimefinder.h"
 #include "engine.h"
 #include "slime_math.h"
+#include "sweep_engine.h"
 
 using slime::control;
 
@@ -41,12 +43,10 @@ bool valid_args(int64_t rd_max, int64_t rd_max_limit, int32_t min_size, int32_t 
     return true;
 }
 
-int64_t search_common(int64_t seed, int64_t rd_max, int32_t min_size, int32_t max_size,
-                      int64_t rd_min, int32_t cx, int32_t cz, ExtChunkResult* out,
-                      int32_t max_results, int64_t rd_max_limit) {
-    if (!valid_args(rd_max, rd_max_limit, min_size, max_size, rd_min, cx, cz, out, max_results))
-        return -1;
-    control().progress.store(1, std::memory_order_relaxed);
+// The square of centers within rd_max (Chebyshev) of (cx, cz), with the
+// annulus filter rd_min (SlimeCore.cpp:430-449, 504-512).
+slime::SearchSpec square_spec(int64_t seed, int64_t rd_max, int32_t min_size, int32_t max_size,
+                              int64_t rd_min, int32_t cx, int32_t cz) {
     slime::SearchSpec spec;
     spec.seed = seed;
     spec.x0 = (int32_t)((int64_t)cx - rd_max);
@@ -58,6 +58,16 @@ int64_t search_common(int64_t seed, int64_t rd_max, int32_t min_size, int32_t ma
     spec.rd_min = rd_min;
     spec.acx = cx;
     spec.acz = cz;
+    return spec;
+}
+
+int64_t search_common(int64_t seed, int64_t rd_max, int32_t min_size, int32_t max_size,
+                      int64_t rd_min, int32_t cx, int32_t cz, ExtChunkResult* out,
+                      int32_t max_results, int64_t rd_max_limit) {
+    if (!valid_args(rd_max, rd_max_limit, min_size, max_size, rd_min, cx, cz, out, max_results))
+        return -1;
+    control().progress.store(1, std::memory_order_relaxed);
+    const slime::SearchSpec spec = square_spec(seed, rd_max, min_size, max_size, rd_min, cx, cz);
     g_last_shape = 0;
     g_last_algorithm = 0;
     return slime::run_search(spec, reinterpret_cast<slime::ResultRow*>(out), max_results);
@@ -253,5 +263,75 @@ void set_gpu_v1_rng_override(int32_t value) {
 int32_t get_gpu_v34_shape(void) { return get_gpu_v1_shape(); }
 int32_t get_gpu_v34_rng(void) { return get_gpu_v1_rng(); }
 void set_gpu_v34_rng_override(int32_t value) { set_gpu_v1_rng_override(value); }
+
+int32_t slime_search_batch(int32_t n, const int64_t* seeds, const int64_t* rd_max,
+                           const int32_t* min_size, const int32_t* max_size, const int64_t* rd_min,
+                           const int32_t* center_x, const int32_t* center_z,
+                           ExtChunkResult* results_buffer, int32_t max_results, int64_t* totals,
+                           int32_t flags) {
+    if (n < 0 || (n > 0 && (!seeds || !rd_max || !min_size || !max_size || !rd_min ||
+                            !center_x || !center_z || !results_buffer || !totals)) ||
+        max_results <= 0)
+        return -1;
+    std::vector<slime::SearchSpec> specs;
+    std::vector<int32_t> index;
+    for (int32_t i = 0; i < n; ++i) {
+        totals[i] = -1;
+        if (!valid_args(rd_max[i], kGpuRdLimit, min_size[i], max_size[i], rd_min[i], center_x[i],
+                        center_z[i], results_buffer, max_results))
+            continue;
+        specs.push_back(square_spec(seeds[i], rd_max[i], min_size[i], max_size[i], rd_min[i],
+                                    center_x[i], center_z[i]));
+        index.push_back(i);
+    }
+    if (specs.empty()) return 0;
+    std::vector<slime::ResultRow> rows(specs.size() * (size_t)max_results);
+    std::vector<int64_t> tot(specs.size(), -1);
+    slime::SweepOptions opt;
+    opt.use_graph = (flags & 1) != 0;
+    if (slime::run_batch(specs.data(), (int32_t)specs.size(), max_results, rows.data(),
+                         tot.data(), opt, nullptr) != 0)
+        return -1;
+    for (size_t k = 0; k < specs.size(); ++k) {
+        const int32_t i = index[k];
+        totals[i] = tot[k];
+        const size_t m = (size_t)std::max<int64_t>(0, std::min<int64_t>(tot[k], max_results));
+        std::memcpy(results_buffer + (size_t)i * max_results, rows.data() + k * max_results,
+                    sizeof(ExtChunkResult) * m);
+    }
+    return 0;
+}
+
+void slime_batch_stats(int64_t* centers, int64_t* kernel_ns, int64_t* prepare_ns,
+                       int64_t* sweep_ns, int64_t* wall_ns, int32_t* launches, int32_t* jobs,
+                       int32_t* fallbacks) {
+    slime::BatchStats st;
+    slime::last_batch_stats(&st);
+    if (centers) *centers = st.centers;
+    if (kernel_ns) *kernel_ns = st.kernel_ns;
+    if (prepare_ns) *prepare_ns = st.prepare_ns;
+    if (sweep_ns) *sweep_ns = st.sweep_ns;
+    if (wall_ns) *wall_ns = st.wall_ns;
+    if (launches) *launches = st.launches;
+    if (jobs) *jobs = st.jobs;
+    if (fallbacks) *fallbacks = st.fallbacks;
+}
+
+int32_t slime_sweep_costs(double* ns_row, double* extra_rows, int32_t* nb, int32_t* ph,
+                          int32_t cap) {
+    double a[slime::kSweepClasses], b[slime::kSweepClasses];
+    if (!slime::sweep_cost_table(a, b)) {
+        (void)hipGetLastError();
+        return -1;
+    }
+    const int32_t n = std::min<int32_t>(cap, slime::kSweepClasses);
+    for (int32_t c = 0; c < n; ++c) {
+        if (ns_row) ns_row[c] = a[c];
+        if (extra_rows) extra_rows[c] = b[c];
+        if (nb) nb[c] = slime::kSweepClass[c].nb;
+        if (ph) ph[c] = slime::kSweepClass[c].ph;
+    }
+    return n;
+}
 
 }  // extern "C"

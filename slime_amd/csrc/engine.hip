@@ -26,6 +26,7 @@
 #include <thread>
 
 #include "engine.h"
+#include "search_util.h"
 #include "slime_math.h"
 
 namespace slime {
@@ -40,7 +41,6 @@ namespace {
 constexpr uint32_t kHitCap = 1u << 24;                       // hits per launch (192 MB)
 constexpr int64_t kPlainCentersPerLaunch = int64_t(1) << 37;
 constexpr int64_t kBandCentersPerLaunch = int64_t(1) << 39;
-constexpr uint32_t kMismatchCap = 1u << 22;
 constexpr int kBandTileH = 2048;
 constexpr int kMinBandTileH = 64;
 // Bands per banded launch (kernel instantiations: 1, 2, 4, ..., 16; up to 14
@@ -149,33 +149,6 @@ bool ensure_fixes(size_t n) {
     g_buf.fixes_cap = want;
     return true;
 }
-
-// Largest score s with at least k counted hits of score >= s; `floor` if none.
-int32_t gate_for(const int64_t* hist, int64_t k, int32_t floor) {
-    int64_t acc = 0;
-    for (int32_t s = kHistBins - 1; s >= 0; --s) {
-        acc += std::max<int64_t>(0, hist[s]);
-        if (acc >= k) return std::max(s, floor);
-    }
-    return floor;
-}
-
-// The first-draw mismatch chunks of one search, sorted by (z, x).
-struct MismatchIndex {
-    std::vector<MismatchChunk> mm;
-
-    static bool in_disk(int64_t dx, int64_t dz) { return dx * dx + dz * dz <= 68; }
-
-    // Does the disk of center (cx, cz) contain a mismatch chunk?
-    bool affects(int32_t cx, int32_t cz) const {
-        if (mm.empty()) return false;
-        auto it = std::lower_bound(mm.begin(), mm.end(), (int64_t)cz - 8,
-                                   [](const MismatchChunk& m, int64_t z) { return m.z < z; });
-        for (; it != mm.end() && (int64_t)it->z <= (int64_t)cz + 8; ++it)
-            if (in_disk((int64_t)it->x - cx, (int64_t)it->z - cz)) return true;
-        return false;
-    }
-};
 
 // Score bookkeeping shared by every launch of one search.
 struct Ranker {
@@ -308,7 +281,7 @@ bool split_work(const Work& wk, Work& a, Work& b) {
     return true;
 }
 
-void fill_filters(const SearchSpec& spec, int32_t& min_size, int32_t& max_size,
+void fill_plain_filters(const SearchSpec& spec, int32_t& min_size, int32_t& max_size,
                   int32_t& use_annulus, int64_t& rd_min_sq, int32_t& acx, int32_t& acz) {
     min_size = spec.min_size;
     max_size = spec.max_size;
@@ -339,7 +312,7 @@ void plan_plain(const SearchSpec& spec, int32_t x0, int64_t w, int32_t z0, int64
         p.tile_h = th;
         p.tiles_x = tiles_x;
         p.tiles_total = (int32_t)(tiles_x * ((rows + th - 1) / th));
-        fill_filters(spec, p.min_size, p.max_size, p.use_annulus, p.rd_min_sq, p.acx, p.acz);
+        fill_plain_filters(spec, p.min_size, p.max_size, p.use_annulus, p.rd_min_sq, p.acx, p.acz);
         wk.centers = rows * w;
         out.push_back(wk);
     }
@@ -357,96 +330,41 @@ void plan_plain_rows(const SearchSpec& spec, int32_t x0, int64_t w, std::deque<W
     }
 }
 
-// Banded launches over every center of the spec.
-//
-// Along x: a region wider than P pairs columns P apart (two phases): whole 2P
-// column blocks from x0 form the main part; a remainder becomes one block
-// aligned to the right edge whose tiles start at the first slot column that
-// reaches a new center, with the already covered centers masked out; a region
-// between P and 2P wide is one block whose phase-1 columns past the edge are
-// masked out. A region at most P wide is scanned with one phase.
-// Along z: rows P apart form bands, grouped by at most g_max_bands per launch.
-// Tile height: the largest (up to kBandTileH) that still gives every CU a few
-// tiles of the plan; small regions trade halo rows for occupancy.
+// Banded launches over every center of the spec: two x-phases above P
+// columns, one below (x_parts); bands grouped evenly by at most g_max_bands
+// per launch. Tile height: the largest (up to kBandTileH) that still gives
+// every CU a few tiles of the plan; small regions trade halo rows for
+// occupancy.
 void plan_banded(const SearchSpec& spec, std::deque<Work>& out) {
-    const int64_t h = spec.h, W = spec.w;
-    struct XPart {
-        int32_t x0, w, ox_min, slot_lo, nblocks, phase_cols, ph;
-        int64_t centers_per_row;
-    };
-    std::vector<XPart> parts;
-    if (W > kPeriod) {
-        const int64_t nfull = W / (2 * kPeriod), rem = W % (2 * kPeriod);
-        if (nfull == 0) {
-            parts.push_back(XPart{spec.x0, (int32_t)W, 0, 0, 1, kPeriod, 2, W});
-        } else {
-            parts.push_back(XPart{spec.x0, (int32_t)(nfull * 2 * kPeriod), 0, 0, (int32_t)nfull,
-                                  kPeriod, 2, nfull * 2 * kPeriod});
-            if (rem > 0)
-                parts.push_back(XPart{(int32_t)((uint32_t)spec.x0 + (uint32_t)(W - 2 * kPeriod)),
-                                      2 * kPeriod, (int32_t)(2 * kPeriod - rem),
-                                      (int32_t)std::max<int64_t>(0, kPeriod - rem), 1, kPeriod, 2,
-                                      rem});
-        }
-    } else {
-        parts.push_back(XPart{spec.x0, (int32_t)W, 0, 0, 1, (int32_t)W, 1, W});
+    const std::vector<XPart> parts = x_parts(spec, spec.w > kPeriod ? 2 : 1);
+    const int64_t nbz = bands_of(spec);
+    const int64_t ngroups = (nbz + g_max_bands - 1) / g_max_bands;
+    std::vector<int> groups;
+    for (int64_t gi = 0, left = nbz; gi < ngroups; ++gi) {
+        const int64_t nb = (left + (ngroups - gi) - 1) / (ngroups - gi);
+        groups.push_back((int)nb);
+        left -= nb;
     }
-    const int64_t nbz = (h + kPeriod - 1) / kPeriod;
-    const int64_t groups = (nbz + g_max_bands - 1) / g_max_bands;
-    int64_t slot_rows64 = std::min<int64_t>(h, kPeriod);
-    if (spec.band_rows > 0) slot_rows64 = std::min<int64_t>(slot_rows64, spec.band_rows);
-    const int32_t slot_rows = (int32_t)slot_rows64;
+    const int32_t slot_rows = slot_rows_of(spec);
     int64_t col_tiles = 0;  // x-tiles of one slot row over all parts and groups
-    for (const XPart& xp : parts)
-        col_tiles += (int64_t)xp.nblocks * ((xp.phase_cols - xp.slot_lo + kBandTileOut - 1) / kBandTileOut);
-    col_tiles *= groups;
+    for (const XPart& xp : parts) col_tiles += (int64_t)xp.nblocks * xp.tiles_per_block();
+    col_tiles *= ngroups;
     const int64_t want_tiles = 4 * (int64_t)g_buf.cus * 3;
     const int64_t fit_h = (slot_rows * col_tiles + want_tiles - 1) / want_tiles;
     const int32_t tile_h = (int32_t)std::min<int64_t>(
         std::min<int64_t>(kBandTileH, (slot_rows + 3) / 4 * 4),
         std::max<int64_t>(kMinBandTileH, (fit_h + 3) / 4 * 4));
-    int64_t b_first = 0;
-    for (int64_t gi = 0; gi < groups; ++gi) {
-        const int64_t nb = (nbz - b_first + (groups - gi) - 1) / (groups - gi);
-        for (const XPart& xp : parts) {
-            const int32_t tiles_per_block =
-                (xp.phase_cols - xp.slot_lo + kBandTileOut - 1) / kBandTileOut;
-            // slot rows per launch: about kBandCentersPerLaunch centers
-            int64_t rows = kBandCentersPerLaunch / std::max<int64_t>(1, nb * xp.centers_per_row);
-            rows = std::max<int64_t>(tile_h, rows / tile_h * tile_h);
-            for (int64_t s = 0; s < slot_rows; s += rows) {
-                Work wk;
-                wk.banded = true;
-                wk.nb = (int)nb;
-                wk.ph = xp.ph;
-                BandParams& p = wk.band;
-                p.seed = spec.seed;
-                p.x0 = xp.x0;
-                p.w = xp.w;
-                p.ox_min = xp.ox_min;
-                p.slot_lo = xp.slot_lo;
-                p.phase_cols = xp.phase_cols;
-                p.tiles_per_block = tiles_per_block;
-                p.z0 = (int32_t)((uint32_t)spec.z0 + (uint32_t)(b_first * kPeriod));
-                p.h_valid = (int32_t)std::min<int64_t>(h - b_first * kPeriod, nb * kPeriod);
-                p.s_begin = (int32_t)s;
-                p.s_end = (int32_t)std::min<int64_t>(slot_rows, s + rows);
-                p.tile_h = tile_h;
-                p.tiles_x = xp.nblocks * tiles_per_block;
-                p.tiles_total = p.tiles_x * ((p.s_end - p.s_begin + tile_h - 1) / tile_h);
-                p.tag = 0;
-                fill_filters(spec, p.min_size, p.max_size, p.use_annulus, p.rd_min_sq, p.acx,
-                             p.acz);
-                int64_t c = 0;
-                for (int64_t b = 0; b < nb; ++b) {
-                    const int64_t hi = std::min<int64_t>(p.s_end, p.h_valid - b * kPeriod);
-                    if (hi > p.s_begin) c += (hi - p.s_begin) * xp.centers_per_row;
-                }
-                wk.centers = c;
-                out.push_back(wk);
-            }
-        }
-        b_first += nb;
+    std::vector<BandJob> jobs;
+    band_jobs(spec, parts, groups, tile_h, kBandCentersPerLaunch, jobs);
+    for (const BandJob& j : jobs) {
+        Work wk;
+        wk.banded = true;
+        wk.nb = j.nb;
+        wk.ph = j.ph;
+        wk.band = j.p;
+        wk.band.tag = 0;
+        wk.centers = j.centers;
+        out.push_back(wk);
     }
 }
 
@@ -470,40 +388,14 @@ bool prepare_fixes(const SearchSpec& spec, MismatchIndex& index, Ranker& rk,
     if (count && (hipMemcpy(mm.data(), g_buf.d_mismatch, sizeof(MismatchChunk) * count,
                             hipMemcpyDeviceToHost) != hipSuccess))
         return false;
-    std::sort(mm.begin(), mm.end(), [](const MismatchChunk& a, const MismatchChunk& b) {
-        return a.z != b.z ? a.z < b.z : a.x < b.x;
-    });
-    const int64_t rmin_sq = spec.rd_min * spec.rd_min;
-    fixes.clear();
-    for (size_t i = 0; i < mm.size(); ++i) {
-        const MismatchChunk& m = mm[i];
-        // earlier mismatches close enough to share a center already added it
-        size_t first_near = i;
-        while (first_near > 0 && (int64_t)mm[first_near - 1].z >= (int64_t)m.z - 16) --first_near;
-        for (int dz = -8; dz <= 8; ++dz) {
-            const int hw = disk_half_width(dz < 0 ? -dz : dz);
-            for (int dx = -hw; dx <= hw; ++dx) {
-                const int64_t cx = (int64_t)m.x + dx, cz = (int64_t)m.z + dz;
-                const int64_t ox = cx - spec.x0, oz = cz - spec.z0;
-                if (ox < 0 || ox >= spec.w || oz < 0 || oz >= spec.h) continue;
-                if (spec.band_rows > 0 && oz % kPeriod >= spec.band_rows) continue;
-                if (spec.rd_min > 0) {
-                    const int64_t ax = cx - spec.acx, az = cz - spec.acz;
-                    if (ax * ax + az * az < rmin_sq) continue;
-                }
-                bool seen = false;
-                for (size_t j = first_near; j < i && !seen; ++j)
-                    seen = MismatchIndex::in_disk((int64_t)mm[j].x - cx, (int64_t)mm[j].z - cz);
-                if (!seen) fixes.push_back(CenterFix{(int32_t)cx, (int32_t)cz, 0, 0});
-            }
-        }
-    }
+    index.sort();
+    collect_fixes(spec, index, fixes);
     rk.recount = &index;
     if (fixes.empty()) return true;
     if (!ensure_fixes(fixes.size()) ||
         hipMemcpyAsync(g_buf.d_fixes, fixes.data(), sizeof(CenterFix) * fixes.size(),
                        hipMemcpyHostToDevice, g_buf.stream) != hipSuccess ||
-        launch_recount(spec.seed, g_buf.d_fixes, (int32_t)fixes.size(), g_buf.stream) != hipSuccess ||
+        launch_recount(g_buf.d_fixes, (int32_t)fixes.size(), g_buf.stream) != hipSuccess ||
         hipMemcpyAsync(fixes.data(), g_buf.d_fixes, sizeof(CenterFix) * fixes.size(),
                        hipMemcpyDeviceToHost, g_buf.stream) != hipSuccess ||
         hipStreamSynchronize(g_buf.stream) != hipSuccess)
@@ -615,22 +507,7 @@ int64_t run_search(const SearchSpec& spec, ResultRow* out, int32_t max_results,
     int64_t total = 0;
     for (int s = 0; s < kHistBins; ++s) total += rk.hist[s];
     if (hist_out) std::memcpy(hist_out, rk.hist, sizeof(rk.hist));
-    std::vector<Hit>& cand = rk.cand;
-    const size_t n = std::min(cand.size(), rk.keep);
-    if (n < cand.size())
-        std::partial_sort(cand.begin(), cand.begin() + n, cand.end(), ranks_before);
-    else
-        std::sort(cand.begin(), cand.end(), ranks_before);
-    for (size_t i = 0; i < n; ++i) {
-        const int64_t bx = (int64_t)cand[i].cx * 16 + 8;
-        const int64_t bz = (int64_t)cand[i].cz * 16 + 8;
-        out[i].size = cand[i].size;
-        out[i].center_x = (int32_t)(uint32_t)(uint64_t)bx;
-        out[i].center_z = (int32_t)(uint32_t)(uint64_t)bz;
-        out[i].obs_count = 0;
-        out[i].afk_x = out[i].center_x;
-        out[i].afk_z = out[i].center_z;
-    }
+    write_rows(rk.cand, rk.keep, out);
     ctl.progress.store(90, std::memory_order_relaxed);
     return total;
 }

@@ -44,9 +44,38 @@ __global__ __launch_bounds__(256) void find_mismatches_kernel(
     }
 }
 
-__global__ __launch_bounds__(256) void recount_kernel(int64_t seed, CenterFix* fixes, int32_t n) {
+__global__ __launch_bounds__(256) void find_mismatches_multi_kernel(
+    const MismatchQuery* q, int32_t nq, int64_t total_rows, const uint64_t* table, int32_t n,
+    TaggedMismatch* out, uint32_t cap, unsigned int* count) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total_rows; i += stride) {
+        int32_t lo = 0, hi = nq - 1;  // last query with row_begin <= i
+        while (lo < hi) {
+            const int32_t mid = (lo + hi + 1) >> 1;
+            if (q[mid].row_begin <= i) lo = mid; else hi = mid - 1;
+        }
+        const MismatchQuery& qq = q[lo];
+        const int64_t r = i - qq.row_begin;
+        if (r < 0 || r >= qq.rows) continue;
+        const int32_t z = (int32_t)((uint32_t)qq.z_first + (uint32_t)r);
+        int32_t xs[8];
+        const int found = row_table_hits(qq.seed, z, table, n, qq.x_first, qq.cols, xs, 8);
+        if (found == 0) continue;
+        if (found > 8) {  // never seen (~6 candidates per row, far fewer in range)
+            atomicAdd(count, (unsigned int)(cap + 1));
+            continue;
+        }
+        for (int f = 0; f < found; ++f) {
+            const unsigned int slot = atomicAdd(count, 1u);
+            if (slot < cap) out[slot] = TaggedMismatch{xs[f], z, qq.tag};
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void recount_kernel(CenterFix* fixes, int32_t n) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const int64_t seed = fixes[i].seed;
     const int32_t cx = fixes[i].cx, cz = fixes[i].cz;
     int32_t fd = 0, exact = 0;
     for (int dz = -8; dz <= 8; ++dz) {
@@ -83,9 +112,20 @@ hipError_t launch_find_mismatches(int64_t seed, int32_t x_first, int64_t cols, i
     return hipGetLastError();
 }
 
-hipError_t launch_recount(int64_t seed, CenterFix* fixes, int32_t n, hipStream_t stream) {
+hipError_t launch_find_mismatches_multi(const MismatchQuery* q, int32_t nq, int64_t total_rows,
+                                        const uint64_t* table, int32_t table_n,
+                                        TaggedMismatch* out, uint32_t cap, unsigned int* count,
+                                        hipStream_t stream) {
+    if (nq <= 0 || total_rows <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((total_rows + 255) / 256, 65536);
+    hipLaunchKernelGGL(find_mismatches_multi_kernel, dim3((unsigned)blocks), dim3(256), 0, stream,
+                       q, nq, total_rows, table, table_n, out, cap, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_recount(CenterFix* fixes, int32_t n, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(recount_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, seed, fixes, n);
+    hipLaunchKernelGGL(recount_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, fixes, n);
     return hipGetLastError();
 }
 

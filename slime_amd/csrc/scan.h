@@ -82,6 +82,51 @@ struct BandParams {
 // One workgroup per tile; nb bands (1..kMaxBands), ph x-phases (1 or 2).
 hipError_t launch_scan_band(const BandParams& p, int nb, int ph, hipStream_t stream);
 
+// ---- multi-search sweep -----------------------------------------------------
+//
+// Jobs of many searches (each a BandParams with its own outputs) scanned by
+// one persistent launch per kernel class: every workgroup claims tiles from
+// per-job cursors in the given job order, skipping jobs whose request is not
+// active, until the order is exhausted or the launch's claim budget is spent.
+struct SweepClass {
+    int nb, ph;
+};
+constexpr int kSweepClasses = 11;
+constexpr SweepClass kSweepClass[kSweepClasses] = {{1, 1}, {2, 1}, {3, 1}, {4, 1},
+                                                   {1, 2}, {2, 2}, {3, 2}, {4, 2},
+                                                   {6, 2}, {8, 2}, {12, 2}};
+constexpr int kSweepMaxBands = 12;
+// Class index of (nb, ph), or -1.
+inline int sweep_class_of(int nb, int ph) {
+    for (int c = 0; c < kSweepClasses; ++c)
+        if (kSweepClass[c].nb == nb && kSweepClass[c].ph == ph) return c;
+    return -1;
+}
+
+constexpr int32_t kReqActive = 1;  // request status: tiles may be claimed
+
+struct SweepJob {
+    BandParams p;
+    int32_t cls;  // index into kSweepClass
+    int32_t req;  // request slot whose status gates the job
+};
+struct SweepArgs {
+    const SweepJob* jobs;
+    const int32_t* order;   // job indices in claim order
+    int32_t norder;
+    unsigned int* cursor;   // per job: tiles claimed so far
+    unsigned int* done;     // per job: tiles finished so far
+    const int32_t* status;  // per request slot
+    unsigned int* claims;   // claim attempts in this launch (cleared before it)
+    uint32_t budget;        // claim attempts allowed in this launch
+};
+// Workgroups of class cls's sweep kernel that fit one CU at once.
+int sweep_blocks_per_cu(int cls);
+// The persistent kernel of class cls over a.order (jobs of that class only).
+hipError_t launch_sweep(const SweepArgs& a, int cls, int grid, hipStream_t stream);
+// Sets n words to v with a kernel (usable inside a captured graph).
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t stream);
+
 // ---- first-draw mismatches (Java's nextInt(10) rejection) ------------------
 //
 // The banded scan counts first draws (next(31) % 10 == 0). It differs from
@@ -92,7 +137,11 @@ hipError_t launch_scan_band(const BandParams& p, int nb, int ph, hipStream_t str
 struct MismatchChunk {
     int32_t x, z;
 };
+// Mismatch chunks one search may hold (about 7.5e-10 per chunk: ~10^4 in a
+// full world).
+constexpr uint32_t kMismatchCap = 1u << 22;
 struct CenterFix {
+    int64_t seed;
     int32_t cx, cz;
     int32_t first_draw;  // 221-cell count from first draws (what the scan saw)
     int32_t exact;       // Java's count
@@ -106,7 +155,23 @@ hipError_t launch_find_mismatches(int64_t seed, int32_t x_first, int64_t cols, i
                                   int64_t rows, const uint64_t* table, int32_t table_n,
                                   MismatchChunk* out, uint32_t cap, unsigned int* count,
                                   hipStream_t stream);
-// Fills first_draw and exact of n centers.
-hipError_t launch_recount(int64_t seed, CenterFix* fixes, int32_t n, hipStream_t stream);
+// Mismatch search of many regions at once: query q covers rows
+// [row_begin, row_begin + rows) of a virtual row list (ascending row_begin).
+struct MismatchQuery {
+    int64_t seed;
+    int32_t x_first, z_first;
+    int64_t cols, rows;
+    int64_t row_begin;
+    int32_t tag;
+};
+struct TaggedMismatch {
+    int32_t x, z, tag;
+};
+hipError_t launch_find_mismatches_multi(const MismatchQuery* q, int32_t nq, int64_t total_rows,
+                                        const uint64_t* table, int32_t table_n,
+                                        TaggedMismatch* out, uint32_t cap, unsigned int* count,
+                                        hipStream_t stream);
+// Fills first_draw and exact of n centers (each with its own seed).
+hipError_t launch_recount(CenterFix* fixes, int32_t n, hipStream_t stream);
 
 }  // namespace slime

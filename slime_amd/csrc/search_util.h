@@ -1,0 +1,235 @@
+// Host-side pieces of a search shared by the per-search engine and the
+// multi-search sweep: band-job geometry, the first-draw fix list, ranking.
+#pragma once
+
+#include <stdint.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "engine.h"
+#include "scan.h"
+#include "slime_math.h"
+
+namespace slime {
+
+// Largest score s with at least k counted hits of score >= s; `floor` if none.
+inline int32_t gate_for(const int64_t* hist, int64_t k, int32_t floor) {
+    int64_t acc = 0;
+    for (int32_t s = kHistBins - 1; s >= 0; --s) {
+        acc += std::max<int64_t>(0, hist[s]);
+        if (acc >= k) return std::max(s, floor);
+    }
+    return floor;
+}
+
+// The first-draw mismatch chunks of one search, sorted by (z, x).
+struct MismatchIndex {
+    std::vector<MismatchChunk> mm;
+
+    static bool in_disk(int64_t dx, int64_t dz) { return dx * dx + dz * dz <= 68; }
+
+    void sort() {
+        std::sort(mm.begin(), mm.end(), [](const MismatchChunk& a, const MismatchChunk& b) {
+            return a.z != b.z ? a.z < b.z : a.x < b.x;
+        });
+    }
+
+    // Does the disk of center (cx, cz) contain a mismatch chunk?
+    bool affects(int32_t cx, int32_t cz) const {
+        if (mm.empty()) return false;
+        auto it = std::lower_bound(mm.begin(), mm.end(), (int64_t)cz - 8,
+                                   [](const MismatchChunk& m, int64_t z) { return m.z < z; });
+        for (; it != mm.end() && (int64_t)it->z <= (int64_t)cz + 8; ++it)
+            if (in_disk((int64_t)it->x - cx, (int64_t)it->z - cz)) return true;
+        return false;
+    }
+};
+
+// Every center of the spec whose disk holds a mismatch chunk (index sorted),
+// once each, as fixes to recount.
+inline void collect_fixes(const SearchSpec& spec, const MismatchIndex& index,
+                          std::vector<CenterFix>& fixes) {
+    const std::vector<MismatchChunk>& mm = index.mm;
+    const int64_t rmin_sq = spec.rd_min * spec.rd_min;
+    fixes.clear();
+    for (size_t i = 0; i < mm.size(); ++i) {
+        const MismatchChunk& m = mm[i];
+        // earlier mismatches close enough to share a center already added it
+        size_t first_near = i;
+        while (first_near > 0 && (int64_t)mm[first_near - 1].z >= (int64_t)m.z - 16) --first_near;
+        for (int dz = -8; dz <= 8; ++dz) {
+            const int hw = disk_half_width(dz < 0 ? -dz : dz);
+            for (int dx = -hw; dx <= hw; ++dx) {
+                const int64_t cx = (int64_t)m.x + dx, cz = (int64_t)m.z + dz;
+                const int64_t ox = cx - spec.x0, oz = cz - spec.z0;
+                if (ox < 0 || ox >= spec.w || oz < 0 || oz >= spec.h) continue;
+                if (spec.band_rows > 0 && oz % kPeriod >= spec.band_rows) continue;
+                if (spec.rd_min > 0) {
+                    const int64_t ax = cx - spec.acx, az = cz - spec.acz;
+                    if (ax * ax + az * az < rmin_sq) continue;
+                }
+                bool seen = false;
+                for (size_t j = first_near; j < i && !seen; ++j)
+                    seen = MismatchIndex::in_disk((int64_t)mm[j].x - cx, (int64_t)mm[j].z - cz);
+                if (!seen)
+                    fixes.push_back(CenterFix{spec.seed, (int32_t)cx, (int32_t)cz, 0, 0});
+            }
+        }
+    }
+}
+
+// Ranks the candidates (size desc, x asc, z asc) and writes the best `keep`
+// as reference rows (block coordinates cx*16+8, precise_afk == 0 columns).
+inline size_t write_rows(std::vector<Hit>& cand, size_t keep, ResultRow* out) {
+    const size_t n = std::min(cand.size(), keep);
+    if (n < cand.size())
+        std::partial_sort(cand.begin(), cand.begin() + n, cand.end(), ranks_before);
+    else
+        std::sort(cand.begin(), cand.end(), ranks_before);
+    for (size_t i = 0; i < n; ++i) {
+        const int64_t bx = (int64_t)cand[i].cx * 16 + 8;
+        const int64_t bz = (int64_t)cand[i].cz * 16 + 8;
+        out[i].size = cand[i].size;
+        out[i].center_x = (int32_t)(uint32_t)(uint64_t)bx;
+        out[i].center_z = (int32_t)(uint32_t)(uint64_t)bz;
+        out[i].obs_count = 0;
+        out[i].afk_x = out[i].center_x;
+        out[i].afk_z = out[i].center_z;
+    }
+    return n;
+}
+
+inline void fill_filters(const SearchSpec& spec, BandParams& p) {
+    p.min_size = spec.min_size;
+    p.max_size = spec.max_size;
+    p.emit_min = spec.min_size;
+    p.use_annulus = spec.rd_min > 0;
+    p.rd_min_sq = spec.rd_min * spec.rd_min;
+    p.acx = spec.acx;
+    p.acz = spec.acz;
+}
+
+// ---- band-job geometry -------------------------------------------------------
+//
+// Along x, two phases (ph = 2, only for regions wider than P): whole 2P column
+// blocks from x0 form the main part; a remainder becomes one block aligned to
+// the right edge whose tiles start at the first slot column that reaches a new
+// center, with the already covered centers masked out; a region between P and
+// 2P wide is one block whose phase-1 columns past the edge are masked out.
+// One phase (ph = 1): a single part spanning the region.
+struct XPart {
+    int32_t x0, w, ox_min, slot_lo, nblocks, phase_cols, ph;
+    int64_t centers_per_row;
+
+    int32_t tiles_per_block() const {
+        return (phase_cols - slot_lo + kBandTileOut - 1) / kBandTileOut;
+    }
+};
+
+inline std::vector<XPart> x_parts(const SearchSpec& spec, int ph) {
+    const int64_t W = spec.w;
+    std::vector<XPart> parts;
+    if (ph == 2) {
+        const int64_t nfull = W / (2 * kPeriod), rem = W % (2 * kPeriod);
+        if (nfull == 0) {
+            parts.push_back(XPart{spec.x0, (int32_t)W, 0, 0, 1, kPeriod, 2, W});
+        } else {
+            parts.push_back(XPart{spec.x0, (int32_t)(nfull * 2 * kPeriod), 0, 0, (int32_t)nfull,
+                                  kPeriod, 2, nfull * 2 * kPeriod});
+            if (rem > 0)
+                parts.push_back(XPart{(int32_t)((uint32_t)spec.x0 + (uint32_t)(W - 2 * kPeriod)),
+                                      2 * kPeriod, (int32_t)(2 * kPeriod - rem),
+                                      (int32_t)std::max<int64_t>(0, kPeriod - rem), 1, kPeriod, 2,
+                                      rem});
+        }
+    } else {
+        parts.push_back(XPart{spec.x0, (int32_t)W, 0, 0, 1, (int32_t)W, 1, W});
+    }
+    return parts;
+}
+
+// Slot rows per band: min(h, P), or the band_rows of a strided band set.
+inline int32_t slot_rows_of(const SearchSpec& spec) {
+    int64_t r = std::min<int64_t>(spec.h, kPeriod);
+    if (spec.band_rows > 0) r = std::min<int64_t>(r, spec.band_rows);
+    return (int32_t)r;
+}
+
+inline int64_t bands_of(const SearchSpec& spec) { return (spec.h + kPeriod - 1) / kPeriod; }
+
+struct BandJob {
+    BandParams p;
+    int nb, ph;
+    int64_t centers;  // centers this job decides
+};
+
+// Jobs of one x-part and one group of nb consecutive bands from band b_first,
+// each over slot rows [0, s_rows), cut into slot-row ranges of about
+// max_centers centers. Output pointers are left null.
+inline void band_jobs_part(const SearchSpec& spec, const XPart& xp, int64_t b_first, int nb,
+                           int32_t s_rows, int32_t tile_h, int64_t max_centers,
+                           std::vector<BandJob>& out) {
+    const int32_t tpb = xp.tiles_per_block();
+    int64_t rows = max_centers / std::max<int64_t>(1, nb * xp.centers_per_row);
+    rows = std::max<int64_t>(tile_h, rows / tile_h * tile_h);
+    for (int64_t s = 0; s < s_rows; s += rows) {
+        BandJob j{};
+        j.nb = nb;
+        j.ph = xp.ph;
+        BandParams& p = j.p;
+        p.seed = spec.seed;
+        p.x0 = xp.x0;
+        p.w = xp.w;
+        p.ox_min = xp.ox_min;
+        p.slot_lo = xp.slot_lo;
+        p.phase_cols = xp.phase_cols;
+        p.tiles_per_block = tpb;
+        p.z0 = (int32_t)((uint32_t)spec.z0 + (uint32_t)(b_first * kPeriod));
+        p.h_valid = (int32_t)std::min<int64_t>(spec.h - b_first * kPeriod, (int64_t)nb * kPeriod);
+        p.s_begin = (int32_t)s;
+        p.s_end = (int32_t)std::min<int64_t>(s_rows, s + rows);
+        p.tile_h = tile_h;
+        p.tiles_x = xp.nblocks * tpb;
+        p.tiles_total = p.tiles_x * ((p.s_end - p.s_begin + tile_h - 1) / tile_h);
+        fill_filters(spec, p);
+        int64_t c = 0;
+        for (int64_t b = 0; b < nb; ++b) {
+            const int64_t hi = std::min<int64_t>(p.s_end, p.h_valid - b * kPeriod);
+            if (hi > p.s_begin) c += (hi - p.s_begin) * xp.centers_per_row;
+        }
+        j.centers = c;
+        out.push_back(j);
+    }
+}
+
+// Jobs over every center of the spec: for each group of consecutive bands
+// (sizes in `groups`, summing to bands_of(spec)) and each x-part, all slot
+// rows of the spec.
+inline void band_jobs(const SearchSpec& spec, const std::vector<XPart>& parts,
+                      const std::vector<int>& groups, int32_t tile_h, int64_t max_centers,
+                      std::vector<BandJob>& out) {
+    const int32_t slot_rows = slot_rows_of(spec);
+    int64_t b_first = 0;
+    for (const int nb : groups) {
+        for (const XPart& xp : parts)
+            band_jobs_part(spec, xp, b_first, nb, slot_rows, tile_h, max_centers, out);
+        b_first += nb;
+    }
+}
+
+// Two phases over the whole 2P blocks from x0 and one phase over the
+// remaining columns (for regions wider than 2P).
+inline std::vector<XPart> x_parts_one_phase_tail(const SearchSpec& spec) {
+    const int64_t W = spec.w;
+    const int64_t nfull = W / (2 * kPeriod), rem = W % (2 * kPeriod);
+    std::vector<XPart> parts;
+    parts.push_back(XPart{spec.x0, (int32_t)(nfull * 2 * kPeriod), 0, 0, (int32_t)nfull, kPeriod,
+                          2, nfull * 2 * kPeriod});
+    if (rem > 0)
+        parts.push_back(XPart{(int32_t)((uint32_t)spec.x0 + (uint32_t)(nfull * 2 * kPeriod)),
+                              (int32_t)rem, 0, 0, 1, (int32_t)rem, 1, rem});
+    return parts;
+}
+
+}  // namespace slime

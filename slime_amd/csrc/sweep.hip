@@ -1,0 +1,161 @@
+// Persistent multi-search sweep: tiles of many banded jobs claimed from a
+// device work queue by a fixed grid of workgroups.
+//
+// The queue is an ordered job list with one claim cursor per job. Wave 0 of a
+// workgroup looks at 64 jobs at once from its current position, takes the
+// first one whose request is active and whose cursor is below its tile count,
+// and claims a tile with one atomic add; positions only move forward, so
+// every workgroup exits once it has passed the end of the list (or the
+// launch's claim budget is spent). There is one kernel per class (bands,
+// x-phases); its tiles run the same code as the per-search kernels
+// (band_tile.h).
+#include "band_tile.h"
+
+namespace slime {
+namespace {
+
+using band::kThreads;
+
+// Claims one tile for the workgroup (called by wave 0, uniform result).
+__device__ bool claim_tile(const SweepArgs& a, int& pos, int& job, int& tile) {
+    const int lane = threadIdx.x & 63;
+    while (pos < a.norder) {
+        const int idx = pos + lane;
+        int j = -1;
+        bool open = false;
+        if (idx < a.norder) {
+            j = a.order[idx];
+            const SweepJob& jb = a.jobs[j];
+            open = a.status[jb.req] == kReqActive &&
+                   __hip_atomic_load(&a.cursor[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                       (unsigned int)jb.p.tiles_total;
+        }
+        const uint64_t m = __ballot(open);
+        if (m == 0) {
+            pos += 64;
+            continue;
+        }
+        const int first = __builtin_ctzll(m);
+        pos += first;  // the jobs before it are finished or not active
+        unsigned int t = 0;
+        if (lane == first) t = atomicAdd(&a.cursor[j], 1u);
+        t = __shfl(t, first);
+        const int jj = __shfl(j, first);
+        if (t < (unsigned int)a.jobs[jj].p.tiles_total) {
+            job = jj;
+            tile = (int)t;
+            return true;
+        }
+        pos += 1;  // another workgroup took the last tile
+    }
+    return false;
+}
+
+// The persistent kernel of one class: a workgroup claims a tile, scans it,
+// flushes its histogram and claims the next, until the queue or the budget is
+// exhausted. One kernel per class keeps each class's register allocation
+// (a runtime switch over all classes in one kernel spills).
+// Waves per SIMD each class is built for: the occupancy its per-search kernel
+// reaches with the same tile code (fewer bands need fewer registers).
+constexpr int kSweepWaves[kSweepClasses] = {6, 5, 5, 5, 5, 5, 4, 4, 4, 3, 3};
+
+template <int C>
+__global__ __launch_bounds__(kThreads, kSweepWaves[C]) void sweep_kernel(SweepArgs a) {
+    constexpr int NB = kSweepClass[C].nb, PH = kSweepClass[C].ph;
+    __shared__ band::TileLds<NB> lds;
+    __shared__ int s_job, s_tile;
+    for (int i = threadIdx.x; i < kHistBins; i += kThreads) lds.lhist[i] = 0;
+    int pos = 0;
+    for (;;) {
+        if (threadIdx.x < 64) {
+            unsigned int c = 0;
+            if (threadIdx.x == 0) c = atomicAdd(a.claims, 1u);
+            c = __shfl(c, 0);
+            int job = -1, tile = 0;
+            const bool ok = c < a.budget && claim_tile(a, pos, job, tile);
+            if (threadIdx.x == 0) {
+                s_job = ok ? job : -1;
+                s_tile = tile;
+            }
+        }
+        __syncthreads();
+        const int job = __builtin_amdgcn_readfirstlane(s_job);
+        const int tile = __builtin_amdgcn_readfirstlane(s_tile);
+        if (job < 0) break;
+        // read in place (scalar loads as needed) rather than copied into registers
+        const BandParams& p = a.jobs[job].p;
+        band::band_tile<NB, PH>(p, p.emit_min, tile, lds);
+        band::flush_hist(lds.lhist, p.hist);
+        if (threadIdx.x == 0) atomicAdd(&a.done[job], 1u);
+        __syncthreads();
+    }
+}
+
+template <int C>
+hipError_t launch_class(const SweepArgs& a, int grid, hipStream_t stream) {
+    hipLaunchKernelGGL(sweep_kernel<C>, dim3(grid), dim3(kThreads), 0, stream, a);
+    return hipGetLastError();
+}
+
+template <int C>
+int blocks_per_cu() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sweep_kernel<C>, kThreads, 0) !=
+            hipSuccess ||
+        n < 1)
+        return 1;
+    return n;
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+}  // namespace
+
+int sweep_blocks_per_cu(int cls) {
+    switch (cls) {
+        case 0: return blocks_per_cu<0>();
+        case 1: return blocks_per_cu<1>();
+        case 2: return blocks_per_cu<2>();
+        case 3: return blocks_per_cu<3>();
+        case 4: return blocks_per_cu<4>();
+        case 5: return blocks_per_cu<5>();
+        case 6: return blocks_per_cu<6>();
+        case 7: return blocks_per_cu<7>();
+        case 8: return blocks_per_cu<8>();
+        case 9: return blocks_per_cu<9>();
+        case 10: return blocks_per_cu<10>();
+        default: return 1;
+    }
+}
+
+hipError_t launch_sweep(const SweepArgs& a, int cls, int grid, hipStream_t stream) {
+    if (a.norder <= 0 || grid <= 0) return hipSuccess;
+    switch (cls) {
+        case 0: return launch_class<0>(a, grid, stream);
+        case 1: return launch_class<1>(a, grid, stream);
+        case 2: return launch_class<2>(a, grid, stream);
+        case 3: return launch_class<3>(a, grid, stream);
+        case 4: return launch_class<4>(a, grid, stream);
+        case 5: return launch_class<5>(a, grid, stream);
+        case 6: return launch_class<6>(a, grid, stream);
+        case 7: return launch_class<7>(a, grid, stream);
+        case 8: return launch_class<8>(a, grid, stream);
+        case 9: return launch_class<9>(a, grid, stream);
+        case 10: return launch_class<10>(a, grid, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+static_assert(kSweepClasses == 11, "one case per class");
+
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, p, n, v);
+    return hipGetLastError();
+}
+
+}  // namespace slime

@@ -1,0 +1,158 @@
+// Multi-search sweep: device-resident state of many searches scanned by the
+// persistent per-class kernels (batched calls and the served session).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <vector>
+
+#include "engine.h"
+#include "search_util.h"
+
+namespace slime {
+
+// Measured cost of each kernel class: device ns per tile row at full
+// occupancy (the whole GPU busy with that class) and a fixed per-tile cost
+// in rows.
+struct CostTable {
+    bool ready = false;
+    double ns_row[kSweepClasses] = {};
+    double extra_rows[kSweepClasses] = {};
+};
+
+// Host state of one search in the sweeper.
+struct SweepRequest {
+    SearchSpec spec;
+    int32_t keep = 0;  // max_results
+    // plan
+    std::vector<BandJob> jobs;
+    std::vector<int32_t> job_cls;
+    int64_t centers = 0;
+    double est_ns = 0;   // modelled device time of the whole request
+    int64_t hit_cap = 0;
+    // reservation
+    int32_t slot = -1;
+    int64_t job_off = -1, hit_off = -1;
+    // progress
+    int32_t status = 0;  // mirror of the device status word
+    std::vector<uint32_t> done;
+    double remaining_ns = 0;
+    bool complete = false;
+    // first-draw fixes
+    MismatchIndex mm;
+    std::vector<CenterFix> fixes;
+    // result
+    int64_t total = -1;
+    bool fallback = false;
+    std::vector<ResultRow> rows;
+};
+
+struct RoundPolicy {
+    int64_t round_ns = 0;         // device time per class launch (0: unbounded)
+    bool shortest_first = false;  // claim order: least work left first (else largest tiles first)
+    bool use_graph = false;       // replay the launch sequence from a captured hipGraph
+};
+
+struct RoundStats {
+    int64_t kernel_ns = 0;
+    int32_t launches = 0;
+    int32_t rounds = 0;
+};
+
+class Sweeper {
+   public:
+    ~Sweeper();
+    bool init(int32_t req_cap, int64_t job_cap, int64_t hit_cap);
+    void release();
+    bool ready() const { return device_ >= 0; }
+    int32_t req_cap() const { return req_cap_; }
+    int64_t job_cap() const { return job_cap_; }
+    int64_t hit_cap() const { return hit_cap_; }
+    int32_t free_slots() const { return (int32_t)free_slots_.size(); }
+    const CostTable& costs() const { return costs_; }
+    void set_costs(const CostTable& c) { costs_ = c; }
+
+    bool calibrate();                         // fills the cost table (once)
+    double tile_ns(int cls, int32_t tile_h) const;
+    bool plan(SweepRequest& r) const;         // schedule + hit capacity
+    static int64_t footprint(const SweepRequest& r);  // device bytes it holds
+    bool reserve(SweepRequest& r);            // slot, job range, hit region
+    void unreserve(SweepRequest& r);
+    bool prepare(const std::vector<SweepRequest*>& reqs);  // mismatch fixes
+    bool upload(const std::vector<SweepRequest*>& reqs);   // jobs, cleared outputs, active
+    bool set_status(SweepRequest& r, int32_t status);
+    bool clear_status();  // every request slot inactive
+    bool round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& pol, RoundStats* st);
+    double progress(const SweepRequest& r) const;
+    bool finalize(SweepRequest& r, bool partial);
+
+   private:
+    bool ensure_fix_cap(size_t n);
+
+    int device_ = -1;
+    int cus_ = 256;
+    int grid_[kSweepClasses] = {};
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    int32_t req_cap_ = 0;
+    int64_t job_cap_ = 0, hit_cap_ = 0, fix_cap_ = 0;
+    SweepJob* d_jobs_ = nullptr;
+    uint32_t* d_cursor_ = nullptr;
+    uint32_t* d_done_ = nullptr;
+    uint32_t* h_done_ = nullptr;
+    int32_t* d_order_ = nullptr;
+    int32_t* d_status_ = nullptr;
+    unsigned long long* d_hist_ = nullptr;
+    unsigned long long* d_emitted_ = nullptr;
+    Hit* d_hits_ = nullptr;
+    uint32_t* d_claims_ = nullptr;
+    uint64_t* d_table_ = nullptr;
+    int32_t table_n_ = 0;
+    MismatchQuery* d_queries_ = nullptr;
+    TaggedMismatch* d_mm_ = nullptr;
+    unsigned int* d_mm_count_ = nullptr;
+    CenterFix* d_fixes_ = nullptr;
+    CostTable costs_;
+    std::vector<int32_t> free_slots_;
+    struct Ranges {
+        std::map<int64_t, int64_t> free_;
+        void reset(int64_t n) {
+            free_.clear();
+            if (n > 0) free_[0] = n;
+        }
+        int64_t alloc(int64_t n);
+        void release(int64_t off, int64_t n);
+    };
+    Ranges jobs_alloc_, hits_alloc_;
+    std::map<std::vector<int64_t>, hipGraphExec_t> graphs_;
+};
+
+// ---- batched searches ----
+
+struct SweepOptions {
+    bool use_graph = false;
+};
+
+struct BatchStats {
+    int64_t centers = 0;    // centers decided
+    int64_t kernel_ns = 0;  // device time of the sweep launches
+    int64_t prepare_ns = 0; // planning, mismatch fixes, upload (host clock)
+    int64_t sweep_ns = 0;   // the sweep launches incl. sync (host clock)
+    int64_t wall_ns = 0;
+    int32_t launches = 0;
+    int32_t jobs = 0;
+    int32_t fallbacks = 0;  // searches rerun alone after a hit-region overflow
+};
+
+// Runs n searches in one sweep. out holds n * max_results rows (search i at
+// i * max_results); totals[i] = its hit count. Returns 0, or -1 on a device
+// error.
+int run_batch(const SearchSpec* specs, int32_t n, int32_t max_results, ResultRow* out,
+              int64_t* totals, const SweepOptions& opt, BatchStats* stats);
+void last_batch_stats(BatchStats* st);
+// The autotuner's measured table (kSweepClasses entries each).
+bool sweep_cost_table(double* ns_row, double* extra_rows);
+
+}  // namespace slime

@@ -1,0 +1,851 @@
+// Multi-search sweep on one device: batched searches (all submitted at once,
+// one persistent launch per kernel class) and the served session (continuous
+// batching in rounds) share this state machine.
+//
+// Device-resident per request: its jobs (BandParams pointing at its own
+// histogram, emit counter and hit region), per-job claim/done cursors and a
+// status word that the persistent kernels read before claiming a tile. Pause,
+// resume and cancel are writes of that word; admission reserves a request
+// slot, a job range and a hit region whose bytes count against the session's
+// HBM budget.
+//
+// The autotuner: at first use every kernel class is timed on synthetic jobs
+// at two tile heights, giving a per-class cost per tile row and a fixed cost
+// per tile. Each search's schedule (one or two x-phases, the grouping of its
+// z-bands into kernel classes) is the cheapest under that measured model; the
+// tile height is the largest that fits the region (at most 2048 slot rows).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "engine.h"
+#include "search_util.h"
+#include "sweep_engine.h"
+
+namespace slime {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+int64_t ns_since(Clock::time_point t0) {
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(Clock::now() - t0).count();
+}
+
+// P(Bin(221, 0.1) >= t): the chance that one center reaches t.
+double center_tail(int t) {
+    if (t <= 0) return 1.0;
+    double s = 0.0;
+    for (int k = t; k <= kDiskCells; ++k)
+        s += std::exp(std::lgamma(kDiskCells + 1.0) - std::lgamma(k + 1.0) -
+                      std::lgamma(kDiskCells - k + 1.0) + k * std::log(0.1) +
+                      (kDiskCells - k) * std::log(0.9));
+    return s;
+}
+
+constexpr int64_t kMaxHitCap = int64_t(1) << 24;
+constexpr int32_t kMaxTileH = 2048;
+constexpr int32_t kTileHalo = 16;  // generated rows beyond the tile's slot rows
+
+}  // namespace
+
+// ---- the sweeper -------------------------------------------------------------
+
+// First-fit allocation of index ranges (job slots, hit-region entries).
+int64_t Sweeper::Ranges::alloc(int64_t n) {
+    if (n <= 0) return 0;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+        if (it->second < n) continue;
+        const int64_t off = it->first, len = it->second;
+        free_.erase(it);
+        if (len > n) free_[off + n] = len - n;
+        return off;
+    }
+    return -1;
+}
+
+void Sweeper::Ranges::release(int64_t off, int64_t n) {
+    if (n <= 0 || off < 0) return;
+    auto it = free_.emplace(off, n).first;
+    auto next = std::next(it);
+    if (next != free_.end() && it->first + it->second == next->first) {
+        it->second += next->second;
+        free_.erase(next);
+    }
+    if (it != free_.begin()) {
+        auto prev = std::prev(it);
+        if (prev->first + prev->second == it->first) {
+            prev->second += it->second;
+            free_.erase(it);
+        }
+    }
+}
+
+Sweeper::~Sweeper() { release(); }
+
+void Sweeper::release() {
+    for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+    graphs_.clear();
+    auto f = [](void* p) {
+        if (p) (void)hipFree(p);
+    };
+    if (device_ >= 0) (void)hipSetDevice(device_);
+    f(d_jobs_);
+    f(d_cursor_);
+    f(d_done_);
+    f(d_order_);
+    f(d_status_);
+    f(d_hist_);
+    f(d_emitted_);
+    f(d_hits_);
+    f(d_claims_);
+    f(d_table_);
+    f(d_queries_);
+    f(d_mm_);
+    f(d_mm_count_);
+    f(d_fixes_);
+    if (h_done_) (void)hipHostFree(h_done_);
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+    d_jobs_ = nullptr;
+    d_cursor_ = d_done_ = d_claims_ = nullptr;
+    d_order_ = d_status_ = nullptr;
+    d_hist_ = d_emitted_ = nullptr;
+    d_hits_ = nullptr;
+    d_table_ = nullptr;
+    d_queries_ = nullptr;
+    d_mm_ = nullptr;
+    d_mm_count_ = nullptr;
+    d_fixes_ = nullptr;
+    h_done_ = nullptr;
+    ev0_ = ev1_ = nullptr;
+    stream_ = nullptr;
+    device_ = -1;
+    req_cap_ = 0;
+    job_cap_ = hit_cap_ = fix_cap_ = 0;
+}
+
+bool Sweeper::init(int32_t req_cap, int64_t job_cap, int64_t hit_cap) {
+    release();
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    device_ = dev;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    cus_ = cus;
+    const std::vector<uint64_t>& table = mismatch_table();
+    table_n_ = (int32_t)table.size();
+    req_cap_ = req_cap;
+    job_cap_ = job_cap;
+    hit_cap_ = hit_cap;
+    fix_cap_ = 1 << 16;
+    bool ok =
+        hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) == hipSuccess &&
+        hipEventCreate(&ev0_) == hipSuccess && hipEventCreate(&ev1_) == hipSuccess &&
+        hipMalloc(&d_jobs_, sizeof(SweepJob) * job_cap) == hipSuccess &&
+        hipMalloc(&d_cursor_, sizeof(uint32_t) * job_cap) == hipSuccess &&
+        hipMalloc(&d_done_, sizeof(uint32_t) * job_cap) == hipSuccess &&
+        hipMalloc(&d_order_, sizeof(int32_t) * job_cap) == hipSuccess &&
+        hipHostMalloc(&h_done_, sizeof(uint32_t) * job_cap, hipHostMallocDefault) == hipSuccess &&
+        hipMalloc(&d_status_, sizeof(int32_t) * req_cap) == hipSuccess &&
+        hipMalloc(&d_hist_, sizeof(unsigned long long) * kHistBins * req_cap) == hipSuccess &&
+        hipMalloc(&d_emitted_, sizeof(unsigned long long) * req_cap) == hipSuccess &&
+        hipMalloc(&d_hits_, sizeof(Hit) * hit_cap) == hipSuccess &&
+        hipMalloc(&d_claims_, sizeof(uint32_t) * kSweepClasses) == hipSuccess &&
+        hipMalloc(&d_table_, sizeof(uint64_t) * table.size()) == hipSuccess &&
+        hipMemcpy(d_table_, table.data(), sizeof(uint64_t) * table.size(),
+                  hipMemcpyHostToDevice) == hipSuccess &&
+        hipMalloc(&d_queries_, sizeof(MismatchQuery) * req_cap) == hipSuccess &&
+        hipMalloc(&d_mm_, sizeof(TaggedMismatch) * kMismatchCap) == hipSuccess &&
+        hipMalloc(&d_mm_count_, sizeof(unsigned int)) == hipSuccess &&
+        hipMalloc(&d_fixes_, sizeof(CenterFix) * fix_cap_) == hipSuccess &&
+        hipMemsetAsync(d_status_, 0, sizeof(int32_t) * req_cap, stream_) == hipSuccess &&
+        hipStreamSynchronize(stream_) == hipSuccess;
+    if (!ok) {
+        (void)hipGetLastError();
+        release();
+        return false;
+    }
+    jobs_alloc_.reset(job_cap);
+    hits_alloc_.reset(hit_cap);
+    free_slots_.clear();
+    for (int32_t s = req_cap - 1; s >= 0; --s) free_slots_.push_back(s);
+    for (int c = 0; c < kSweepClasses; ++c) grid_[c] = cus_ * sweep_blocks_per_cu(c);
+    return true;
+}
+
+int64_t Sweeper::footprint(const SweepRequest& r) {
+    return r.hit_cap * (int64_t)sizeof(Hit) +
+           (int64_t)r.jobs.size() * (int64_t)(sizeof(SweepJob) + 3 * sizeof(uint32_t)) +
+           (int64_t)(kHistBins + 2) * 8 + (int64_t)r.fixes.size() * (int64_t)sizeof(CenterFix);
+}
+
+// ---- autotuner ----
+
+bool Sweeper::calibrate() {
+    if (costs_.ready) return true;
+    // One synthetic job per class and tile height; min_size above any count so
+    // nothing is emitted. Slot 0 holds the outputs; its status is set active.
+    const int32_t slot = 0;
+    const int32_t active = kReqActive;
+    if (hipMemcpyAsync(d_status_ + slot, &active, sizeof(int32_t), hipMemcpyHostToDevice,
+                       stream_) != hipSuccess)
+        return false;
+    for (int c = 0; c < kSweepClasses; ++c) {
+        const int nb = kSweepClass[c].nb, ph = kSweepClass[c].ph;
+        double a[2] = {0, 0}, r[2] = {0, 0};
+        const int32_t ths[2] = {64, 1024};
+        for (int k = -1; k < 2; ++k) {  // k = -1: warm-up launch
+            const int32_t th = ths[k < 0 ? 0 : k];
+            SearchSpec spec;
+            spec.seed = 0x5EED5EED5EEDll + c;
+            spec.x0 = -kPeriod;
+            spec.z0 = 12345;
+            spec.w = ph == 2 ? 2 * kPeriod : kPeriod;
+            const int64_t tiles_x = ph == 2 ? kTilesPerPhase : kTilesPerPhase;
+            const int64_t want = (k < 0 ? 1 : (th == 64 ? 24 : 6)) * (int64_t)grid_[c];
+            const int64_t row_tiles = std::max<int64_t>(1, (want + tiles_x - 1) / tiles_x);
+            const int64_t slot_rows = std::min<int64_t>(kPeriod / th * th, row_tiles * th);
+            spec.h = (int64_t)(nb - 1) * kPeriod + slot_rows;
+            spec.band_rows = (int32_t)slot_rows;
+            spec.min_size = kDiskCells;  // count only, emit nothing
+            spec.max_size = kDiskCells;
+            std::vector<BandJob> jobs;
+            band_jobs(spec, x_parts(spec, ph), std::vector<int>{nb}, th, int64_t(1) << 62, jobs);
+            if (jobs.size() != 1) return false;
+            SweepJob sj{};
+            sj.p = jobs[0].p;
+            sj.cls = c;
+            sj.req = slot;
+            sj.p.tag = slot;
+            sj.p.hits = d_hits_;
+            sj.p.hit_cap = 0;
+            sj.p.emitted = d_emitted_ + slot;
+            sj.p.hist = d_hist_ + (size_t)slot * kHistBins;
+            const int32_t order0 = 0;
+            float ms = 0.f;
+            if (hipMemcpyAsync(d_jobs_, &sj, sizeof(sj), hipMemcpyHostToDevice, stream_) !=
+                    hipSuccess ||
+                hipMemsetAsync(d_cursor_, 0, sizeof(uint32_t), stream_) != hipSuccess ||
+                hipMemsetAsync(d_done_, 0, sizeof(uint32_t), stream_) != hipSuccess ||
+                hipMemsetAsync(d_claims_, 0, sizeof(uint32_t) * kSweepClasses, stream_) !=
+                    hipSuccess ||
+                hipMemcpyAsync(d_order_, &order0, sizeof(int32_t), hipMemcpyHostToDevice,
+                               stream_) != hipSuccess ||
+                hipEventRecord(ev0_, stream_) != hipSuccess)
+                return false;
+            SweepArgs args{d_jobs_, d_order_, 1, d_cursor_, d_done_, d_status_, d_claims_ + c,
+                           0xFFFFFFFFu};
+            if (launch_sweep(args, c, grid_[c], stream_) != hipSuccess ||
+                hipEventRecord(ev1_, stream_) != hipSuccess ||
+                hipStreamSynchronize(stream_) != hipSuccess ||
+                hipEventElapsedTime(&ms, ev0_, ev1_) != hipSuccess)
+                return false;
+            if (k >= 0) {
+                a[k] = (double)ms * 1e6 / (double)sj.p.tiles_total;  // ns per tile
+                r[k] = (double)(th + kTileHalo);
+            }
+        }
+        // ns per tile = T * (rows + o)
+        double T = (a[1] - a[0]) / (r[1] - r[0]);
+        if (!(T > 0)) T = a[1] / r[1];
+        const double o = std::max(0.0, a[0] / T - r[0]);
+        costs_.ns_row[c] = T;
+        costs_.extra_rows[c] = o;
+    }
+    int32_t zero = 0;
+    if (hipMemcpyAsync(d_status_ + slot, &zero, sizeof(int32_t), hipMemcpyHostToDevice, stream_) !=
+            hipSuccess ||
+        hipStreamSynchronize(stream_) != hipSuccess)
+        return false;
+    costs_.ready = true;
+    return true;
+}
+
+double Sweeper::tile_ns(int cls, int32_t tile_h) const {
+    return costs_.ns_row[cls] * ((double)tile_h + kTileHalo + costs_.extra_rows[cls]);
+}
+
+// The schedule of one search under the measured costs. Choices:
+//  * x: one phase over the whole width; two phases (whole 2P blocks plus a
+//    right-aligned block for the remainder, or one masked block between P and
+//    2P); or two-phase blocks plus a one-phase strip for the remainder;
+//  * z, per x-part: the bands grouped into kernel classes (cheapest cover),
+//    either all over P slot rows, or the full bands so and the last, partial
+//    band as its own job over just its rows.
+bool Sweeper::plan(SweepRequest& r) const {
+    const SearchSpec& spec = r.spec;
+    struct Group {
+        int64_t b_first;
+        int nb;
+        int32_t s_rows;
+    };
+    auto tile_for = [](int32_t s_rows) { return std::min(kMaxTileH, (s_rows + 3) / 4 * 4); };
+    auto group_cost = [&](const XPart& xp, int nb, int32_t s_rows) {
+        const int c = sweep_class_of(nb, xp.ph);
+        if (c < 0) return -1.0;
+        const int32_t th = tile_for(s_rows);
+        const int64_t tiles = (int64_t)xp.nblocks * xp.tiles_per_block() * ((s_rows + th - 1) / th);
+        return (double)tiles * tile_ns(c, th);
+    };
+    // cheapest cover of bands [b0, b0 + n) over s_rows slot rows
+    auto cover = [&](const XPart& xp, int64_t b0, int64_t n, int32_t s_rows,
+                     std::vector<Group>& out) {
+        std::vector<double> cost(n + 1, -1.0);
+        std::vector<int> pick(n + 1, 0);
+        cost[0] = 0;
+        for (int64_t k = 1; k <= n; ++k)
+            for (int c = 0; c < kSweepClasses; ++c) {
+                const int nb = kSweepClass[c].nb;
+                if (kSweepClass[c].ph != xp.ph || nb > k || cost[k - nb] < 0) continue;
+                const double v = cost[k - nb] + group_cost(xp, nb, s_rows);
+                if (cost[k] < 0 || v < cost[k]) {
+                    cost[k] = v;
+                    pick[k] = nb;
+                }
+            }
+        if (cost[n] < 0) return -1.0;
+        std::vector<int> sizes;
+        for (int64_t k = n; k > 0; k -= pick[k]) sizes.push_back(pick[k]);
+        std::sort(sizes.begin(), sizes.end(), std::greater<int>());
+        int64_t b = b0;
+        for (int nb : sizes) {
+            out.push_back(Group{b, nb, s_rows});
+            b += nb;
+        }
+        return cost[n];
+    };
+    auto best_z = [&](const XPart& xp, std::vector<Group>& out) {
+        const int32_t slot_rows = slot_rows_of(spec);
+        const int64_t nbz = bands_of(spec);
+        std::vector<Group> a;
+        double ca = cover(xp, 0, nbz, slot_rows, a);
+        const int64_t nf = spec.h / kPeriod, rem = spec.h % kPeriod;
+        if (spec.band_rows == 0 && nf >= 1 && rem > 0) {
+            std::vector<Group> b;
+            double cb = cover(xp, 0, nf, kPeriod, b);
+            const double cp = group_cost(xp, 1, (int32_t)rem);
+            if (cb >= 0 && cp >= 0 && (ca < 0 || cb + cp < ca)) {
+                b.push_back(Group{nf, 1, (int32_t)rem});
+                out = b;
+                return cb + cp;
+            }
+        }
+        out = a;
+        return ca;
+    };
+    std::vector<std::vector<XPart>> xopts;
+    xopts.push_back(x_parts(spec, 1));
+    if (spec.w > kPeriod) xopts.push_back(x_parts(spec, 2));
+    if (spec.w > 2 * kPeriod && spec.w % (2 * kPeriod) != 0)
+        xopts.push_back(x_parts_one_phase_tail(spec));
+    double best = -1;
+    std::vector<XPart> best_parts;
+    std::vector<std::vector<Group>> best_groups;
+    for (const std::vector<XPart>& parts : xopts) {
+        double total = 0;
+        std::vector<std::vector<Group>> groups(parts.size());
+        for (size_t i = 0; i < parts.size() && total >= 0; ++i) {
+            const double c = best_z(parts[i], groups[i]);
+            total = c < 0 ? -1 : total + c;
+        }
+        if (total >= 0 && (best < 0 || total < best)) {
+            best = total;
+            best_parts = parts;
+            best_groups = groups;
+        }
+    }
+    if (best < 0) return false;
+    r.jobs.clear();
+    for (size_t i = 0; i < best_parts.size(); ++i)
+        for (const Group& g : best_groups[i])
+            band_jobs_part(spec, best_parts[i], g.b_first, g.nb, g.s_rows, tile_for(g.s_rows),
+                           int64_t(1) << 62, r.jobs);
+    r.job_cls.clear();
+    r.centers = 0;
+    for (const BandJob& j : r.jobs) {
+        const int c = sweep_class_of(j.nb, j.ph);
+        if (c < 0) return false;
+        r.job_cls.push_back(c);
+        r.centers += j.centers;
+    }
+    r.est_ns = best;
+    // every hit >= min_size is stored: room for four times the expected count
+    const double expect = (double)r.centers * center_tail(spec.min_size);
+    r.hit_cap = (int64_t)std::min<double>((double)kMaxHitCap, 4.0 * expect + 8192.0);
+    return true;
+}
+
+bool Sweeper::reserve(SweepRequest& r) {
+    if (free_slots_.empty()) return false;
+    const int64_t jo = jobs_alloc_.alloc((int64_t)r.jobs.size());
+    if (jo < 0) return false;
+    const int64_t ho = hits_alloc_.alloc(r.hit_cap);
+    if (ho < 0) {
+        jobs_alloc_.release(jo, (int64_t)r.jobs.size());
+        return false;
+    }
+    r.slot = free_slots_.back();
+    free_slots_.pop_back();
+    r.job_off = jo;
+    r.hit_off = ho;
+    r.done.assign(r.jobs.size(), 0);
+    r.complete = false;
+    return true;
+}
+
+void Sweeper::unreserve(SweepRequest& r) {
+    if (r.slot < 0) return;
+    jobs_alloc_.release(r.job_off, (int64_t)r.jobs.size());
+    hits_alloc_.release(r.hit_off, r.hit_cap);
+    free_slots_.push_back(r.slot);
+    r.slot = -1;
+    r.job_off = r.hit_off = -1;
+}
+
+bool Sweeper::ensure_fix_cap(size_t n) {
+    if ((int64_t)n <= fix_cap_) return true;
+    if (d_fixes_) (void)hipFree(d_fixes_);
+    d_fixes_ = nullptr;
+    const int64_t want = std::max<int64_t>((int64_t)n, 2 * fix_cap_);
+    if (hipMalloc(&d_fixes_, sizeof(CenterFix) * want) != hipSuccess) {
+        fix_cap_ = 0;
+        return false;
+    }
+    fix_cap_ = want;
+    return true;
+}
+
+bool Sweeper::prepare(const std::vector<SweepRequest*>& reqs) {
+    if (reqs.empty()) return true;
+    // 1. first-draw mismatches of every region in one launch
+    std::vector<MismatchQuery> q;
+    int64_t rows = 0;
+    for (size_t i = 0; i < reqs.size(); ++i) {
+        const SearchSpec& s = reqs[i]->spec;
+        MismatchQuery m;
+        m.seed = s.seed;
+        m.x_first = (int32_t)((int64_t)s.x0 - 8);
+        m.z_first = (int32_t)((int64_t)s.z0 - 8);
+        m.cols = s.w + 16;
+        m.rows = s.h + 16;
+        m.row_begin = rows;
+        m.tag = (int32_t)i;
+        rows += m.rows;
+        q.push_back(m);
+        reqs[i]->mm.mm.clear();
+    }
+    for (size_t base = 0; base < q.size(); base += (size_t)req_cap_) {
+        const size_t nq = std::min(q.size() - base, (size_t)req_cap_);
+        const int64_t first_row = q[base].row_begin;
+        std::vector<MismatchQuery> chunk(q.begin() + base, q.begin() + base + nq);
+        for (MismatchQuery& m : chunk) m.row_begin -= first_row;
+        const int64_t nrows = chunk.back().row_begin + chunk.back().rows;
+        unsigned int count = 0;
+        if (hipMemcpyAsync(d_queries_, chunk.data(), sizeof(MismatchQuery) * nq,
+                           hipMemcpyHostToDevice, stream_) != hipSuccess ||
+            hipMemsetAsync(d_mm_count_, 0, sizeof(unsigned int), stream_) != hipSuccess ||
+            launch_find_mismatches_multi(d_queries_, (int32_t)nq, nrows, d_table_, table_n_, d_mm_,
+                                         kMismatchCap, d_mm_count_, stream_) != hipSuccess ||
+            hipMemcpyAsync(&count, d_mm_count_, sizeof(count), hipMemcpyDeviceToHost, stream_) !=
+                hipSuccess ||
+            hipStreamSynchronize(stream_) != hipSuccess || count > kMismatchCap)
+            return false;
+        std::vector<TaggedMismatch> mm(count);
+        if (count && hipMemcpy(mm.data(), d_mm_, sizeof(TaggedMismatch) * count,
+                               hipMemcpyDeviceToHost) != hipSuccess)
+            return false;
+        for (const TaggedMismatch& t : mm)
+            reqs[base + t.tag]->mm.mm.push_back(MismatchChunk{t.x, t.z});
+    }
+    // 2. the centers they touch, recounted exactly in one launch
+    std::vector<CenterFix> all;
+    std::vector<size_t> first(reqs.size() + 1, 0);
+    for (size_t i = 0; i < reqs.size(); ++i) {
+        reqs[i]->mm.sort();
+        collect_fixes(reqs[i]->spec, reqs[i]->mm, reqs[i]->fixes);
+        first[i] = all.size();
+        all.insert(all.end(), reqs[i]->fixes.begin(), reqs[i]->fixes.end());
+    }
+    first[reqs.size()] = all.size();
+    if (!all.empty()) {
+        if (!ensure_fix_cap(all.size()) ||
+            hipMemcpyAsync(d_fixes_, all.data(), sizeof(CenterFix) * all.size(),
+                           hipMemcpyHostToDevice, stream_) != hipSuccess ||
+            launch_recount(d_fixes_, (int32_t)all.size(), stream_) != hipSuccess ||
+            hipMemcpyAsync(all.data(), d_fixes_, sizeof(CenterFix) * all.size(),
+                           hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+            hipStreamSynchronize(stream_) != hipSuccess)
+            return false;
+        for (size_t i = 0; i < reqs.size(); ++i)
+            std::copy(all.begin() + first[i], all.begin() + first[i + 1], reqs[i]->fixes.begin());
+    }
+    return true;
+}
+
+bool Sweeper::upload(const std::vector<SweepRequest*>& reqs) {
+    std::vector<SweepJob> staged;
+    for (SweepRequest* r : reqs) {
+        staged.clear();
+        for (size_t j = 0; j < r->jobs.size(); ++j) {
+            SweepJob sj{};
+            sj.p = r->jobs[j].p;
+            sj.cls = r->job_cls[j];
+            sj.req = r->slot;
+            sj.p.tag = r->slot;
+            sj.p.hits = d_hits_ + r->hit_off;
+            sj.p.hit_cap = (uint32_t)r->hit_cap;
+            sj.p.emitted = d_emitted_ + r->slot;
+            sj.p.hist = d_hist_ + (size_t)r->slot * kHistBins;
+            staged.push_back(sj);
+        }
+        const size_t nj = staged.size();
+        if ((nj && (hipMemcpyAsync(d_jobs_ + r->job_off, staged.data(), sizeof(SweepJob) * nj,
+                                   hipMemcpyHostToDevice, stream_) != hipSuccess ||
+                    hipMemsetAsync(d_cursor_ + r->job_off, 0, sizeof(uint32_t) * nj, stream_) !=
+                        hipSuccess ||
+                    hipMemsetAsync(d_done_ + r->job_off, 0, sizeof(uint32_t) * nj, stream_) !=
+                        hipSuccess)) ||
+            hipMemsetAsync(d_hist_ + (size_t)r->slot * kHistBins, 0,
+                           sizeof(unsigned long long) * kHistBins, stream_) != hipSuccess ||
+            hipMemsetAsync(d_emitted_ + r->slot, 0, sizeof(unsigned long long), stream_) !=
+                hipSuccess)
+            return false;
+        r->status = kReqActive;
+        if (hipMemcpyAsync(d_status_ + r->slot, &r->status, sizeof(int32_t),
+                           hipMemcpyHostToDevice, stream_) != hipSuccess)
+            return false;
+        r->remaining_ns = r->est_ns;
+    }
+    return hipStreamSynchronize(stream_) == hipSuccess;
+}
+
+bool Sweeper::clear_status() {
+    return hipMemsetAsync(d_status_, 0, sizeof(int32_t) * (size_t)req_cap_, stream_) ==
+               hipSuccess &&
+           hipStreamSynchronize(stream_) == hipSuccess;
+}
+
+bool Sweeper::set_status(SweepRequest& r, int32_t status) {
+    r.status = status;
+    return hipMemcpyAsync(d_status_ + r.slot, &r.status, sizeof(int32_t), hipMemcpyHostToDevice,
+                          stream_) == hipSuccess &&
+           hipStreamSynchronize(stream_) == hipSuccess;
+}
+
+bool Sweeper::round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& pol,
+                    RoundStats* st) {
+    // per-class claim order over the jobs of active requests with tiles left
+    struct Item {
+        double key;
+        int64_t seq;
+        int32_t job;
+        int64_t left;
+        int32_t tile_h;
+    };
+    std::vector<Item> items[kSweepClasses];
+    int64_t seq = 0;
+    for (const SweepRequest* r : reqs) {
+        if (r->status != kReqActive || r->complete) continue;
+        for (size_t j = 0; j < r->jobs.size(); ++j) {
+            const int64_t left = (int64_t)r->jobs[j].p.tiles_total - r->done[j];
+            const int c = r->job_cls[j];
+            // largest tiles first (batch), or the request with the least work left
+            const double key = pol.shortest_first ? r->remaining_ns : -tile_ns(c, r->jobs[j].p.tile_h);
+            if (left > 0)
+                items[c].push_back(Item{key, seq, (int32_t)(r->job_off + (int64_t)j), left,
+                                        r->jobs[j].p.tile_h});
+            ++seq;
+        }
+    }
+    std::vector<int32_t> order;
+    struct Launch {
+        int cls;
+        int32_t off, n;
+        uint32_t budget;
+        double min_key;
+    };
+    std::vector<Launch> launches;
+    for (int c = 0; c < kSweepClasses; ++c) {
+        std::vector<Item>& v = items[c];
+        if (v.empty()) continue;
+        std::sort(v.begin(), v.end(), [](const Item& a, const Item& b) {
+            return a.key != b.key ? a.key < b.key : a.seq < b.seq;
+        });
+        Launch l;
+        l.cls = c;
+        l.off = (int32_t)order.size();
+        l.n = (int32_t)v.size();
+        l.min_key = v.front().key;
+        int64_t tiles = 0;
+        double tile_sum = 0;
+        for (const Item& it : v) {
+            order.push_back(it.job);
+            tiles += it.left;
+            tile_sum += tile_ns(c, it.tile_h) * (double)it.left;
+        }
+        if (pol.round_ns <= 0) {
+            l.budget = 0xFFFFFFFFu;
+        } else {
+            // claims that take about round_ns of the whole device
+            const double per_tile = tile_sum / (double)std::max<int64_t>(1, tiles);
+            const double want = (double)pol.round_ns / std::max(1e-3, per_tile);
+            l.budget = (uint32_t)std::min<double>(4e9, std::max<double>((double)grid_[c], want));
+        }
+        launches.push_back(l);
+    }
+    if (launches.empty()) return true;
+    // classes holding the most urgent request go first
+    std::stable_sort(launches.begin(), launches.end(),
+                     [](const Launch& a, const Launch& b) { return a.min_key < b.min_key; });
+    if (hipMemcpyAsync(d_order_, order.data(), sizeof(int32_t) * order.size(),
+                       hipMemcpyHostToDevice, stream_) != hipSuccess)
+        return false;
+
+    auto enqueue = [&]() -> bool {
+        if (launch_fill_u32(d_claims_, kSweepClasses, 0u, stream_) != hipSuccess) return false;
+        for (const Launch& l : launches) {
+            SweepArgs a{d_jobs_, d_order_ + l.off, l.n, d_cursor_, d_done_, d_status_,
+                        d_claims_ + l.cls, l.budget};
+            if (launch_sweep(a, l.cls, grid_[l.cls], stream_) != hipSuccess) return false;
+        }
+        return true;
+    };
+    if (hipEventRecord(ev0_, stream_) != hipSuccess) return false;
+    if (pol.use_graph) {
+        // the launch sequence, keyed by its arguments; replays reuse the device
+        // arrays (jobs, orders) whose contents were written above
+        std::vector<int64_t> key;
+        for (const Launch& l : launches) {
+            key.push_back(l.cls);
+            key.push_back(l.off);
+            key.push_back(l.n);
+            key.push_back(l.budget);
+        }
+        auto it = graphs_.find(key);
+        if (it == graphs_.end()) {
+            hipGraph_t g = nullptr;
+            hipGraphExec_t ge = nullptr;
+            if (hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal) != hipSuccess)
+                return false;
+            const bool ok = enqueue();
+            if (hipStreamEndCapture(stream_, &g) != hipSuccess || !ok) {
+                if (g) (void)hipGraphDestroy(g);
+                return false;
+            }
+            const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return false;
+            it = graphs_.emplace(key, ge).first;
+        }
+        if (hipGraphLaunch(it->second, stream_) != hipSuccess) return false;
+    } else if (!enqueue()) {
+        return false;
+    }
+    float ms = 0.f;
+    if (hipEventRecord(ev1_, stream_) != hipSuccess ||
+        hipMemcpyAsync(h_done_, d_done_, sizeof(uint32_t) * (size_t)job_cap_,
+                       hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+        hipStreamSynchronize(stream_) != hipSuccess ||
+        hipEventElapsedTime(&ms, ev0_, ev1_) != hipSuccess)
+        return false;
+    if (st) {
+        st->kernel_ns += (int64_t)((double)ms * 1e6);
+        st->launches += (int32_t)launches.size();
+        st->rounds += 1;
+    }
+    for (SweepRequest* r : reqs) {
+        if (r->slot < 0 || r->complete) continue;
+        bool all = true;
+        double left_ns = 0;
+        for (size_t j = 0; j < r->jobs.size(); ++j) {
+            r->done[j] = h_done_[r->job_off + (int64_t)j];
+            const int64_t left = (int64_t)r->jobs[j].p.tiles_total - r->done[j];
+            if (left > 0) {
+                all = false;
+                left_ns += (double)left * tile_ns(r->job_cls[j], r->jobs[j].p.tile_h);
+            }
+        }
+        r->remaining_ns = left_ns;
+        r->complete = all;
+    }
+    return true;
+}
+
+double Sweeper::progress(const SweepRequest& r) const {
+    int64_t done = 0, total = 0;
+    for (size_t j = 0; j < r.jobs.size(); ++j) {
+        done += r.done[j];
+        total += r.jobs[j].p.tiles_total;
+    }
+    return total ? (double)done / (double)total : 1.0;
+}
+
+bool Sweeper::finalize(SweepRequest& r, bool partial) {
+    unsigned long long hist[kHistBins];
+    unsigned long long emitted = 0;
+    if (hipMemcpyAsync(hist, d_hist_ + (size_t)r.slot * kHistBins, sizeof(hist),
+                       hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+        hipMemcpyAsync(&emitted, d_emitted_ + r.slot, sizeof(emitted), hipMemcpyDeviceToHost,
+                       stream_) != hipSuccess ||
+        hipStreamSynchronize(stream_) != hipSuccess)
+        return false;
+    if ((int64_t)emitted > r.hit_cap) {
+        // more hits than the reserved region: rerun alone with a rising gate
+        r.rows.assign((size_t)r.keep, ResultRow{});
+        r.total = run_search(r.spec, r.rows.data(), r.keep);
+        r.rows.resize((size_t)std::max<int64_t>(0, std::min<int64_t>(r.total, r.keep)));
+        r.fallback = true;
+        return r.total >= 0;
+    }
+    std::vector<Hit> hits((size_t)emitted);
+    if (emitted && (hipMemcpyAsync(hits.data(), d_hits_ + r.hit_off, sizeof(Hit) * emitted,
+                                   hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+                    hipStreamSynchronize(stream_) != hipSuccess))
+        return false;
+    int64_t h[kHistBins];
+    for (int s = 0; s < kHistBins; ++s) h[s] = (int64_t)hist[s];
+    std::vector<Hit> cand;
+    cand.reserve(hits.size());
+    for (const Hit& x : hits)
+        if (!r.mm.affects(x.cx, x.cz)) cand.push_back(Hit{x.size, x.cx, x.cz, 0});
+    if (!partial) {
+        // replace the provisional first-draw counts of recounted centers
+        for (const CenterFix& f : r.fixes) {
+            if (f.first_draw >= r.spec.min_size && f.first_draw <= r.spec.max_size) h[f.first_draw]--;
+            if (f.exact >= r.spec.min_size && f.exact <= r.spec.max_size) {
+                h[f.exact]++;
+                cand.push_back(Hit{f.exact, f.cx, f.cz, 0});
+            }
+        }
+    }
+    int64_t total = 0;
+    for (int s = 0; s < kHistBins; ++s) total += h[s];
+    r.total = total;
+    r.rows.assign((size_t)std::min<int64_t>((int64_t)cand.size(), r.keep), ResultRow{});
+    write_rows(cand, (size_t)r.keep, r.rows.data());
+    return true;
+}
+
+// ---- batch ----------------------------------------------------------------------
+
+namespace {
+std::mutex g_batch_mutex;
+Sweeper g_batch_sweeper;
+BatchStats g_last_batch;
+}  // namespace
+
+int run_batch(const SearchSpec* specs, int32_t n, int32_t max_results, ResultRow* out,
+              int64_t* totals, const SweepOptions& opt, BatchStats* stats) {
+    std::lock_guard<std::mutex> lock(g_batch_mutex);
+    const auto t0 = Clock::now();
+    BatchStats st{};
+    Sweeper& sw = g_batch_sweeper;
+    std::vector<SweepRequest> reqs((size_t)n);
+    int64_t need_jobs = 0, need_hits = 0;
+    auto fail = [&]() {
+        (void)hipGetLastError();
+        return -1;
+    };
+    // capacity first: plans only need the cost table, which needs a device state
+    if (!sw.ready() && !sw.init(std::max(n, 1), 4096, int64_t(1) << 20)) return fail();
+    if (!sw.calibrate()) return fail();
+    for (int32_t i = 0; i < n; ++i) {
+        reqs[i].spec = specs[i];
+        reqs[i].keep = max_results;
+        if (!sw.plan(reqs[i])) return fail();
+        need_jobs += (int64_t)reqs[i].jobs.size();
+        need_hits += reqs[i].hit_cap;
+    }
+    if (sw.req_cap() < n || sw.job_cap() < need_jobs || sw.hit_cap() < need_hits) {
+        const CostTable keep = sw.costs();
+        if (!sw.init(std::max(n, sw.req_cap()), std::max(need_jobs, sw.job_cap()),
+                     std::max(need_hits, sw.hit_cap())))
+            return fail();
+        sw.set_costs(keep);
+    }
+    std::vector<SweepRequest*> ptrs;
+    for (SweepRequest& r : reqs) {
+        if (!sw.reserve(r)) return fail();
+        ptrs.push_back(&r);
+    }
+    auto cleanup = [&]() {
+        (void)sw.clear_status();
+        for (SweepRequest& r : reqs) sw.unreserve(r);
+    };
+    if (!sw.prepare(ptrs) || !sw.upload(ptrs)) {
+        cleanup();
+        return fail();
+    }
+    st.prepare_ns = ns_since(t0);
+    RoundPolicy pol;
+    pol.round_ns = 0;  // one unbounded round: every tile in one launch per class
+    pol.shortest_first = false;
+    pol.use_graph = opt.use_graph;
+    RoundStats rs{};
+    const auto t1 = Clock::now();
+    bool ok = sw.round(ptrs, pol, &rs);
+    for (int guard = 0; ok && guard < 4; ++guard) {  // only after a budget cut-off
+        bool all = true;
+        for (SweepRequest* r : ptrs) all = all && r->complete;
+        if (all) break;
+        ok = sw.round(ptrs, pol, &rs);
+    }
+    st.sweep_ns = ns_since(t1);
+    if (!ok) {
+        cleanup();
+        return fail();
+    }
+    for (int32_t i = 0; i < n; ++i) {
+        SweepRequest& r = reqs[i];
+        if (!r.complete || !sw.finalize(r, false)) {
+            cleanup();
+            return fail();
+        }
+        totals[i] = r.total;
+        std::memcpy(out + (size_t)i * max_results, r.rows.data(),
+                    sizeof(ResultRow) * r.rows.size());
+        st.centers += r.centers;
+        st.fallbacks += r.fallback ? 1 : 0;
+        st.jobs += (int32_t)r.jobs.size();
+    }
+    cleanup();
+    st.kernel_ns = rs.kernel_ns;
+    st.launches = rs.launches;
+    st.wall_ns = ns_since(t0);
+    g_last_batch = st;
+    if (stats) *stats = st;
+    return 0;
+}
+
+void last_batch_stats(BatchStats* st) {
+    std::lock_guard<std::mutex> lock(g_batch_mutex);
+    if (st) *st = g_last_batch;
+}
+
+bool sweep_cost_table(double* ns_row, double* extra_rows) {
+    std::lock_guard<std::mutex> lock(g_batch_mutex);
+    Sweeper& sw = g_batch_sweeper;
+    if (!sw.ready() && !sw.init(16, 4096, int64_t(1) << 20)) return false;
+    if (!sw.calibrate()) return false;
+    for (int c = 0; c < kSweepClasses; ++c) {
+        if (ns_row) ns_row[c] = sw.costs().ns_row[c];
+        if (extra_rows) extra_rows[c] = sw.costs().extra_rows[c];
+    }
+    return true;
+}
+
+}  // namespace slime
