@@ -14,6 +14,18 @@ ops per center x centers / kernel time, kernel time from HIP events on the
 launch stream) and a CPU baseline: the reference's SlimeCore search compiled
 from its sources (oracle/_ref), timed on a bounded sample of the same
 workload on this host's cores.
+
+Two more legs ride in the same line:
+  batched  256 searches from a splitmix64 stream (radius log-uniform in
+           [256, 200000], centers within +-1.8M chunks, thresholds 50..62),
+           run as one sweep (persistent kernels, optionally replayed from a
+           hipGraph) against 256 separate calls of the same library and 256
+           calls of its plain per-chunk kernel; all results must be identical.
+  served   Poisson arrivals of the served mix (70% radius 256..4096, 25%
+           radius 50,000, 5% full world; thresholds 50..65; 10% pause then
+           resume, 5% cancel) into one session, against the same arrivals
+           served one search at a time in arrival order; p50/p99 latency and
+           aggregate T/s of both, results identical.
 """
 import argparse
 import ctypes as C
@@ -36,8 +48,8 @@ OPS_PER_CENTER = 6      # algorithmic INT32 ops per candidate center
 CPU_SAMPLE_RADIUS = 131072
 # HBM bytes per scan_band_kernel launch (rocprofv3 FETCH_SIZE + WRITE_SIZE of
 # the full-world run, scaled by the same counters on a copy of known size;
-# profiles/scan_band_kernel_full_world_thr60.md): 165.4 KiB x 2.0 + 0.03 KiB x 1.0.
-TRAFFIC_BYTES = 338800
+# profiles/scan_band_kernel_full_world_thr60.md): 91.1 KiB x 2.0 + 0.03 KiB x 1.0.
+TRAFFIC_BYTES = 186600
 
 
 def hip_sync():
@@ -94,6 +106,138 @@ def cpu_baseline(gpu_lib):
             "sample": sample, "seconds": dt, "gpu_matches_sample": gpu == result}
 
 
+def batched_leg(repeats):
+    """The batched configuration: sweep vs separate calls (same results)."""
+    import slime_amd
+    from slime_amd.workloads import batched_searches, BATCH_STATE
+    searches = batched_searches(256)
+    cols = list(zip(*searches))
+    k = 1000
+    centers = sum((2 * s[1] + 1) ** 2 for s in searches)
+    res = slime_amd.search_batch(*cols, max_results=k)  # warm-up, calibrates the autotuner
+    out = {"workload": "256 searches, radius log-uniform [256, 200000], thr 50..62",
+           "state": hex(BATCH_STATE), "centers": centers, "max_results": k}
+    for graph in (False, True):
+        times = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            got = slime_amd.search_batch(*cols, max_results=k, use_graph=graph)
+            times.append(time.perf_counter() - t0)
+            if got != res:
+                raise SystemExit("batched results differ between runs")
+        dt = min(times)
+        st = slime_amd.batch_stats()
+        key = "sweep_graph" if graph else "sweep"
+        out[key] = {"seconds": dt, "t_centers_per_s": centers / dt / 1e12,
+                    "kernel_s": st["kernel_ns"] / 1e9, "launches": st["launches"],
+                    "jobs": st["jobs"], "fallbacks": st["fallbacks"]}
+        print(f"batched {key}: {dt:.3f} s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    per = [slime_amd.search_slime_clusters_gpu_v1_centered(*s, max_results=k) for s in searches]
+    dt = time.perf_counter() - t0
+    out["per_search"] = {"seconds": dt, "t_centers_per_s": centers / dt / 1e12}
+    print(f"batched per-search: {dt:.3f} s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    plain = []
+    for s in searches:
+        seed, rd, thr, mx, _, cx, cz = s
+        tot, rows, _ = slime_amd.scan_bands(seed, cx - rd, cz - rd, 2 * rd + 1, 2 * rd + 1, thr, mx,
+                                            0, slime_amd.MODE_PLAIN, k)
+        plain.append((tot, rows))
+    dt = time.perf_counter() - t0
+    out["per_search_plain_kernel"] = {"seconds": dt, "t_centers_per_s": centers / dt / 1e12}
+    print(f"batched per-search plain kernel: {dt:.3f} s", file=sys.stderr, flush=True)
+    best = max(out["sweep"]["t_centers_per_s"], out["sweep_graph"]["t_centers_per_s"])
+    out["speedup_vs_per_search"] = best / out["per_search"]["t_centers_per_s"]
+    out["speedup_vs_per_search_plain_kernel"] = best / out["per_search_plain_kernel"]["t_centers_per_s"]
+    out["identical"] = per == res and plain == res
+    return out
+
+
+def percentile(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    i = min(len(xs) - 1, max(0, int(round(q / 100.0 * (len(xs) - 1)))))
+    return xs[i]
+
+
+def served_leg(n, rate):
+    """The served configuration: one session vs one search at a time."""
+    import threading
+    import slime_amd
+    from slime_amd.workloads import served_requests, SERVED_STATE
+    reqs = served_requests(n, rate)
+    k = 1000
+    pause_s, cancel_s = 0.005, 0.005
+    # served: continuous batching
+    lat, results = {}, {}
+    with slime_amd.Session(hbm_budget=16 << 30, max_active=64, round_ms=20) as sess:
+        ids = [None] * n
+        t_start = time.perf_counter()
+
+        def arrivals():
+            for i, r in enumerate(reqs):
+                delay = t_start + r["arrival"] - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                ids[i] = sess.submit(*r["search"], max_results=k)
+                if r["action"] == "pause":
+                    threading.Timer(pause_s, sess.pause, (ids[i],)).start()
+                    threading.Timer(pause_s + 0.05, sess.resume, (ids[i],)).start()
+                elif r["action"] == "cancel":
+                    threading.Timer(cancel_s, sess.cancel, (ids[i],)).start()
+
+        th = threading.Thread(target=arrivals)
+        th.start()
+        th.join()
+        t_last = t_start
+        for i, r in enumerate(reqs):
+            st = sess.wait(ids[i], -1)
+            if st == slime_amd.DONE:
+                state, _, total, lat_ns = sess.poll(ids[i])
+                if r["action"] != "cancel":
+                    lat[i] = lat_ns / 1e9
+                    results[i] = sess.result(ids[i], k)
+        t_last = time.perf_counter()
+        stats = sess.stats()
+    served_wall = t_last - t_start
+    done = [i for i in results]
+    centers = sum((2 * reqs[i]["search"][1] + 1) ** 2 for i in done)
+    print(f"served: {len(done)} requests in {served_wall:.2f} s", file=sys.stderr, flush=True)
+    # sequential: one search at a time in arrival order (cancelled ones skipped,
+    # a paused one holds the device for the pause time)
+    seq_lat, seq_res = {}, {}
+    t0 = time.perf_counter()
+    for i, r in enumerate(reqs):
+        now = time.perf_counter() - t0
+        if now < r["arrival"]:
+            time.sleep(r["arrival"] - now)
+        if r["action"] == "cancel":
+            continue
+        if r["action"] == "pause":
+            time.sleep(0.05)
+        seq_res[i] = slime_amd.search_slime_clusters_gpu_v1_centered(*r["search"], max_results=k)
+        seq_lat[i] = time.perf_counter() - t0 - r["arrival"]
+    seq_wall = time.perf_counter() - t0
+    print(f"sequential: {len(seq_res)} requests in {seq_wall:.2f} s", file=sys.stderr, flush=True)
+    seq_centers = sum((2 * reqs[i]["search"][1] + 1) ** 2 for i in seq_res)
+    ident = all(results[i] == seq_res.get(i) for i in done) and set(done) == set(seq_res)
+    return {
+        "workload": "Poisson arrivals: 70% radius 256..4096, 25% radius 50000, 5% full world; "
+                    "thr 50..65; 10% pause+resume, 5% cancel",
+        "state": hex(SERVED_STATE), "requests": n, "rate_per_s": rate, "completed": len(done),
+        "served": {"wall_s": served_wall, "t_centers_per_s": centers / served_wall / 1e12,
+                   "p50_s": percentile(list(lat.values()), 50),
+                   "p99_s": percentile(list(lat.values()), 99),
+                   "kernel_s": stats["kernel_ns"] / 1e9, "rounds": stats["rounds"]},
+        "sequential": {"wall_s": seq_wall, "t_centers_per_s": seq_centers / seq_wall / 1e12,
+                       "p50_s": percentile(list(seq_lat.values()), 50),
+                       "p99_s": percentile(list(seq_lat.values()), 99)},
+        "identical": ident,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +250,12 @@ def main():
                     help="also run one device-to-device copy of BYTES (profiler scale)")
     ap.add_argument("--traffic-kib", type=float, default=None,
                     help="HBM KiB per launch of the dominant kernel measured by rocprofv3")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="run only the batched leg (for profiling its kernels)")
+    ap.add_argument("--no-served", action="store_true", help="skip the served leg")
+    ap.add_argument("--served-requests", type=int, default=80)
+    ap.add_argument("--served-rate", type=float, default=6.0, help="arrivals per second")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -117,6 +267,9 @@ def main():
     if lib.get_cuda_device_count() < 1:
         print("bench.py: no HIP device", file=sys.stderr)
         return 1
+    if args.only_batched:
+        print(json.dumps({"batched": batched_leg(1)}), flush=True)
+        return 0
     rd = args.radius
     centers = (2 * rd + 1) ** 2
     if args.copy_calibration:
@@ -187,6 +340,10 @@ def main():
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(lib)
+    if not args.no_batched:
+        out["batched"] = batched_leg(2)
+    if not args.no_served:
+        out["served"] = served_leg(args.served_requests, args.served_rate)
     print(json.dumps(out), flush=True)
     return 0
 

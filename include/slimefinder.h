@@ -160,6 +160,46 @@ void slime_batch_stats(int64_t* centers, int64_t* kernel_ns, int64_t* prepare_ns
 int32_t slime_sweep_costs(double* ns_row, double* extra_rows, int32_t* nb, int32_t* ph,
                           int32_t cap);
 
+/* ---- served searches (extension; the reference serves one search at a time
+ * behind process-global flags, SlimeCore.cpp:16-27) -------------------------
+ * A session keeps every admitted search's state on the device (jobs, claim
+ * cursors, histogram, hit region, status word) and scans all of them together
+ * in rounds of persistent launches (continuous batching). Requests are
+ * admitted in arrival order while their device footprint fits hbm_budget
+ * bytes and fewer than max_active are admitted; round_ms bounds the device
+ * time of one round (the latency of admission, pause and cancel). Each result
+ * equals search_slime_clusters_gpu_v1_centered with the same arguments.
+ * Request states: 0 queued, 1 running, 2 paused, 3 done, 4 cancelled
+ * (partial result), 5 failed. */
+void* slime_session_create(int64_t hbm_budget, int32_t max_active, int32_t round_ms);
+void slime_session_destroy(void* session);
+/* Returns a request id (> 0), or -1 for invalid arguments. */
+int64_t slime_session_submit(void* session, int64_t seed, int64_t rd_max, int32_t min_size,
+                             int32_t max_size, int64_t rd_min, int32_t search_center_x,
+                             int32_t search_center_z, int32_t max_results);
+/* 0 on success, -1 if the request is unknown or already finished. */
+int32_t slime_session_pause(void* session, int64_t id);
+int32_t slime_session_resume(void* session, int64_t id);
+int32_t slime_session_cancel(void* session, int64_t id);
+/* Returns the state (or -1 for an unknown id); progress in percent, total hits
+ * (-1 until done) and submit-to-finish latency in ns (-1 until done). */
+int32_t slime_session_poll(void* session, int64_t id, int32_t* progress_pct, int64_t* total,
+                           int64_t* latency_ns);
+/* Blocks until the request finishes or timeout_ms passes (< 0: no limit);
+ * returns its state. */
+int32_t slime_session_wait(void* session, int64_t id, int32_t timeout_ms);
+/* Copies up to cap ranked rows of a finished request; returns its total, or
+ * -1 if it is not finished. */
+int64_t slime_session_result(void* session, int64_t id, ExtChunkResult* results_buffer,
+                             int32_t cap);
+/* Forgets a finished request; 0 on success. */
+int32_t slime_session_release(void* session, int64_t id);
+/* Session figures: centers of completed requests, device time, rounds,
+ * admitted and queued requests, device bytes held and the budget. */
+void slime_session_stats(void* session, int64_t* centers_done, int64_t* kernel_ns,
+                         int32_t* rounds, int32_t* active, int32_t* queued, int64_t* hbm_in_use,
+                         int64_t* hbm_budget);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,7 +29,10 @@ EXPORTS = (
     "get_gpu_v1_shape", "get_gpu_v1_algorithm", "get_gpu_v1_rng", "set_gpu_v1_rng_override",
     "get_gpu_v34_shape", "get_gpu_v34_rng", "set_gpu_v34_rng_override", "slime_scan_rect",
     "slime_kernel_stats", "slime_device_info", "slime_scan_bands", "slime_search_batch",
-    "slime_batch_stats", "slime_sweep_costs",
+    "slime_batch_stats", "slime_sweep_costs", "slime_session_create", "slime_session_destroy",
+    "slime_session_submit", "slime_session_pause", "slime_session_resume",
+    "slime_session_cancel", "slime_session_poll", "slime_session_wait", "slime_session_result",
+    "slime_session_release", "slime_session_stats",
 )
 
 
@@ -104,6 +107,25 @@ def lib():
     L.slime_batch_stats.restype = None
     L.slime_sweep_costs.argtypes = [PD, PD, PI32, PI32, i32]
     L.slime_sweep_costs.restype = i32
+    VP = C.c_void_p
+    L.slime_session_create.argtypes = [i64, i32, i32]
+    L.slime_session_create.restype = VP
+    L.slime_session_destroy.argtypes = [VP]
+    L.slime_session_destroy.restype = None
+    L.slime_session_submit.argtypes = [VP, i64, i64, i32, i32, i64, i32, i32, i32]
+    L.slime_session_submit.restype = i64
+    for name in ("slime_session_pause", "slime_session_resume", "slime_session_cancel",
+                 "slime_session_release"):
+        getattr(L, name).argtypes = [VP, i64]
+        getattr(L, name).restype = i32
+    L.slime_session_poll.argtypes = [VP, i64, PI32, PI64, PI64]
+    L.slime_session_poll.restype = i32
+    L.slime_session_wait.argtypes = [VP, i64, i32]
+    L.slime_session_wait.restype = i32
+    L.slime_session_result.argtypes = [VP, i64, P, i32]
+    L.slime_session_result.restype = i64
+    L.slime_session_stats.argtypes = [VP, PI64, PI64, PI32, PI32, PI32, PI64, PI64]
+    L.slime_session_stats.restype = None
     _lib = L
     return L
 
@@ -236,3 +258,70 @@ def sweep_costs():
     if n < 0:
         raise RuntimeError("slime_sweep_costs failed")
     return [(nb[i], ph[i], a[i], b[i]) for i in range(n)]
+
+
+QUEUED, RUNNING, PAUSED, DONE, CANCELLED, FAILED = range(6)
+
+
+class Session:
+    """Served searches with continuous batching (slime_session_*): requests
+    are admitted by device footprint, scanned together in rounds, and can be
+    paused, resumed and cancelled while others run."""
+
+    def __init__(self, hbm_budget=4 << 30, max_active=64, round_ms=20):
+        self._h = lib().slime_session_create(hbm_budget, max_active, round_ms)
+        if not self._h:
+            raise RuntimeError("slime_session_create failed")
+
+    def close(self):
+        if self._h:
+            lib().slime_session_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, seed, rd_max, min_size, max_size=221, rd_min=0, center_x=0, center_z=0,
+               max_results=1000):
+        return lib().slime_session_submit(self._h, seed, rd_max, min_size, max_size, rd_min,
+                                          center_x, center_z, max_results)
+
+    def pause(self, rid):
+        return lib().slime_session_pause(self._h, rid) == 0
+
+    def resume(self, rid):
+        return lib().slime_session_resume(self._h, rid) == 0
+
+    def cancel(self, rid):
+        return lib().slime_session_cancel(self._h, rid) == 0
+
+    def poll(self, rid):
+        """(state, progress_pct, total, latency_ns)."""
+        pct, tot, lat = C.c_int32(), C.c_int64(), C.c_int64()
+        st = lib().slime_session_poll(self._h, rid, C.byref(pct), C.byref(tot), C.byref(lat))
+        return st, pct.value, tot.value, lat.value
+
+    def wait(self, rid, timeout_ms=-1):
+        return lib().slime_session_wait(self._h, rid, timeout_ms)
+
+    def result(self, rid, cap=1000):
+        """(total, rows) of a finished request."""
+        buf = (ExtChunkResult * max(1, cap))()
+        total = lib().slime_session_result(self._h, rid, buf, cap)
+        return total, _rows(buf, total, cap)
+
+    def release(self, rid):
+        return lib().slime_session_release(self._h, rid) == 0
+
+    def stats(self):
+        v64 = [C.c_int64() for _ in range(4)]
+        v32 = [C.c_int32() for _ in range(3)]
+        lib().slime_session_stats(self._h, C.byref(v64[0]), C.byref(v64[1]), C.byref(v32[0]),
+                                  C.byref(v32[1]), C.byref(v32[2]), C.byref(v64[2]),
+                                  C.byref(v64[3]))
+        return {"centers_done": v64[0].value, "kernel_ns": v64[1].value, "rounds": v32[0].value,
+                "active": v32[1].value, "queued": v32[2].value, "hbm_in_use": v64[2].value,
+                "hbm_budget": v64[3].value}

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstring>
+#include <new>
 #include <vector>
 
 #include "../../include/slimefinder.h"
@@ -332,6 +333,84 @@ int32_t slime_sweep_costs(double* ns_row, double* extra_rows, int32_t* nb, int32
         if (ph) ph[c] = slime::kSweepClass[c].ph;
     }
     return n;
+}
+
+void* slime_session_create(int64_t hbm_budget, int32_t max_active, int32_t round_ms) {
+    if (hbm_budget <= 0 || max_active <= 0 || round_ms <= 0) return nullptr;
+    auto* s = new (std::nothrow) slime::Session(hbm_budget, max_active, round_ms);
+    if (!s) return nullptr;
+    if (!s->ok()) {
+        delete s;
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return s;
+}
+
+void slime_session_destroy(void* session) { delete static_cast<slime::Session*>(session); }
+
+int64_t slime_session_submit(void* session, int64_t seed, int64_t rd_max, int32_t min_size,
+                             int32_t max_size, int64_t rd_min, int32_t search_center_x,
+                             int32_t search_center_z, int32_t max_results) {
+    ExtChunkResult probe;  // valid_args checks the buffer pointer only
+    if (!session || !valid_args(rd_max, kGpuRdLimit, min_size, max_size, rd_min,
+                                search_center_x, search_center_z, &probe, max_results))
+        return -1;
+    return static_cast<slime::Session*>(session)->submit(
+        square_spec(seed, rd_max, min_size, max_size, rd_min, search_center_x, search_center_z),
+        max_results);
+}
+
+int32_t slime_session_pause(void* session, int64_t id) {
+    return session && static_cast<slime::Session*>(session)->pause(id) ? 0 : -1;
+}
+
+int32_t slime_session_resume(void* session, int64_t id) {
+    return session && static_cast<slime::Session*>(session)->resume(id) ? 0 : -1;
+}
+
+int32_t slime_session_cancel(void* session, int64_t id) {
+    return session && static_cast<slime::Session*>(session)->cancel(id) ? 0 : -1;
+}
+
+int32_t slime_session_poll(void* session, int64_t id, int32_t* progress_pct, int64_t* total,
+                           int64_t* latency_ns) {
+    if (!session) return -1;
+    slime::ReqState st;
+    double prog = 0;
+    if (!static_cast<slime::Session*>(session)->poll(id, &st, &prog, total, latency_ns)) return -1;
+    if (progress_pct) *progress_pct = (int32_t)(prog * 100.0);
+    return (int32_t)st;
+}
+
+int32_t slime_session_wait(void* session, int64_t id, int32_t timeout_ms) {
+    if (!session) return -1;
+    return (int32_t) static_cast<slime::Session*>(session)->wait(id, timeout_ms);
+}
+
+int64_t slime_session_result(void* session, int64_t id, ExtChunkResult* results_buffer,
+                             int32_t cap) {
+    if (!session) return -1;
+    return static_cast<slime::Session*>(session)->result(
+        id, reinterpret_cast<slime::ResultRow*>(results_buffer), cap);
+}
+
+int32_t slime_session_release(void* session, int64_t id) {
+    return session && static_cast<slime::Session*>(session)->release(id) ? 0 : -1;
+}
+
+void slime_session_stats(void* session, int64_t* centers_done, int64_t* kernel_ns,
+                         int32_t* rounds, int32_t* active, int32_t* queued, int64_t* hbm_in_use,
+                         int64_t* hbm_budget) {
+    if (!session) return;
+    const slime::SessionStats st = static_cast<slime::Session*>(session)->stats();
+    if (centers_done) *centers_done = st.centers_done;
+    if (kernel_ns) *kernel_ns = st.kernel_ns;
+    if (rounds) *rounds = st.rounds;
+    if (active) *active = st.active;
+    if (queued) *queued = st.queued;
+    if (hbm_in_use) *hbm_in_use = st.hbm_in_use;
+    if (hbm_budget) *hbm_budget = st.hbm_budget;
 }
 
 }  // extern "C"

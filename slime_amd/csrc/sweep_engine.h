@@ -5,7 +5,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <condition_variable>
+#include <deque>
 #include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "engine.h"
@@ -154,5 +159,72 @@ int run_batch(const SearchSpec* specs, int32_t n, int32_t max_results, ResultRow
 void last_batch_stats(BatchStats* st);
 // The autotuner's measured table (kSweepClasses entries each).
 bool sweep_cost_table(double* ns_row, double* extra_rows);
+
+}  // namespace slime
+
+namespace slime {
+
+// ---- served searches: continuous batching --------------------------------------
+//
+// A session owns a sweeper and a worker thread. Requests are admitted in
+// arrival order while their device footprint fits the HBM budget and a slot
+// is free; every round the worker applies pause/resume/cancel as writes of
+// the requests' device status words, admits what fits, runs one bounded
+// launch per kernel class over all active requests (least work left first)
+// and finishes the requests whose tiles are all done.
+enum class ReqState : int32_t {
+    kQueued = 0,
+    kRunning = 1,
+    kPaused = 2,
+    kDone = 3,
+    kCancelled = 4,
+    kFailed = 5,
+};
+
+struct SessionStats {
+    int64_t centers_done = 0;  // centers of completed requests
+    int64_t kernel_ns = 0;
+    int32_t rounds = 0;
+    int32_t active = 0, queued = 0;
+    int64_t hbm_in_use = 0;
+    int64_t hbm_budget = 0;
+};
+
+class Session {
+   public:
+    Session(int64_t hbm_budget, int32_t max_active, int32_t round_ms);
+    ~Session();
+    bool ok() const { return ok_; }
+    int64_t submit(const SearchSpec& spec, int32_t keep);
+    bool pause(int64_t id);
+    bool resume(int64_t id);
+    bool cancel(int64_t id);
+    // state, progress in [0, 1], total (-1 until done), latency submit -> done
+    bool poll(int64_t id, ReqState* state, double* progress, int64_t* total, int64_t* latency_ns);
+    ReqState wait(int64_t id, int32_t timeout_ms);
+    int64_t result(int64_t id, ResultRow* out, int32_t cap);
+    bool release(int64_t id);
+    SessionStats stats();
+
+   private:
+    struct Entry;
+    void worker();
+    bool ok_ = false;
+    int64_t budget_ = 0;
+    int32_t max_active_ = 0;
+    int64_t round_ns_ = 0;
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    std::thread thread_;
+    bool stop_ = false;
+    Sweeper sw_;
+    std::map<int64_t, std::unique_ptr<Entry>> entries_;
+    std::deque<int64_t> queue_;
+    std::vector<int64_t> admitted_;
+    int64_t next_id_ = 1;
+    int64_t in_use_ = 0;
+    bool admit_blocked_ = false;  // the sweeper had no contiguous room for the queue head
+    SessionStats st_;
+};
 
 }  // namespace slime

@@ -849,3 +849,295 @@ bool sweep_cost_table(double* ns_row, double* extra_rows) {
 }
 
 }  // namespace slime
+
+// ---- served session --------------------------------------------------------------
+
+namespace slime {
+
+struct Session::Entry {
+    SweepRequest req;
+    ReqState state = ReqState::kQueued;
+    bool want_pause = false, want_resume = false, want_cancel = false;
+    Clock::time_point t_submit, t_done;
+    int64_t footprint = 0;
+    double progress = 0;
+};
+
+Session::Session(int64_t hbm_budget, int32_t max_active, int32_t round_ms) {
+    budget_ = std::max<int64_t>(hbm_budget, int64_t(64) << 20);
+    max_active_ = std::max(1, max_active);
+    round_ns_ = (int64_t)std::max(1, round_ms) * 1000000;
+    // hit regions take most of the budget; jobs are a few per request
+    const int64_t hits = std::max<int64_t>(int64_t(1) << 16, budget_ / (int64_t)sizeof(Hit));
+    if (!sw_.init(max_active_, 64 * (int64_t)max_active_ + 4096, hits) || !sw_.calibrate()) return;
+    st_.hbm_budget = budget_;
+    ok_ = true;
+    thread_ = std::thread([this] { worker(); });
+}
+
+Session::~Session() {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        stop_ = true;
+    }
+    cv_.notify_all();
+    if (thread_.joinable()) thread_.join();
+}
+
+int64_t Session::submit(const SearchSpec& spec, int32_t keep) {
+    if (!ok_ || keep <= 0) return -1;
+    auto e = std::make_unique<Entry>();
+    e->req.spec = spec;
+    e->req.keep = keep;
+    e->t_submit = Clock::now();
+    if (!sw_.plan(e->req)) return -1;  // reads only the cost table
+    e->footprint = Sweeper::footprint(e->req);
+    if (e->footprint > budget_) return -1;  // could never be admitted
+    std::lock_guard<std::mutex> lk(mu_);
+    const int64_t id = next_id_++;
+    entries_[id] = std::move(e);
+    queue_.push_back(id);
+    cv_.notify_all();
+    return id;
+}
+
+bool Session::pause(int64_t id) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = entries_.find(id);
+    if (it == entries_.end()) return false;
+    Entry& e = *it->second;
+    if (e.state != ReqState::kQueued && e.state != ReqState::kRunning) return false;
+    e.want_pause = true;
+    e.want_resume = false;
+    cv_.notify_all();
+    return true;
+}
+
+bool Session::resume(int64_t id) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = entries_.find(id);
+    if (it == entries_.end()) return false;
+    Entry& e = *it->second;
+    if (e.state != ReqState::kPaused && !e.want_pause) return false;
+    e.want_resume = true;
+    e.want_pause = false;
+    cv_.notify_all();
+    return true;
+}
+
+bool Session::cancel(int64_t id) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = entries_.find(id);
+    if (it == entries_.end()) return false;
+    Entry& e = *it->second;
+    if (e.state == ReqState::kDone || e.state == ReqState::kCancelled ||
+        e.state == ReqState::kFailed)
+        return false;
+    e.want_cancel = true;
+    cv_.notify_all();
+    return true;
+}
+
+bool Session::poll(int64_t id, ReqState* state, double* progress, int64_t* total,
+                   int64_t* latency_ns) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = entries_.find(id);
+    if (it == entries_.end()) return false;
+    const Entry& e = *it->second;
+    const bool fin = e.state == ReqState::kDone || e.state == ReqState::kCancelled;
+    if (state) *state = e.state;
+    if (progress) *progress = e.state == ReqState::kDone ? 1.0 : e.progress;
+    if (total) *total = fin ? e.req.total : -1;
+    if (latency_ns)
+        *latency_ns = fin ? std::chrono::duration_cast<std::chrono::nanoseconds>(e.t_done -
+                                                                                 e.t_submit)
+                                .count()
+                          : -1;
+    return true;
+}
+
+ReqState Session::wait(int64_t id, int32_t timeout_ms) {
+    std::unique_lock<std::mutex> lk(mu_);
+    auto finished = [&] {
+        auto it = entries_.find(id);
+        if (it == entries_.end()) return true;
+        const ReqState s = it->second->state;
+        return s == ReqState::kDone || s == ReqState::kCancelled || s == ReqState::kFailed;
+    };
+    if (timeout_ms < 0)
+        done_cv_.wait(lk, finished);
+    else
+        done_cv_.wait_for(lk, std::chrono::milliseconds(timeout_ms), finished);
+    auto it = entries_.find(id);
+    return it == entries_.end() ? ReqState::kFailed : it->second->state;
+}
+
+int64_t Session::result(int64_t id, ResultRow* out, int32_t cap) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = entries_.find(id);
+    if (it == entries_.end()) return -1;
+    const Entry& e = *it->second;
+    if (e.state != ReqState::kDone && e.state != ReqState::kCancelled) return -1;
+    const size_t n = std::min(e.req.rows.size(), (size_t)std::max(0, cap));
+    if (out && n) std::memcpy(out, e.req.rows.data(), sizeof(ResultRow) * n);
+    return e.req.total;
+}
+
+bool Session::release(int64_t id) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = entries_.find(id);
+    if (it == entries_.end()) return false;
+    const ReqState s = it->second->state;
+    if (s != ReqState::kDone && s != ReqState::kCancelled && s != ReqState::kFailed) return false;
+    entries_.erase(it);
+    return true;
+}
+
+SessionStats Session::stats() {
+    std::lock_guard<std::mutex> lk(mu_);
+    SessionStats s = st_;
+    s.active = (int32_t)admitted_.size();
+    s.queued = (int32_t)queue_.size();
+    s.hbm_in_use = in_use_;
+    return s;
+}
+
+void Session::worker() {
+    std::unique_lock<std::mutex> lk(mu_);
+    for (;;) {
+        auto has_work = [&] {
+            if (stop_) return true;
+            if (!queue_.empty() && !admit_blocked_ &&
+                (int32_t)admitted_.size() < max_active_ &&
+                in_use_ + entries_[queue_.front()]->footprint <= budget_)
+                return true;
+            for (int64_t id : queue_)
+                if (entries_[id]->want_cancel) return true;
+            for (int64_t id : admitted_) {
+                const Entry& e = *entries_[id];
+                if (e.state == ReqState::kRunning || e.want_cancel || e.want_resume ||
+                    e.want_pause)
+                    return true;
+            }
+            return false;
+        };
+        cv_.wait(lk, has_work);
+        if (stop_) break;
+
+        // 1. control edits of queued requests, then admission in arrival order
+        std::vector<Entry*> finished;
+        for (auto qi = queue_.begin(); qi != queue_.end();) {
+            Entry& e = *entries_[*qi];
+            if (e.want_cancel) {
+                e.want_cancel = false;
+                e.state = ReqState::kCancelled;
+                e.req.total = 0;
+                e.t_done = Clock::now();
+                qi = queue_.erase(qi);
+                done_cv_.notify_all();
+                continue;
+            }
+            ++qi;
+        }
+        std::vector<Entry*> admit;
+        while (!queue_.empty() && (int32_t)admitted_.size() < max_active_) {
+            Entry& e = *entries_[queue_.front()];
+            if (in_use_ + e.footprint > budget_) break;
+            if (!sw_.reserve(e.req)) {  // fragmented: wait until a request leaves
+                admit_blocked_ = true;
+                break;
+            }
+            in_use_ += e.footprint;
+            admitted_.push_back(queue_.front());
+            queue_.pop_front();
+            admit.push_back(&e);
+        }
+        // status edits of admitted requests (applied on the device below)
+        std::vector<std::pair<Entry*, int32_t>> edits;
+        std::vector<Entry*> cancelled;
+        for (int64_t id : admitted_) {
+            Entry& e = *entries_[id];
+            if (e.want_cancel) {
+                e.want_cancel = false;
+                cancelled.push_back(&e);
+            } else if (e.want_pause) {
+                e.want_pause = false;
+                if (e.state == ReqState::kRunning) edits.push_back({&e, 2});
+                else if (e.state == ReqState::kQueued) e.state = ReqState::kPaused;
+            } else if (e.want_resume) {
+                e.want_resume = false;
+                if (e.state == ReqState::kPaused) edits.push_back({&e, kReqActive});
+            }
+        }
+        std::vector<SweepRequest*> running;
+        lk.unlock();
+
+        // 2. device work without the lock
+        bool ok = true;
+        std::vector<SweepRequest*> fresh;
+        for (Entry* e : admit) fresh.push_back(&e->req);
+        if (!fresh.empty()) ok = sw_.prepare(fresh) && sw_.upload(fresh);
+        for (Entry* e : admit) {
+            // a pause that arrived while queued holds it right away
+            if (ok && e->state == ReqState::kPaused) ok = sw_.set_status(e->req, 2);
+        }
+        for (auto& ed : edits) ok = ok && sw_.set_status(ed.first->req, ed.second);
+        for (Entry* e : cancelled) ok = ok && sw_.set_status(e->req, 0);
+        RoundStats rs{};
+        lk.lock();
+        for (Entry* e : admit)
+            if (e->state == ReqState::kQueued) e->state = ReqState::kRunning;
+        for (auto& ed : edits)
+            ed.first->state = ed.second == kReqActive ? ReqState::kRunning : ReqState::kPaused;
+        for (int64_t id : admitted_) {
+            Entry& e = *entries_[id];
+            if (e.state == ReqState::kRunning &&
+                std::find(cancelled.begin(), cancelled.end(), &e) == cancelled.end())
+                running.push_back(&e.req);
+        }
+        lk.unlock();
+        if (ok && !running.empty()) {
+            RoundPolicy pol;
+            pol.round_ns = round_ns_;
+            pol.shortest_first = true;
+            ok = sw_.round(running, pol, &rs);
+        }
+        // finish completed and cancelled requests
+        for (int64_t k = 0; ok && k < (int64_t)running.size(); ++k) {
+            SweepRequest* r = running[k];
+            if (r->complete) ok = sw_.finalize(*r, false);
+        }
+        for (Entry* e : cancelled) ok = ok && sw_.finalize(e->req, true);
+
+        lk.lock();
+        st_.kernel_ns += rs.kernel_ns;
+        st_.rounds += rs.rounds;
+        const auto now = Clock::now();
+        for (auto it = admitted_.begin(); it != admitted_.end();) {
+            Entry& e = *entries_[*it];
+            e.progress = sw_.progress(e.req);
+            const bool was_cancelled =
+                std::find(cancelled.begin(), cancelled.end(), &e) != cancelled.end();
+            const bool done = e.state == ReqState::kRunning && e.req.complete;
+            if (!ok || done || was_cancelled) {
+                e.state = !ok ? ReqState::kFailed
+                              : was_cancelled ? ReqState::kCancelled : ReqState::kDone;
+                e.t_done = now;
+                if (done && ok) st_.centers_done += e.req.centers;
+                sw_.unreserve(e.req);
+                in_use_ -= e.footprint;
+                admit_blocked_ = false;
+                it = admitted_.erase(it);
+                continue;
+            }
+            ++it;
+        }
+        done_cv_.notify_all();
+        if (!ok) {
+            // a device error fails every admitted request; queued ones wait
+            (void)hipGetLastError();
+        }
+    }
+}
+
+}  // namespace slime

@@ -15,7 +15,8 @@ HEADER = os.path.join(ROOT, "include", "slimefinder.h")
 def declared_functions():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    names = re.findall(r"^\s*(?:int64_t|int32_t|void|bool)\s+(\w+)\s*\(", text, flags=re.M)
+    names = re.findall(r"^\s*(?:int64_t|int32_t|void\s*\*|void|bool)\s*(\w+)\s*\(", text,
+                       flags=re.M)
     return sorted(set(names))
 
 
