@@ -1,0 +1,77 @@
+/* Biome filter of the slime-cluster search: the reference's gpu_noise C ABI
+ * (gpu_noise/mcgpu.h, MinecraftGPUNoise.cu:1076-1105, 1837-1986) on MI355X.
+ *
+ * A context holds one world's overworld biome source (Minecraft's multi-noise
+ * sampler, parameter tree of version 26.2) on the current HIP device. Return
+ * codes: MCGPU_OK, MCGPU_ERR_ARG for invalid arguments, MCGPU_ERR_CUDA for a
+ * device error (the name is the reference's). */
+#ifndef SLIME_MCGPU_H
+#define SLIME_MCGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { MCGPU_OK = 0, MCGPU_ERR_ARG = -1, MCGPU_ERR_CUDA = -2, MCGPU_ERR_ALLOC = -3 };
+
+/* MinecraftGPUNoise.cu:1076-1079: the ABI version, 1. */
+int mcgpu_api_version(void);
+/* MinecraftGPUNoise.cu:1102-1105: a descriptive name of the build. */
+const char* mcgpu_build_name(void);
+
+/* MinecraftGPUNoise.cu:1837-1880: a context for world_seed with buffers for
+ * `capacity` candidates or points; NULL on failure. Only version codes
+ * >= 2602 (the 26.2 parameter tree) are accepted here; the reference also
+ * carries the trees of 1.18-1.21. */
+void* mcgpu_modern_ctx_create(int32_t version_code, uint64_t world_seed, int32_t capacity);
+/* MinecraftGPUNoise.cu:1950-1966: flags bit 0 = large biomes. */
+void* mcgpu_modern_ctx_create_ex(int32_t version_code, uint64_t world_seed, int32_t flags,
+                                 int32_t capacity);
+/* MinecraftGPUNoise.cu:1935-1948. */
+void mcgpu_modern_ctx_destroy(void* ctx);
+
+/* MinecraftGPUNoise.cu:1910-1933 (kernel :978-1074): for each candidate
+ * center chunk, mask_out gets bit 0 if some slime chunk of its 221-chunk disk
+ * has Deep Dark at a quart of y -16..0 and bit 1 if one has Mushroom Fields at
+ * quart y 16, for the bits requested in flags (1, 2 or 3). */
+int mcgpu_modern_ctx_filter_candidates(void* ctx, const int32_t* center_chunk_x,
+                                       const int32_t* center_chunk_z, int32_t count,
+                                       int32_t flags, int32_t* mask_out);
+
+/* MinecraftGPUNoise.cu:1882-1908: biome ids at quart coordinates. */
+int mcgpu_modern_ctx_query(void* ctx, const int32_t* xs, const int32_t* ys, const int32_t* zs,
+                           int32_t count, int32_t* biome_out);
+
+/* MinecraftGPUNoise.cu:1081-1100: a batch size for filter calls. */
+int32_t mcgpu_modern_recommended_filter_batch(void);
+
+/* Extension with no counterpart in the reference: the y-independent climate of
+ * quart columns, six int32 per point (quantized temperature, humidity,
+ * continentalness, erosion, weirdness, and the bits of the float depth-spline
+ * offset), for checking the noise stage on its own. */
+int mcgpu_modern_ctx_columns(void* ctx, const int32_t* xs, const int32_t* zs, int32_t count,
+                             int32_t* out6);
+
+/* Extension with no counterpart in the reference: device time (ms, HIP events)
+ * of the last filter call of the context. */
+float mcgpu_modern_ctx_last_filter_ms(void* ctx);
+
+/* Extension with no counterpart in the reference: the filter of candidates of
+ * many worlds in one launch (the batched search's top sites per seed). World w
+ * has seed world_seeds[w] and candidates [first[w], first[w+1]) of the arrays
+ * (first[0] == 0); flags and mask_out as in mcgpu_modern_ctx_filter_candidates;
+ * large_biomes != 0 selects the large-biome noises. Uses the 26.2 tree. */
+int mcgpu_filter_multi(int32_t nworlds, const uint64_t* world_seeds, const int32_t* first,
+                       const int32_t* center_chunk_x, const int32_t* center_chunk_z,
+                       int32_t flags, int32_t large_biomes, int32_t* mask_out);
+/* Extension with no counterpart in the reference: device time (ms, HIP events)
+ * of the last mcgpu_filter_multi launch. */
+float mcgpu_filter_multi_last_ms(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SLIME_MCGPU_H */

@@ -123,3 +123,57 @@ def ref_search(lib, seed, rd_max, min_size, max_size=221, rd_min=0, cx=0, cz=0,
                                                cx, cz, buf, max_results, threads, 0)
     rows = rows_to_tuples(buf, min(max(total, 0), max_results)) if total >= 0 else []
     return total, rows
+
+
+# ---- biome filter checker ----------------------------------------------------------
+
+_biome = None
+
+
+def biome_lib():
+    """libbiome_oracle.so: the C restatement of the biome filter."""
+    global _biome
+    if _biome is None:
+        path = os.path.join(HERE, "libbiome_oracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        u64, i32, i64, P32 = C.c_uint64, C.c_int32, C.c_int64, C.POINTER(C.c_int32)
+        L.ob_xoro_stream.argtypes = [u64, C.c_int, C.POINTER(u64)]
+        L.ob_xoro_stream.restype = None
+        L.ob_perlin_sample_seeded.argtypes = [u64, C.c_double, C.c_double, C.c_double]
+        L.ob_perlin_sample_seeded.restype = C.c_double
+        L.ob_climate_new.argtypes = [u64, C.c_int]
+        L.ob_climate_new.restype = C.c_void_p
+        L.ob_climate_free.argtypes = [C.c_void_p]
+        L.ob_climate_free.restype = None
+        L.ob_column.argtypes = [C.c_void_p, i32, i32, P32]
+        L.ob_column.restype = None
+        L.ob_biome_at.argtypes = [C.c_void_p, i32, i32, i32]
+        L.ob_biome_at.restype = C.c_int
+        L.ob_filter.argtypes = [C.c_void_p, i64, i32, i32, C.c_int]
+        L.ob_filter.restype = C.c_int
+        L.ob_spline.argtypes = [C.POINTER(C.c_float)]
+        L.ob_spline.restype = C.c_float
+        L.ob_tree_leaf.argtypes = [P32]
+        L.ob_tree_leaf.restype = C.c_int
+        L.ob_tree_leaf_exhaustive.argtypes = [P32]
+        L.ob_tree_leaf_exhaustive.restype = C.c_int
+        L.ob_node_distance.argtypes = [P32, C.c_int]
+        L.ob_node_distance.restype = u64
+        L.ob_leaf_biome.argtypes = [C.c_int]
+        L.ob_leaf_biome.restype = C.c_int
+        L.ob_tree_len.restype = C.c_int
+        _biome = L
+    return _biome
+
+
+def ref_biome_lib():
+    """_ref/libbiome_ref.so (the reference's spline source), or None."""
+    path = os.path.join(HERE, "_ref", "libbiome_ref.so")
+    if not os.path.exists(path):
+        return None
+    L = C.CDLL(path)
+    L.ref_depth_spline.argtypes = [C.POINTER(C.c_float)]
+    L.ref_depth_spline.restype = C.c_float
+    return L

@@ -35,6 +35,14 @@ EXPORTS = (
     "slime_session_release", "slime_session_stats",
 )
 
+# Every symbol include/mcgpu.h declares (the biome filter).
+BIOME_EXPORTS = (
+    "mcgpu_api_version", "mcgpu_build_name", "mcgpu_modern_ctx_create",
+    "mcgpu_modern_ctx_create_ex", "mcgpu_modern_ctx_destroy", "mcgpu_modern_ctx_filter_candidates",
+    "mcgpu_modern_ctx_query", "mcgpu_modern_recommended_filter_batch", "mcgpu_modern_ctx_columns",
+    "mcgpu_modern_ctx_last_filter_ms", "mcgpu_filter_multi", "mcgpu_filter_multi_last_ms",
+)
+
 
 class ExtChunkResult(C.Structure):
     """24-byte result row (SlimeCore.cpp:182-189)."""
@@ -126,6 +134,26 @@ def lib():
     L.slime_session_result.restype = i64
     L.slime_session_stats.argtypes = [VP, PI64, PI64, PI32, PI32, PI32, PI64, PI64]
     L.slime_session_stats.restype = None
+    L.mcgpu_api_version.restype = C.c_int
+    L.mcgpu_build_name.restype = C.c_char_p
+    L.mcgpu_modern_ctx_create.argtypes = [i32, C.c_uint64, i32]
+    L.mcgpu_modern_ctx_create.restype = VP
+    L.mcgpu_modern_ctx_create_ex.argtypes = [i32, C.c_uint64, i32, i32]
+    L.mcgpu_modern_ctx_create_ex.restype = VP
+    L.mcgpu_modern_ctx_destroy.argtypes = [VP]
+    L.mcgpu_modern_ctx_destroy.restype = None
+    L.mcgpu_modern_ctx_filter_candidates.argtypes = [VP, PI32, PI32, i32, i32, PI32]
+    L.mcgpu_modern_ctx_filter_candidates.restype = C.c_int
+    L.mcgpu_modern_ctx_query.argtypes = [VP, PI32, PI32, PI32, i32, PI32]
+    L.mcgpu_modern_ctx_query.restype = C.c_int
+    L.mcgpu_modern_ctx_columns.argtypes = [VP, PI32, PI32, i32, PI32]
+    L.mcgpu_modern_ctx_columns.restype = C.c_int
+    L.mcgpu_modern_recommended_filter_batch.restype = i32
+    L.mcgpu_modern_ctx_last_filter_ms.argtypes = [VP]
+    L.mcgpu_modern_ctx_last_filter_ms.restype = C.c_float
+    L.mcgpu_filter_multi.argtypes = [i32, C.POINTER(C.c_uint64), PI32, PI32, PI32, i32, i32, PI32]
+    L.mcgpu_filter_multi.restype = C.c_int
+    L.mcgpu_filter_multi_last_ms.restype = C.c_float
     _lib = L
     return L
 
@@ -325,3 +353,85 @@ class Session:
         return {"centers_done": v64[0].value, "kernel_ns": v64[1].value, "rounds": v32[0].value,
                 "active": v32[1].value, "queued": v32[2].value, "hbm_in_use": v64[2].value,
                 "hbm_budget": v64[3].value}
+
+
+DEEP_DARK, MUSHROOM_FIELDS = 1, 2  # filter flags / mask bits
+
+
+class BiomeContext:
+    """One world's overworld biome source on the GPU (mcgpu_modern_ctx_*):
+    the candidate filter (Deep Dark / Mushroom Fields under a cluster's slime
+    chunks) and biome point queries at quart coordinates."""
+
+    def __init__(self, world_seed, capacity=1 << 16, version=2602, large_biomes=False):
+        self._h = lib().mcgpu_modern_ctx_create_ex(version, world_seed & ((1 << 64) - 1),
+                                                   1 if large_biomes else 0, capacity)
+        if not self._h:
+            raise RuntimeError("mcgpu_modern_ctx_create_ex failed")
+        self.capacity = capacity
+
+    def close(self):
+        if self._h:
+            lib().mcgpu_modern_ctx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @staticmethod
+    def _arr(v):
+        return (C.c_int32 * max(1, len(v)))(*v)
+
+    def filter(self, center_x, center_z, flags=DEEP_DARK | MUSHROOM_FIELDS):
+        n = len(center_x)
+        out = (C.c_int32 * max(1, n))()
+        rc = lib().mcgpu_modern_ctx_filter_candidates(self._h, self._arr(center_x),
+                                                      self._arr(center_z), n, flags, out)
+        if rc != 0:
+            raise RuntimeError(f"filter failed ({rc})")
+        return list(out)[:n]
+
+    def query(self, xs, ys, zs):
+        n = len(xs)
+        out = (C.c_int32 * max(1, n))()
+        rc = lib().mcgpu_modern_ctx_query(self._h, self._arr(xs), self._arr(ys), self._arr(zs), n,
+                                          out)
+        if rc != 0:
+            raise RuntimeError(f"query failed ({rc})")
+        return list(out)[:n]
+
+    def columns(self, xs, zs):
+        n = len(xs)
+        out = (C.c_int32 * max(1, 6 * n))()
+        rc = lib().mcgpu_modern_ctx_columns(self._h, self._arr(xs), self._arr(zs), n, out)
+        if rc != 0:
+            raise RuntimeError(f"columns failed ({rc})")
+        return [tuple(out[6 * i:6 * i + 6]) for i in range(n)]
+
+    def last_filter_ms(self):
+        return lib().mcgpu_modern_ctx_last_filter_ms(self._h)
+
+
+def filter_multi(world_seeds, candidates, flags=DEEP_DARK | MUSHROOM_FIELDS, large_biomes=False):
+    """Biome filter of candidates of many worlds in one launch: candidates[w]
+    is a list of (center_chunk_x, center_chunk_z) of world world_seeds[w].
+    Returns the masks per world and the device time in ms."""
+    n = len(world_seeds)
+    first = [0]
+    for c in candidates:
+        first.append(first[-1] + len(c))
+    flat = [c for cs in candidates for c in cs]
+    total = first[-1]
+    seeds = (C.c_uint64 * max(1, n))(*[s & ((1 << 64) - 1) for s in world_seeds])
+    f = (C.c_int32 * (n + 1))(*first)
+    xs = (C.c_int32 * max(1, total))(*[c[0] for c in flat])
+    zs = (C.c_int32 * max(1, total))(*[c[1] for c in flat])
+    out = (C.c_int32 * max(1, total))()
+    rc = lib().mcgpu_filter_multi(n, seeds, f, xs, zs, flags, 1 if large_biomes else 0, out)
+    if rc != 0:
+        raise RuntimeError(f"mcgpu_filter_multi failed ({rc})")
+    masks = [list(out[first[w]:first[w + 1]]) for w in range(n)]
+    return masks, lib().mcgpu_filter_multi_last_ms()

@@ -10,13 +10,14 @@ import pytest
 from conftest import ROOT, load_golden
 
 HEADER = os.path.join(ROOT, "include", "slimefinder.h")
+BIOME_HEADER = os.path.join(ROOT, "include", "mcgpu.h")
 
 
-def declared_functions():
-    text = open(HEADER).read()
+def declared_functions(header=HEADER):
+    text = open(header).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    names = re.findall(r"^\s*(?:int64_t|int32_t|void\s*\*|void|bool)\s*(\w+)\s*\(", text,
-                       flags=re.M)
+    names = re.findall(r"^\s*(?:int64_t|int32_t|int|float|const char\s*\*|void\s*\*|void|bool)"
+                       r"\s*(\w+)\s*\(", text, flags=re.M)
     return sorted(set(names))
 
 
@@ -30,6 +31,15 @@ def test_library_exports_every_declared_symbol(native):
 def test_python_mirror_lists_every_export():
     import slime_amd
     assert sorted(slime_amd.EXPORTS) == declared_functions()
+    assert sorted(slime_amd.BIOME_EXPORTS) == declared_functions(BIOME_HEADER)
+
+
+def test_library_exports_every_biome_symbol(native):
+    names = declared_functions(BIOME_HEADER)
+    assert len(names) == 12
+    for name in names:
+        assert hasattr(native, name), name
+    assert native.mcgpu_api_version() == 1
 
 
 def test_invalid_arguments_return_minus_one(native):

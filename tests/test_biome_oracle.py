@@ -1,0 +1,248 @@
+"""The biome filter's CPU checker (oracle/biome_oracle.c) and the data it
+shares with the kernels (slime_amd/csrc/biome_data.inc), pinned without the
+reference's CUDA build:
+  * noise seeds: MD5 digests of Minecraft's noise names;
+  * xoroshiro128++ with Minecraft's seeding, and improved Perlin noise:
+    against independent Python restatements of the published algorithms;
+  * the terrain spline: bit-exact against the reference's own spline source
+    compiled for the CPU (oracle/_ref, where built);
+  * the parameter tree: the generated table equals the reference's table text;
+    the pruned nearest-leaf search equals an exhaustive search.
+Biome ids against the reference's CUDA build or cubiomes: parity unpinned.
+"""
+import ctypes as C
+import hashlib
+import math
+import os
+import random
+import re
+import struct
+
+import pytest
+
+from conftest import ROOT
+
+INC = os.path.join(ROOT, "slime_amd", "csrc", "biome_data.inc")
+REF_TREE = "/root/reference/gpu_noise/tables/btree262.h"
+M64 = (1 << 64) - 1
+
+
+@pytest.fixture(scope="module")
+def bl():
+    from oracle import pyoracle
+    return pyoracle.biome_lib()
+
+
+def inc_text():
+    return open(INC).read()
+
+
+def md5_halves(name):
+    d = hashlib.md5(name.encode()).digest()
+    return int.from_bytes(d[:8], "big"), int.from_bytes(d[8:], "big")
+
+
+def test_noise_seeds_are_md5_of_noise_names():
+    text = inc_text()
+    rows = re.findall(r"\{0x([0-9a-f]{16})ull, 0x([0-9a-f]{16})ull, 0x([0-9a-f]{16})ull, "
+                      r"0x([0-9a-f]{16})ull, (-?\d+), (-?\d+), (\d+),", text)
+    names = [("minecraft:offset", None), ("minecraft:temperature", "minecraft:temperature_large"),
+             ("minecraft:vegetation", "minecraft:vegetation_large"),
+             ("minecraft:continentalness", "minecraft:continentalness_large"),
+             ("minecraft:erosion", "minecraft:erosion_large"), ("minecraft:ridge", None)]
+    assert len(rows) == 6
+    for (lo, hi, llo, lhi, *_), (n, large) in zip(rows, names):
+        assert (int(lo, 16), int(hi, 16)) == md5_halves(n)
+        assert (int(llo, 16), int(lhi, 16)) == md5_halves(large or n)
+    octs = re.findall(r"\{0x([0-9a-f]{16})ull, 0x([0-9a-f]{16})ull\},\s*// octave_(-?\d+)", text)
+    assert len(octs) == 17
+    for lo, hi, i in octs:
+        assert (int(lo, 16), int(hi, 16)) == md5_halves(f"octave_{i}")
+
+
+# -- independent restatements of the published algorithms ---------------------------
+
+def rotl(x, r):
+    return ((x << r) | (x >> (64 - r))) & M64
+
+
+def mix13(z):
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+    return z ^ (z >> 31)
+
+
+class Xoroshiro128PlusPlus:
+    """Blackman & Vigna's xoroshiro128++, seeded as Minecraft's
+    RandomSupport.upgradeSeedTo128bit (silver/golden ratio, Stafford mix 13)."""
+
+    def __init__(self, seed):
+        lo = (seed ^ 0x6A09E667F3BCC909) & M64
+        hi = (lo + 0x9E3779B97F4A7C15) & M64
+        self.s0, self.s1 = mix13(lo), mix13(hi)
+
+    def next_long(self):
+        s0, s1 = self.s0, self.s1
+        r = (rotl((s0 + s1) & M64, 17) + s0) & M64
+        s1 ^= s0
+        self.s0 = rotl(s0, 49) ^ s1 ^ ((s1 << 21) & M64)
+        self.s1 = rotl(s1, 28)
+        return r
+
+    def next_int(self, n):
+        while True:
+            r = (self.next_long() & 0xFFFFFFFF) * n
+            if (r & 0xFFFFFFFF) >= (-n % (1 << 32)) % n:
+                return r >> 32
+
+    def next_double(self):
+        return (self.next_long() >> 11) * 2.0 ** -53
+
+
+GRAD = [(1, 1, 0), (-1, 1, 0), (1, -1, 0), (-1, -1, 0), (1, 0, 1), (-1, 0, 1), (1, 0, -1),
+        (-1, 0, -1), (0, 1, 1), (0, -1, 1), (0, 1, -1), (0, -1, -1), (1, 1, 0), (0, -1, 1),
+        (-1, 1, 0), (0, -1, -1)]
+
+
+class ImprovedNoise:
+    """Ken Perlin's improved noise as Minecraft seeds it (origin from three
+    doubles, Fisher-Yates permutation)."""
+
+    def __init__(self, rng):
+        self.o = [rng.next_double() * 256.0 for _ in range(3)]
+        p = list(range(256))
+        for i in range(256):
+            j = i + rng.next_int(256 - i)
+            p[i], p[j] = p[j], p[i]
+        self.p = p + [p[0]]
+
+    @staticmethod
+    def fade(t):
+        return t * t * t * (t * (t * 6.0 - 15.0) + 10.0)
+
+    @staticmethod
+    def lerp(t, a, b):
+        return a + t * (b - a)
+
+    def grad(self, h, x, y, z):
+        g = GRAD[h & 15]
+        terms = [v if s > 0 else -v for s, v in zip(g, (x, y, z)) if s]
+        return terms[0] + terms[1]
+
+    def sample(self, x, y, z):
+        x, y, z = x + self.o[0], y + self.o[1], z + self.o[2]
+        fx, fy, fz = math.floor(x), math.floor(y), math.floor(z)
+        x, y, z = x - fx, y - fy, z - fz
+        X, Y, Z = int(fx) & 255, int(fy) & 255, int(fz) & 255
+        u, v, w = self.fade(x), self.fade(y), self.fade(z)
+        p = self.p
+        a, b = (p[X] + Y) & 255, (p[X + 1] + Y) & 255
+        aa, ba, ab, bb = (p[a] + Z) & 255, (p[b] + Z) & 255, (p[a + 1] + Z) & 255, (p[b + 1] + Z) & 255
+        L = self.lerp
+        g = self.grad
+        x0 = L(u, g(p[aa], x, y, z), g(p[ba], x - 1, y, z))
+        x1 = L(u, g(p[ab], x, y - 1, z), g(p[bb], x - 1, y - 1, z))
+        x2 = L(u, g(p[aa + 1], x, y, z - 1), g(p[ba + 1], x - 1, y, z - 1))
+        x3 = L(u, g(p[ab + 1], x, y - 1, z - 1), g(p[bb + 1], x - 1, y - 1, z - 1))
+        return L(w, L(v, x0, x1), L(v, x2, x3))
+
+
+def test_xoroshiro_matches_published_algorithm(bl):
+    for seed in (0, 1, 12345, M64, 0x5EED5EED5EED):
+        ref = Xoroshiro128PlusPlus(seed)
+        want = [ref.next_long() for _ in range(64)]
+        got = (C.c_uint64 * 64)()
+        bl.ob_xoro_stream(seed, 64, got)
+        assert list(got) == want, seed
+
+
+def test_perlin_matches_published_algorithm(bl):
+    rng = random.Random(8)
+    for seed in (0, 7, 0x123456789ABCDEF, M64 - 3):
+        noise = ImprovedNoise(Xoroshiro128PlusPlus(seed))
+        for _ in range(200):
+            x, y, z = (rng.uniform(-3e6, 3e6), rng.choice([0.0, rng.uniform(-512, 512)]),
+                       rng.uniform(-3e6, 3e6))
+            assert bl.ob_perlin_sample_seeded(seed, x, y, z) == noise.sample(x, y, z)
+
+
+def f32(v):
+    return struct.unpack("<f", struct.pack("<f", v))[0]
+
+
+def test_spline_equals_reference_source(bl):
+    from oracle import pyoracle
+    ref = pyoracle.ref_biome_lib()
+    if ref is None:
+        pytest.skip("oracle/_ref/libbiome_ref.so not built (needs the reference tree)")
+    rng = random.Random(3)
+    text = inc_text()
+    knots = [struct.unpack("<f", struct.pack("<I", int(h, 16)))[0]
+             for h in re.findall(r"\{0x([0-9a-f]{8})u, 0x[0-9a-f]{8}u, 0x[0-9a-f]{8}u, -?\d+\}", text)]
+    cases = 0
+    for i in range(20000):
+        v = [rng.uniform(-1.3, 1.3) for _ in range(4)]
+        if i % 4 == 0:  # land exactly on, just below and just above a knot
+            k = rng.choice(knots)
+            v[rng.randrange(3)] = k + rng.choice([0.0, -1e-7, 1e-7])
+        v[2] = f32(-(abs(abs(v[3]) - 0.6666667) - 0.33333334) * 3.0) if i % 2 else v[2]
+        a = (C.c_float * 4)(*[f32(x) for x in v])
+        assert struct.pack("<f", bl.ob_spline(a)) == struct.pack("<f", ref.ref_depth_spline(a)), v
+        cases += 1
+    assert cases == 20000
+
+
+def test_tree_table_equals_reference_text():
+    if not os.path.exists(REF_TREE):
+        pytest.skip("reference tree table not present")
+    ref = open(REF_TREE).read()
+    ref_nodes = [int(h, 16) for h in re.findall(r"\b0x([0-9A-Fa-f]{16})\b", ref.split("_nodes[]", 1)[1])]
+    ref_param = [(int(a), int(b)) for a, b in re.findall(
+        r"\{\s*(-?\d+)\s*,\s*(-?\d+)\s*\}", re.sub(r"//[^\n]*", "", ref.split("_param[][2]", 1)[1].split("};", 1)[0]))]
+    text = inc_text()
+    nodes = [int(h, 16) for h in re.findall(r"0x([0-9a-f]{16})ull", text.split("kTreeNodes", 1)[1].split("};", 1)[0])]
+    param = [(int(a), int(b)) for a, b in re.findall(
+        r"\{(-?\d+), (-?\d+)\}", text.split("kTreeParams", 1)[1].split("};", 1)[0])]
+    assert nodes == ref_nodes and len(nodes) == 9113
+    assert param == ref_param and len(param) == 143
+
+
+def test_pruned_tree_search_is_nearest(bl):
+    rng = random.Random(11)
+    ties = 0
+    for i in range(3000):
+        np6 = [rng.randint(-12000, 12000) for _ in range(6)]
+        if i % 3 == 0:
+            np6[4] = rng.randint(-2000, 12000)  # depth-like values
+        a = (C.c_int32 * 6)(*np6)
+        leaf = bl.ob_tree_leaf(a)
+        best = bl.ob_tree_leaf_exhaustive(a)
+        d_leaf, d_best = bl.ob_node_distance(a, leaf), bl.ob_node_distance(a, best)
+        assert d_leaf == d_best, np6
+        if leaf != best:
+            ties += 1
+    # equal-distance leaves exist, but the minimum is unique for most points
+    assert ties < 300
+
+
+def test_biome_ids_and_filter_on_a_small_world(bl):
+    """Every quart of a small area gets a leaf's biome id; the filter of a
+    candidate equals the union over its disk's slime chunks' quarts."""
+    text = inc_text()
+    nodes = [int(h, 16) for h in re.findall(r"0x([0-9a-f]{16})ull", text.split("kTreeNodes", 1)[1].split("};", 1)[0])]
+    leaf_ids = {(n >> 48) & 0xFF for n in nodes if n >> 56 == 0xFF}
+    c = bl.ob_climate_new(1234, 0)
+    try:
+        seen = set()
+        rng = random.Random(21)
+        for _ in range(400):
+            x, z = rng.randint(-60000, 60000), rng.randint(-60000, 60000)
+            for y in (-16, -8, 0, 16):
+                b = bl.ob_biome_at(c, x, y, z)
+                assert b in leaf_ids
+                seen.add(b)
+        assert len(seen) >= 10
+        m = bl.ob_filter(c, 1234, 3, -7, 3)
+        assert 0 <= m <= 3
+    finally:
+        bl.ob_climate_free(c)
