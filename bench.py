@@ -21,6 +21,9 @@ Two more legs ride in the same line:
            run as one sweep (persistent kernels, optionally replayed from a
            hipGraph) against 256 separate calls of the same library and 256
            calls of its plain per-chunk kernel; all results must be identical.
+           Their top sites (up to 1000 per search) then go through the biome
+           filter (Deep Dark / Mushroom Fields under the sites' slime chunks,
+           Minecraft 26.2) in one launch, against one context per world.
   served   Poisson arrivals of the served mix (70% radius 256..4096, 25%
            radius 50,000, 5% full world; thresholds 50..65; 10% pause then
            resume, 5% cancel) into one session, against the same arrivals
@@ -151,7 +154,54 @@ def batched_leg(repeats):
     out["speedup_vs_per_search"] = best / out["per_search"]["t_centers_per_s"]
     out["speedup_vs_per_search_plain_kernel"] = best / out["per_search_plain_kernel"]["t_centers_per_s"]
     out["identical"] = per == res and plain == res
+    out["biome"] = biome_leg(searches, res)
     return out
+
+
+FP64_PEAK_TF = 78.6          # MI355X dense FP64 vector peak
+FLOPS_PER_COLUMN = 52 * 60   # 52 double Perlin octave samples of ~60 FP64 ops
+
+
+def biome_leg(searches, res):
+    """The batched + biome configuration: every search's top sites (up to 1000)
+    through the Deep Dark / Mushroom Fields filter of its own world, in one
+    launch, against one context per world (the reference's way of calling)."""
+    import slime_amd
+    seeds = [s[0] for s in searches]
+    cands = [[((r[1] - 8) // 16, (r[2] - 8) // 16) for r in rows] for _, rows in res]
+    sizes = [r[0] for _, rows in res for r in rows]
+    n = len(sizes)
+    slime_amd.filter_multi(seeds, cands, 3)  # warm-up
+    t0 = time.perf_counter()
+    masks, ms = slime_amd.filter_multi(seeds, cands, 3)
+    wall = time.perf_counter() - t0
+    flat = [m for ws in masks for m in ws]
+    t0 = time.perf_counter()
+    per, per_ms = [], 0.0
+    for seed, cs in zip(seeds, cands):
+        if not cs:
+            per.append([])
+            continue
+        with slime_amd.BiomeContext(seed, capacity=len(cs)) as ctx:
+            per.append(ctx.filter([c[0] for c in cs], [c[1] for c in cs], 3))
+            per_ms += ctx.last_filter_ms()
+    per_wall = time.perf_counter() - t0
+    columns = 16 * sum(sizes)  # quart columns under the sites' slime chunks (before early exits)
+    achieved = columns * FLOPS_PER_COLUMN / (ms / 1e3) / 1e12
+    print(f"biome filter: {n} sites in {ms:.1f} ms", file=sys.stderr, flush=True)
+    return {
+        "workload": "top sites (<= 1000) of each batched search, flags Deep Dark | Mushroom Fields, "
+                    "version 26.2",
+        "sites": n, "deep_dark": sum(1 for m in flat if m & 1),
+        "mushroom_fields": sum(1 for m in flat if m & 2),
+        "one_launch": {"kernel_ms": ms, "wall_s": wall, "sites_per_s": n / (ms / 1e3)},
+        "per_world_contexts": {"kernel_ms": per_ms, "wall_s": per_wall},
+        "identical": per == masks,
+        "roofline": {"bound": "fp64", "unit": "TFLOP/s", "peak": FP64_PEAK_TF,
+                     "achieved": achieved, "frac": achieved / FP64_PEAK_TF,
+                     "flops_per_column": FLOPS_PER_COLUMN, "columns": columns,
+                     "note": "the parameter-tree search (integer, divergent) is most of the time"},
+    }
 
 
 def percentile(xs, q):

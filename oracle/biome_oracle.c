@@ -16,9 +16,11 @@
  *
  * Pinning (tests/test_biome_oracle.py): the noise seeds are MD5 digests of the
  * noise names; xoroshiro and Perlin against independent Python restatements
- * of the published algorithms; the spline and tree tables against the
- * reference's own spline source and tree table compiled in oracle/_ref; the
- * pruned tree search against an exhaustive nearest-leaf search. Biome ids
+ * of the published algorithms; the spline against the reference's own spline
+ * source compiled in oracle/_ref; the tree table against the reference's
+ * table text; the pruned tree search against an exhaustive nearest-leaf
+ * search, and the search started from a known leaf against the search from
+ * the root. Biome ids
  * against the reference's CUDA build or cubiomes: parity unpinned (neither
  * runs here). */
 #include <math.h>
@@ -324,6 +326,13 @@ static int nearest(const int32_t* np, int depth, int idx, int best, uint64_t* be
 int ob_tree_leaf(const int32_t* np) {
     uint64_t ds = ~0ull;
     return nearest(np, 0, 0, 0, &ds);
+}
+
+/* The search started from a known leaf with the bound distance(hint) + 1
+ * (what the kernels do from one y layer to the next). */
+int ob_tree_leaf_from(const int32_t* np, int hint) {
+    uint64_t ds = ob_node_distance(np, hint) + 1;
+    return nearest(np, 0, 0, hint, &ds);
 }
 
 int ob_tree_leaf_exhaustive(const int32_t* np) {

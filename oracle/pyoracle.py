@@ -157,6 +157,8 @@ def biome_lib():
         L.ob_spline.restype = C.c_float
         L.ob_tree_leaf.argtypes = [P32]
         L.ob_tree_leaf.restype = C.c_int
+        L.ob_tree_leaf_from.argtypes = [P32, C.c_int]
+        L.ob_tree_leaf_from.restype = C.c_int
         L.ob_tree_leaf_exhaustive.argtypes = [P32]
         L.ob_tree_leaf_exhaustive.restype = C.c_int
         L.ob_node_distance.argtypes = [P32, C.c_int]

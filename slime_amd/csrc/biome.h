@@ -111,26 +111,17 @@ struct Perlin {
 SLIME_HD double fade(double t) { return t * t * t * (t * (t * 6.0 - 15.0) + 10.0); }
 SLIME_HD double lerp(double t, double a, double b) { return a + t * (b - a); }
 
-// Dot product with one of Perlin's 16 gradient vectors.
+// Dot product with one of Perlin's 16 gradient vectors. Every vector has two
+// non-zero unit components, so the product is (+-a) + (+-b) with a in {x, y}
+// and b in {y, z}: selects and sign flips instead of a 16-way branch (the
+// same IEEE result: x - y is x + (-y)).
 SLIME_HD double grad(uint8_t h, double x, double y, double z) {
-    switch (h & 15) {
-        case 0: return x + y;
-        case 1: return -x + y;
-        case 2: return x - y;
-        case 3: return -x - y;
-        case 4: return x + z;
-        case 5: return -x + z;
-        case 6: return x - z;
-        case 7: return -x - z;
-        case 8: return y + z;
-        case 9: return -y + z;
-        case 10: return y - z;
-        case 11: return -y - z;
-        case 12: return x + y;
-        case 13: return -y + z;
-        case 14: return -x + y;
-        default: return -y - z;
-    }
+    const int k = h & 15;
+    const double a = (0xAF00u >> k) & 1u ? y : x;        // hashes 8-11, 13, 15
+    const double b = (0xAFF0u >> k) & 1u ? z : y;         // hashes 4-11, 13, 15
+    const bool na = (0xEAAAu >> k) & 1u;                  // 1, 3, 5, 7, 9, 11, 13, 14, 15
+    const bool nb = (0x8CCCu >> k) & 1u;                  // 2, 3, 6, 7, 10, 11, 15
+    return (na ? -a : a) + (nb ? -b : b);
 }
 
 // Origins from three doubles, then a Fisher-Yates shuffle of 0..255.
@@ -352,13 +343,15 @@ SLIME_HD int32_t depth_at(float offset, int32_t yi) {
 // dimension; bits 48..63 are the first child's index (children are `step`
 // apart) or, with the top byte 0xFF, a leaf whose byte 6 is the biome id.
 
-SLIME_HD uint64_t node_distance(const int32_t np[6], const uint64_t* nodes, int idx) {
+// params: the intervals, flattened (lo, hi) pairs (kTreeParams, or a copy).
+SLIME_HD uint64_t node_distance(const int32_t np[6], const uint64_t* nodes, const int32_t* params,
+                                int idx) {
     const uint64_t node = nodes[idx];
     uint64_t ds = 0;
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
         const int pi = (int)((node >> (8 * i)) & 0xFFu);
-        const int64_t lo = kTreeParams[pi][0], hi = kTreeParams[pi][1], v = np[i];
+        const int64_t lo = params[2 * pi], hi = params[2 * pi + 1], v = np[i];
         const int64_t d = v > hi ? v - hi : (v < lo ? lo - v : 0);
         ds += (uint64_t)(d * d);
     }
@@ -369,19 +362,20 @@ SLIME_HD uint64_t node_distance(const int32_t np[6], const uint64_t* nodes, int 
 // the children whose box is closer than the best leaf so far, in order, and
 // keep a leaf only if strictly closer. Returns the leaf index.
 template <int D>
-SLIME_HD int tree_nearest(const int32_t np[6], const uint64_t* nodes, int len, int idx, int best,
-                          uint64_t best_ds) {
+SLIME_HD int tree_nearest(const int32_t np[6], const uint64_t* nodes, const int32_t* params,
+                          int len, int idx, int best, uint64_t best_ds) {
     if constexpr (D >= kTreeDepth) {
         return idx;
     } else {
         const int step = kTreeSteps[D];
-        if (idx + step >= len) return tree_nearest<D + 1>(np, nodes, len, idx, best, best_ds);
+        if (idx + step >= len)
+            return tree_nearest<D + 1>(np, nodes, params, len, idx, best, best_ds);
         int child = (int)((nodes[idx] >> 48) & 0xFFFFu);
         for (int k = 0; k < kTreeOrder && child < len; ++k, child += step) {
-            const uint64_t dc = node_distance(np, nodes, child);
+            const uint64_t dc = node_distance(np, nodes, params, child);
             if (dc < best_ds) {
-                const int leaf = tree_nearest<D + 1>(np, nodes, len, child, best, best_ds);
-                const uint64_t dl = leaf == child ? dc : node_distance(np, nodes, leaf);
+                const int leaf = tree_nearest<D + 1>(np, nodes, params, len, child, best, best_ds);
+                const uint64_t dl = leaf == child ? dc : node_distance(np, nodes, params, leaf);
                 if (dl < best_ds) {
                     best_ds = dl;
                     best = leaf;
@@ -392,18 +386,34 @@ SLIME_HD int tree_nearest(const int32_t np[6], const uint64_t* nodes, int len, i
     }
 }
 
-SLIME_HD int biome_of(const int32_t np[6], const uint64_t* nodes, int len) {
-    const int leaf = tree_nearest<0>(np, nodes, len, 0, 0, ~0ull);
+// The leaf the search from the root finds: the first leaf in visiting order
+// at the least distance (a box never lies farther than the leaves below it,
+// so that leaf's branch is always entered). Starting from a known leaf `hint`
+// with the bound distance(hint) + 1 prunes more and returns the same leaf:
+// the bound still exceeds the least distance, and only strictly closer leaves
+// are taken. Returns the leaf; hint < 0 starts without one.
+SLIME_HD int tree_leaf(const int32_t np[6], const uint64_t* nodes, const int32_t* params, int len,
+                       int hint) {
+    if (hint < 0) return tree_nearest<0>(np, nodes, params, len, 0, 0, ~0ull);
+    const uint64_t bound = node_distance(np, nodes, params, hint) + 1;
+    return tree_nearest<0>(np, nodes, params, len, 0, hint, bound);
+}
+
+SLIME_HD int leaf_biome(const uint64_t* nodes, int leaf) {
     return (int)((nodes[leaf] >> 48) & 0xFFu);
 }
 
-// Biome at quart (x, y, z) from a column's climate.
-SLIME_HD int biome_at_column(const Column& col, int32_t yi, const uint64_t* nodes, int len) {
+// Biome at quart layer yi of a column; `hint` carries the last leaf found
+// (a start bound for the next search) and is updated.
+SLIME_HD int biome_at_column(const Column& col, int32_t yi, const uint64_t* nodes,
+                             const int32_t* params, int len, int& hint) {
     const int32_t np[6] = {col.t, col.h, col.c, col.e, depth_at(col.offset, yi), col.w};
-    return biome_of(np, nodes, len);
+    hint = tree_leaf(np, nodes, params, len, hint);
+    return leaf_biome(nodes, hint);
 }
 
 constexpr int kTreeLen = (int)(sizeof(kTreeNodes) / sizeof(kTreeNodes[0]));
+constexpr int kTreeParamCount = (int)(sizeof(kTreeParams) / sizeof(kTreeParams[0]));
 
 }  // namespace biome
 }  // namespace slime

@@ -41,17 +41,67 @@ static_assert(sizeof(Climate) % 4 == 0, "climate copied as words");
 
 // LDS of a filtering workgroup besides its climate.
 struct FilterLds {
+    int32_t params[2 * kTreeParamCount];  // the tree's intervals (read with divergent indices)
     uint16_t chunks[kDiskCells];  // slime chunks of the disk (index 17 * (dx+8) + dz+8)
     int nchunks;
     unsigned int mask;
 };
 
+// The leaves of the filter's two biomes (host: found in the tree once).
+struct TargetLeaves {
+    int32_t deep[4], ndeep;
+    int32_t mush[4], nmush;
+};
+
+TargetLeaves find_targets() {
+    TargetLeaves t{};
+    for (int i = 0; i < kTreeLen; ++i) {
+        if ((kTreeNodes[i] >> 56) != 0xFFu) continue;
+        const int b = (int)((kTreeNodes[i] >> 48) & 0xFFu);
+        if (b == kDeepDark && t.ndeep < 4) t.deep[t.ndeep++] = i;
+        if (b == kMushroomFields && t.nmush < 4) t.mush[t.nmush++] = i;
+    }
+    return t;
+}
+
+// Whether the search at np can return one of `leaves`: not if every one of
+// them lies strictly farther than the known leaf `hint` (the nearest leaf is
+// no farther than hint). A biome with no leaf in the tree is never found.
+__device__ __forceinline__ bool may_be(const int32_t np[6], const uint64_t* nodes,
+                                       const int32_t* params, int hint, const int32_t* leaves,
+                                       int nleaves) {
+    if (nleaves == 0) return false;
+    if (hint < 0) return true;
+    const uint64_t known = node_distance(np, nodes, params, hint);
+    for (int i = 0; i < nleaves; ++i)
+        if (node_distance(np, nodes, params, leaves[i]) <= known) return true;
+    return false;
+}
+
+// Whether layer yi of a column is the biome of `leaves`, searching only when
+// a target leaf could be the nearest; hint carries the last leaf found.
+__device__ __forceinline__ bool layer_is(const Column& col, int32_t yi, const uint64_t* nodes,
+                                        const int32_t* params, int32_t len, int& hint,
+                                        const int32_t* leaves, int nleaves, int biome) {
+    const int32_t np[6] = {col.t, col.h, col.c, col.e, depth_at(col.offset, yi), col.w};
+    if (!may_be(np, nodes, params, hint, leaves, nleaves)) return false;
+    hint = tree_leaf(np, nodes, params, len, hint);
+    return leaf_biome(nodes, hint) == biome;
+}
+
+__device__ __forceinline__ void load_params(int32_t* dst) {
+    for (int i = threadIdx.x; i < 2 * kTreeParamCount; i += blockDim.x)
+        dst[i] = kTreeParams[i >> 1][i & 1];
+}
+
 // The mask of one candidate, by the whole workgroup (every thread calls it;
 // the caller synchronises before the first call when the LDS was in use).
 __device__ __forceinline__ unsigned int filter_candidate(const Climate& st, FilterLds& f,
-                                                         const uint64_t* nodes, int32_t len,
+                                                         const uint64_t* nodes,
+                                                         const int32_t* params, int32_t len,
+                                                         const TargetLeaves& tl,
                                                          int64_t world_seed, int32_t cx,
-                                                         int32_t cz, int32_t flags) {
+                                                         int32_t cz, int32_t flags, int& hint) {
     if (threadIdx.x == 0) {
         f.nchunks = 0;
         f.mask = 0u;
@@ -77,10 +127,13 @@ __device__ __forceinline__ unsigned int filter_candidate(const Climate& st, Filt
         const int32_t qz = (int32_t)(((uint32_t)cz + (uint32_t)dz) * 4u + (uint32_t)(q & 3));
         const Column col = column_at(st, qx, qz);
         unsigned int found = 0;
-        if ((want & 2u) && biome_at_column(col, 16, nodes, len) == kMushroomFields) found |= 2u;
+        // hint: the thread's last leaf, a bound for every later layer and column
+        if ((want & 2u) && layer_is(col, 16, nodes, params, len, hint, tl.mush, tl.nmush,
+                                    kMushroomFields))
+            found |= 2u;
         if (want & 1u) {
-            for (int32_t y = -16; y <= 0; ++y) {
-                if (biome_at_column(col, y, nodes, len) == kDeepDark) {
+            for (int32_t y = 0; y >= -16; --y) {
+                if (layer_is(col, y, nodes, params, len, hint, tl.deep, tl.ndeep, kDeepDark)) {
                     found |= 1u;
                     break;
                 }
@@ -94,16 +147,19 @@ __device__ __forceinline__ unsigned int filter_candidate(const Climate& st, Filt
 
 __global__ __launch_bounds__(kThreads) void filter_kernel(const Climate* climate,
                                                           const uint64_t* nodes, int32_t len,
-                                                          int64_t world_seed, const int32_t* ccx,
-                                                          const int32_t* ccz, int32_t count,
-                                                          int32_t flags, int32_t* mask_out) {
+                                                          TargetLeaves tl, int64_t world_seed,
+                                                          const int32_t* ccx, const int32_t* ccz,
+                                                          int32_t count, int32_t flags,
+                                                          int32_t* mask_out) {
     __shared__ Climate st;
     __shared__ FilterLds f;
     load_climate(st, climate);
+    load_params(f.params);
+    int hint = -1;
     for (int cand = blockIdx.x; cand < count; cand += gridDim.x) {
         __syncthreads();  // the previous candidate's LDS is no longer read
-        const unsigned int m = filter_candidate(st, f, nodes, len, world_seed, ccx[cand],
-                                                ccz[cand], flags);
+        const unsigned int m = filter_candidate(st, f, nodes, f.params, len, tl, world_seed,
+                                                ccx[cand], ccz[cand], flags, hint);
         if (threadIdx.x == 0) mask_out[cand] = (int32_t)m;
     }
 }
@@ -113,13 +169,15 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(const Climate* climate
 // so it reloads its LDS climate only where the run crosses into a new world.
 __global__ __launch_bounds__(kThreads) void filter_multi_kernel(
     const Climate* climates, const int64_t* seeds, const int32_t* first, int32_t nworlds,
-    const uint64_t* nodes, int32_t len, const int32_t* ccx, const int32_t* ccz, int32_t count,
-    int32_t per_block, int32_t flags, int32_t* mask_out) {
+    const uint64_t* nodes, int32_t len, TargetLeaves tl, const int32_t* ccx, const int32_t* ccz,
+    int32_t count, int32_t per_block, int32_t flags, int32_t* mask_out) {
     __shared__ Climate st;
     __shared__ FilterLds f;
+    load_params(f.params);
     const int begin = blockIdx.x * per_block;
     const int end = min(count, begin + per_block);
     int world = -1;
+    int hint = -1;
     for (int cand = begin; cand < end; ++cand) {
         int w = world < 0 ? 0 : world;
         while (w + 1 < nworlds && first[w + 1] <= cand) ++w;
@@ -128,8 +186,8 @@ __global__ __launch_bounds__(kThreads) void filter_multi_kernel(
             load_climate(st, climates + w);
             world = w;
         }
-        const unsigned int m =
-            filter_candidate(st, f, nodes, len, seeds[w], ccx[cand], ccz[cand], flags);
+        const unsigned int m = filter_candidate(st, f, nodes, f.params, len, tl, seeds[w],
+                                                ccx[cand], ccz[cand], flags, hint);
         if (threadIdx.x == 0) mask_out[cand] = (int32_t)m;
     }
 }
@@ -140,11 +198,14 @@ __global__ __launch_bounds__(kThreads) void query_kernel(const Climate* climate,
                                                          const int32_t* zs, int32_t count,
                                                          int32_t* out) {
     __shared__ Climate st;
+    __shared__ int32_t params[2 * kTreeParamCount];
     load_climate(st, climate);
+    load_params(params);
     __syncthreads();
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
         const Column col = column_at(st, xs[i], zs[i]);
-        out[i] = biome_at_column(col, ys[i], nodes, len);
+        int hint = -1;
+        out[i] = biome_at_column(col, ys[i], nodes, params, len, hint);
     }
 }
 
@@ -187,6 +248,7 @@ struct Context {
     uint64_t* d_nodes = nullptr;
     int32_t *d_x = nullptr, *d_y = nullptr, *d_z = nullptr, *d_out = nullptr;
     float last_ms = 0.f;
+    TargetLeaves targets = find_targets();
 
     ~Context() {
         if (device >= 0) (void)hipSetDevice(device);
@@ -234,7 +296,7 @@ struct Context {
             hipEventRecord(ev0, stream) != hipSuccess)
             return MCGPU_ERR_CUDA;
         hipLaunchKernelGGL(filter_kernel, dim3(grid), dim3(kThreads), 0, stream, d_climate,
-                           d_nodes, kTreeLen, world_seed, d_x, d_z, count, flags, d_out);
+                           d_nodes, kTreeLen, targets, world_seed, d_x, d_z, count, flags, d_out);
         if (hipGetLastError() != hipSuccess || hipEventRecord(ev1, stream) != hipSuccess ||
             hipMemcpyAsync(out, d_out, ni, hipMemcpyDeviceToHost, stream) != hipSuccess ||
             hipStreamSynchronize(stream) != hipSuccess ||
@@ -323,8 +385,8 @@ int filter_multi(int32_t nworlds, const uint64_t* seeds, const int32_t* first, c
               hipEventRecord(e0, stream) == hipSuccess;
     if (ok) {
         hipLaunchKernelGGL(filter_multi_kernel, dim3(grid), dim3(kThreads), 0, stream, d_clim,
-                           d_seeds, d_first, nworlds, d_nodes, kTreeLen, d_x, d_z, count,
-                           per_block, flags, d_out);
+                           d_seeds, d_first, nworlds, d_nodes, kTreeLen, find_targets(), d_x, d_z,
+                           count, per_block, flags, d_out);
         ok = hipGetLastError() == hipSuccess && hipEventRecord(e1, stream) == hipSuccess &&
              hipMemcpyAsync(out, d_out, ni, hipMemcpyDeviceToHost, stream) == hipSuccess &&
              hipStreamSynchronize(stream) == hipSuccess &&

@@ -246,3 +246,35 @@ def test_biome_ids_and_filter_on_a_small_world(bl):
         assert 0 <= m <= 3
     finally:
         bl.ob_climate_free(c)
+
+
+def test_search_from_any_leaf_equals_search_from_root(bl):
+    """Starting the search from a known leaf (bound: its distance + 1) returns
+    the root search's leaf, for random hints and for the previous layer's leaf
+    of real columns."""
+    rng = random.Random(13)
+    text = inc_text()
+    nodes = [int(h, 16) for h in re.findall(r"0x([0-9a-f]{16})ull", text.split("kTreeNodes", 1)[1].split("};", 1)[0])]
+    leaves = [i for i, n in enumerate(nodes) if n >> 56 == 0xFF]
+    for _ in range(2000):
+        np6 = [rng.randint(-12000, 12000) for _ in range(6)]
+        a = (C.c_int32 * 6)(*np6)
+        cold = bl.ob_tree_leaf(a)
+        assert bl.ob_tree_leaf_from(a, rng.choice(leaves)) == cold
+        assert bl.ob_tree_leaf_from(a, cold) == cold
+    c = bl.ob_climate_new(77, 0)
+    col = (C.c_int32 * 6)()
+    try:
+        for _ in range(300):
+            bl.ob_column(c, rng.randint(-10 ** 6, 10 ** 6), rng.randint(-10 ** 6, 10 ** 6), col)
+            off = struct.unpack("<f", struct.pack("<i", col[5]))[0]
+            hint = -1
+            for y in [16] + list(range(0, -17, -1)):
+                d = f32(1.0 - (y * 4) / 128.0 - 83.0 / 160.0 + off)
+                np6 = (C.c_int32 * 6)(col[0], col[1], col[2], col[3], int(f32(10000.0 * d)), col[4])
+                cold = bl.ob_tree_leaf(np6)
+                if hint >= 0:
+                    assert bl.ob_tree_leaf_from(np6, hint) == cold
+                hint = cold
+    finally:
+        bl.ob_climate_free(c)
