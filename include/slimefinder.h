@@ -37,6 +37,13 @@ typedef struct {
  * <= max_size; fills the best min(total, max_results) rows ranked by size
  * desc, center_x asc, center_z asc; -1 on invalid arguments or device error) */
 
+/* precise_afk != 0 refines the best rows after ranking (see the refinement
+ * section below for the count): the CPU names follow SlimeCore.cpp:571-629
+ * (the first 100 rows with the y scan enabled, else 500; the CPU rule; with
+ * the y scan every row is packed, rows past the limit as (0, platform y)),
+ * the GPU names SlimeCoreGPU.cu:3875-3912 (the first 100 rows; with the y
+ * scan the GPU rule and packed, else the CPU rule unpacked). */
+
 /* SlimeCore.cpp:643-651. `threads` is accepted and ignored (the GPU runs it). */
 int64_t search_slime_clusters_centered(int64_t seed, int64_t rd_max, int32_t min_size,
                                        int32_t max_size, int64_t rd_min, int32_t search_center_x,
@@ -84,6 +91,27 @@ int64_t get_gpu_scan_work_ns(void);
 void set_y_scan_config(int32_t enabled, int32_t platform_y, int32_t y_min, int32_t y_max,
                        int32_t y_step);
 
+/* ---- precise AFK refinement: SlimeCore.cpp:654-773, SlimeCoreGPU.cu:3958-3990 ----
+ * Both work on the caller's rows in place, on the GPU, with the heights of the
+ * current set_y_scan_config. A row's AFK points are its center plus dx, dz in
+ * -16..16 step 4 (scan_xz != 0; else the center only); obs_count is the number
+ * of spawnable block columns of slime chunks at 24 < distance <= 128 from the
+ * best point, packed as obs | (y + 1024) << 20 where noted.
+ *
+ * score_candidates_precise (SlimeCore.cpp:654-726; `threads` is accepted and
+ * ignored): the CPU library's rule - the best starts at (0, center, platform
+ * y) and only a strictly larger count replaces it; packed when the y scan is
+ * enabled. */
+void score_candidates_precise(int64_t seed, ExtChunkResult* results_buffer, int32_t count,
+                              int32_t threads, int32_t scan_xz);
+/* refine_candidates_y (SlimeCoreGPU.cu:3958-3990; the CPU library's version,
+ * SlimeCore.cpp:728-773, adds a `threads` argument, accepted and ignored):
+ * the GPU library's rule - equal counts go to the first (point, height) in
+ * order, so a row whose counts are all zero gets point (-16, -16) and the
+ * first height; always packed. */
+void refine_candidates_y(int64_t seed, ExtChunkResult* results_buffer, int32_t count,
+                         int32_t threads);
+
 /* SlimeCoreGPU.cu:3142-3161 (host-side single-chunk test). */
 bool is_slime_chunk_c(int64_t seed, int64_t cx, int64_t cz);
 
@@ -129,6 +157,11 @@ void slime_kernel_stats(int32_t kind, int64_t* launches, int64_t* kernel_ns, int
  * clock (kHz) and HBM size of the current device; 0 on success, -1 if no
  * device is usable. */
 int32_t slime_device_info(int32_t* compute_units, int32_t* clock_khz, int64_t* hbm_bytes);
+
+/* Extension with no counterpart in the reference: device time (ns, HIP
+ * events) of the last precise-AFK refinement kernel and its number of
+ * (row, AFK point, height within 128 blocks of the platform) evaluations. */
+void slime_refine_stats(int64_t* kernel_ns, int64_t* evaluations);
 
 /* ---- batched searches (extension; the reference runs one search per call)
  * n independent searches, each with the meaning of

@@ -60,6 +60,11 @@ def oracle_lib():
         lib.oracle_search_rect.argtypes = [i64, i64, i64, i64, i64, i32, i32,
                                            C.POINTER(ChunkRow), i32, C.POINTER(C.c_int64)]
         lib.oracle_search_rect.restype = i64
+        lib.oracle_spawnable.argtypes = [i64, i64, i64, i32]
+        lib.oracle_spawnable.restype = i32
+        lib.oracle_refine_rows.argtypes = [i64, C.POINTER(ChunkRow), i32, i32, i32, i32, i32,
+                                           i64, i32, i32]
+        lib.oracle_refine_rows.restype = None
         _oracle = lib
     return _oracle
 
@@ -82,6 +87,12 @@ def ref_lib(avx2=None):
                                                    C.POINTER(ChunkRow), i32, i32, i32]
     lib.search_slime_clusters_centered.restype = i64
     lib.reset_cancel.argtypes = []
+    lib.set_y_scan_config.argtypes = [i32, i32, i32, i32, i32]
+    lib.set_y_scan_config.restype = None
+    lib.score_candidates_precise.argtypes = [i64, C.POINTER(ChunkRow), i32, i32, i32]
+    lib.score_candidates_precise.restype = None
+    lib.refine_candidates_y.argtypes = [i64, C.POINTER(ChunkRow), i32, i32]
+    lib.refine_candidates_y.restype = None
     return lib
 
 
@@ -123,6 +134,76 @@ def ref_search(lib, seed, rd_max, min_size, max_size=221, rd_min=0, cx=0, cz=0,
                                                cx, cz, buf, max_results, threads, 0)
     rows = rows_to_tuples(buf, min(max(total, 0), max_results)) if total >= 0 else []
     return total, rows
+
+
+# ---- precise AFK refinement ---------------------------------------------------------
+
+RULE_KEEP_CENTER = 0   # best starts at (0, center, platform y); strictly larger replaces
+RULE_FIRST_ORDER = 1   # best starts at -1; ties go to the lowest (offset, level) order
+
+
+def pack_obs_y(obs, y):
+    """obs | (y + 1024) << 20 as a wrapped int32 (SlimeCore.cpp:33-35)."""
+    v = (obs | ((y + 1024) << 20)) & 0xFFFFFFFF
+    return v - (1 << 32) if v >= 1 << 31 else v
+
+
+def y_levels(enabled, platform_y, y_min, y_max, y_step):
+    """(y_start, y_step, y_count) of the level list (SlimeCore.cpp:148-159,
+    SlimeCoreGPU.cu:78-86); y_step as stored by set_y_scan_config (<= 0 -> 4)."""
+    y_step = y_step if y_step > 0 else 4
+    start, end, step = (y_min, y_max, y_step) if enabled else (platform_y, platform_y, 1)
+    if end < start:
+        start, end = end, start
+    count = (end - start) // step + 1
+    return start, step, max(count, 1)
+
+
+def oracle_refine(seed, rows, scan_xz, yconf, rule, pack, limit=None):
+    """Refines the first `limit` rows (all by default) of a list of 6-tuples."""
+    lib = oracle_lib()
+    n = len(rows) if limit is None else min(limit, len(rows))
+    buf = (ChunkRow * max(1, n))()
+    for i in range(n):
+        buf[i] = ChunkRow(*rows[i])
+    start, step, count = y_levels(*yconf)
+    lib.oracle_refine_rows(seed, buf, n, 1 if scan_xz else 0, yconf[1], start, step, count,
+                           rule, 1 if pack else 0)
+    return rows_to_tuples(buf, n) + list(rows[n:])
+
+
+def precise_search_rows(family, seed, rows, yconf):
+    """Rows of a search called with precise_afk != 0, from its precise_afk == 0
+    rows. family "cpu": SlimeCore.cpp:571-629 (100 rows with the y scan, else
+    500; every row packed when the y scan is on). family "gpu":
+    SlimeCoreGPU.cu:3875-3912 (100 rows; the y scan uses the block kernel's
+    rule and packs, otherwise the thread kernel's rule, unpacked)."""
+    enabled, platform_y = yconf[0], yconf[1]
+    if family == "cpu":
+        limit = 100 if enabled else 500
+        out = oracle_refine(seed, rows, True, yconf, RULE_KEEP_CENTER, enabled, limit)
+        if enabled:
+            for i in range(limit, len(out)):
+                r = out[i]
+                out[i] = r[:3] + (pack_obs_y(r[3], platform_y),) + r[4:]
+        return out
+    rule = RULE_FIRST_ORDER if enabled else RULE_KEEP_CENTER
+    return oracle_refine(seed, rows, True, yconf, rule, enabled, 100)
+
+
+def ref_refine(lib, fn, seed, rows, yconf, threads=4, scan_xz=1):
+    """The compiled reference's score_candidates_precise ("score") or
+    refine_candidates_y ("refine_y") over a list of 6-tuples."""
+    lib.set_y_scan_config(*yconf)
+    lib.reset_cancel()
+    buf = (ChunkRow * max(1, len(rows)))()
+    for i, r in enumerate(rows):
+        buf[i] = ChunkRow(*r)
+    if fn == "score":
+        lib.score_candidates_precise(seed, buf, len(rows), threads, scan_xz)
+    else:
+        lib.refine_candidates_y(seed, buf, len(rows), threads)
+    return rows_to_tuples(buf, len(rows))
 
 
 # ---- biome filter checker ----------------------------------------------------------

@@ -288,3 +288,104 @@ int64_t oracle_search_rect(int64_t seed, int64_t x0, int64_t z0, int64_t w, int6
     free(heap.v);
     return total;
 }
+
+/* ---- precise AFK refinement ------------------------------------------------
+ * Restates SlimeCore.cpp:148-180 (radius tables), :234-289 (spawnable count),
+ * :578-629 and :654-726 (offset / level loops) and the GPU selection rule of
+ * SlimeCoreGPU.cu:2799-2861. Shape chosen differently from the reference: a
+ * 25x25 chunk bitmap per row, then per x column the ring's z intervals walked
+ * chunk by chunk. */
+
+static int32_t isqrt_exact(int64_t v) {
+    int64_t r = 0;
+    while ((r + 1) * (r + 1) <= v) r++;
+    return (int32_t)r;
+}
+
+static int64_t floor16(int64_t v) { return v >= 0 ? v / 16 : -((15 - v) / 16); }
+
+typedef struct {
+    int64_t cx0, cz0;     /* chunk of bitmap cell (0, 0) */
+    uint8_t bit[25][25];  /* [x][z] */
+} ChunkWindow;
+
+static void window_fill(ChunkWindow* w, int64_t seed, int64_t bx, int64_t bz) {
+    w->cx0 = floor16(bx) - 12;
+    w->cz0 = floor16(bz) - 12;
+    for (int i = 0; i < 25; ++i)
+        for (int j = 0; j < 25; ++j)
+            w->bit[i][j] = (uint8_t)oracle_is_slime(seed, w->cx0 + i, w->cz0 + j);
+}
+
+/* Blocks z in [a, b] of chunk column cx that lie in slime chunks. */
+static int32_t column_run(const ChunkWindow* w, int64_t cx, int64_t a, int64_t b) {
+    int32_t n = 0;
+    for (int64_t cz = floor16(a); cz <= floor16(b); ++cz) {
+        if (!w->bit[cx - w->cx0][cz - w->cz0]) continue;
+        int64_t lo = a > cz * 16 ? a : cz * 16;
+        int64_t hi = b < cz * 16 + 15 ? b : cz * 16 + 15;
+        if (hi >= lo) n += (int32_t)(hi - lo + 1);
+    }
+    return n;
+}
+
+static int32_t window_spawnable(const ChunkWindow* w, int64_t ox, int64_t oz, int64_t dy) {
+    if (dy * dy > 16384) return 0;
+    int32_t total = 0;
+    for (int64_t dx = -128; dx <= 128; ++dx) {
+        int64_t outer2 = 16384 - dy * dy - dx * dx;
+        if (outer2 < 0) continue;
+        int64_t ro = isqrt_exact(outer2);
+        int64_t inner2 = 576 - dy * dy - dx * dx;
+        int64_t cx = floor16(ox + dx);
+        if (inner2 < 0) {
+            total += column_run(w, cx, oz - ro, oz + ro);
+        } else {
+            int64_t ri = isqrt_exact(inner2);
+            total += column_run(w, cx, oz - ro, oz - ri - 1);
+            total += column_run(w, cx, oz + ri + 1, oz + ro);
+        }
+    }
+    return total;
+}
+
+int32_t oracle_spawnable(int64_t seed, int64_t ox, int64_t oz, int32_t dy) {
+    ChunkWindow w;
+    window_fill(&w, seed, ox, oz);
+    return window_spawnable(&w, ox, oz, dy);
+}
+
+void oracle_refine_rows(int64_t seed, OracleResult* rows, int32_t count, int32_t scan_xz,
+                        int32_t platform_y, int32_t y_start, int32_t y_step, int64_t y_count,
+                        int32_t rule, int32_t pack) {
+    ChunkWindow w;
+    const int32_t span = scan_xz ? 16 : 0, step = scan_xz ? 4 : 1;
+    for (int32_t r = 0; r < count; ++r) {
+        const int64_t bx = rows[r].center_x, bz = rows[r].center_z;
+        window_fill(&w, seed, bx, bz);
+        int32_t best_obs = rule ? -1 : 0;
+        int64_t best_x = bx, best_z = bz, best_y = platform_y;
+        for (int32_t dx = -span; dx <= span; dx += step) {
+            for (int32_t dz = -span; dz <= span; dz += step) {
+                for (int64_t i = 0; i < y_count; ++i) {
+                    const int64_t y = (int64_t)y_start + i * y_step;
+                    const int64_t dy = (int64_t)platform_y - y;
+                    /* Levels farther than 128 blocks count 0; skipping them
+                     * keeps huge level lists cheap. */
+                    const int32_t obs =
+                        (dy > 128 || dy < -128) ? 0 : window_spawnable(&w, bx + dx, bz + dz, dy);
+                    if (obs > best_obs) {
+                        best_obs = obs;
+                        best_x = bx + dx;
+                        best_z = bz + dz;
+                        best_y = y;
+                    }
+                }
+            }
+        }
+        rows[r].obs_count =
+            pack ? (int32_t)((uint32_t)best_obs | ((uint32_t)(best_y + 1024) << 20)) : best_obs;
+        rows[r].afk_x = (int32_t)best_x;
+        rows[r].afk_z = (int32_t)best_z;
+    }
+}

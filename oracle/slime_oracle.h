@@ -6,8 +6,9 @@
  * bench.py's cpu_baseline leg use it, as the checker.
  *
  * Each function is a scalar, from-scratch restatement of the reference
- * algorithm and cites the reference file:line it follows. Parity pinning is
- * described in oracle/README.md.
+ * algorithm and cites the reference file:line it follows. The tests under
+ * tests/ pin each one against the compiled reference, known answers or a
+ * brute force.
  */
 #ifndef SLIME_ORACLE_H
 #define SLIME_ORACLE_H
@@ -73,6 +74,23 @@ int64_t oracle_search_rect(int64_t seed, int64_t x0, int64_t z0, int64_t w,
                            int64_t h, int32_t min_size, int32_t max_size,
                            OracleResult* results, int32_t max_results,
                            int64_t* hist222);
+
+/* Spawnable block columns around an AFK point (SlimeCore.cpp:234-289,
+ * count_spawnable_cached): the blocks (x, z) of slime chunks with
+ * 576 < (x-ox)^2 + (z-oz)^2 + dy^2 <= 16384, dy = platform_y - afk_y. */
+int32_t oracle_spawnable(int64_t seed, int64_t ox, int64_t oz, int32_t dy);
+
+/* Precise AFK refinement of result rows in place (SlimeCore.cpp:578-629,
+ * 654-726; SlimeCoreGPU.cu:2749-2892). Offsets: 81 points (dx, dz in
+ * -16..16 step 4, dx outer) when scan_xz, else the center only. Y levels
+ * y_start + i*y_step for i < y_count, evaluated against platform_y. rule 0:
+ * best starts at (0, center, platform_y) and only a strictly larger count
+ * replaces it (the CPU loops and refine_afk_kernel); rule 1: best starts at
+ * -1 and ties go to the lowest (offset, level) order (refine_afk_block_kernel,
+ * refine_afk_xz_kernel). pack != 0 stores obs | (y + 1024) << 20. */
+void oracle_refine_rows(int64_t seed, OracleResult* rows, int32_t count, int32_t scan_xz,
+                        int32_t platform_y, int32_t y_start, int32_t y_step, int64_t y_count,
+                        int32_t rule, int32_t pack);
 
 #ifdef __cplusplus
 }

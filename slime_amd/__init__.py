@@ -15,7 +15,8 @@ LIB_PATH = os.path.join(HERE, "libslimefinder.so")
 __all__ = [
     "ExtChunkResult", "lib", "build", "search_slime_clusters_centered",
     "search_slime_clusters", "search_slime_clusters_gpu_v1_centered", "scan_rect",
-    "EXPORTS",
+    "EXPORTS", "set_y_scan_config", "score_candidates_precise", "refine_candidates_y",
+    "refine_stats",
 ]
 
 # Every symbol include/slimefinder.h declares.
@@ -32,7 +33,8 @@ EXPORTS = (
     "slime_batch_stats", "slime_sweep_costs", "slime_session_create", "slime_session_destroy",
     "slime_session_submit", "slime_session_pause", "slime_session_resume",
     "slime_session_cancel", "slime_session_poll", "slime_session_wait", "slime_session_result",
-    "slime_session_release", "slime_session_stats",
+    "slime_session_release", "slime_session_stats", "score_candidates_precise",
+    "refine_candidates_y", "slime_refine_stats",
 )
 
 # Every symbol include/mcgpu.h declares (the biome filter).
@@ -94,6 +96,12 @@ def lib():
         getattr(L, name).argtypes = []
     L.set_y_scan_config.argtypes = [i32, i32, i32, i32, i32]
     L.set_y_scan_config.restype = None
+    L.score_candidates_precise.argtypes = [i64, P, i32, i32, i32]
+    L.score_candidates_precise.restype = None
+    L.refine_candidates_y.argtypes = [i64, P, i32, i32]
+    L.refine_candidates_y.restype = None
+    L.slime_refine_stats.argtypes = [C.POINTER(i64), C.POINTER(i64)]
+    L.slime_refine_stats.restype = None
     L.is_slime_chunk_c.argtypes = [i64, i64, i64]
     L.is_slime_chunk_c.restype = C.c_bool
     L.get_cuda_device_name.argtypes = [C.c_char_p, i32]
@@ -191,6 +199,39 @@ def search_slime_clusters_gpu_v1_centered(seed, rd_max, min_size, max_size=221, 
                                                         search_center_x, search_center_z, buf,
                                                         max_results, precise_afk)
     return total, _rows(buf, total, max_results)
+
+
+def _row_buffer(rows):
+    buf = (ExtChunkResult * max(1, len(rows)))()
+    for i, r in enumerate(rows):
+        buf[i] = ExtChunkResult(*r)
+    return buf
+
+
+def set_y_scan_config(enabled, platform_y=-64, y_min=-64, y_max=64, y_step=4):
+    """Mirror of SlimeCore.cpp:364-371: the AFK heights of the precise refinement."""
+    lib().set_y_scan_config(enabled, platform_y, y_min, y_max, y_step)
+
+
+def score_candidates_precise(seed, rows, scan_xz=1, threads=0):
+    """Mirror of SlimeCore.cpp:654-726 on a list of 6-tuples; returns the new rows."""
+    buf = _row_buffer(rows)
+    lib().score_candidates_precise(seed, buf, len(rows), threads, scan_xz)
+    return _rows(buf, len(rows), len(rows))
+
+
+def refine_candidates_y(seed, rows, threads=0):
+    """Mirror of SlimeCoreGPU.cu:3958-3990 on a list of 6-tuples; returns the new rows."""
+    buf = _row_buffer(rows)
+    lib().refine_candidates_y(seed, buf, len(rows), threads)
+    return _rows(buf, len(rows), len(rows))
+
+
+def refine_stats():
+    """(kernel_ns, evaluations) of the last precise refinement."""
+    ns, ev = C.c_int64(), C.c_int64()
+    lib().slime_refine_stats(C.byref(ns), C.byref(ev))
+    return ns.value, ev.value
 
 
 def device_info():

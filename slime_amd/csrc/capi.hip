@@ -2,8 +2,10 @@
 //
 // Argument validation, return conventions and control semantics follow
 // SlimeCore.cpp:340-651 (CPU names) and SlimeCoreGPU.cu:3034-3955 (GPU
-// names); the only difference between the two families is the rd_max limit
-// ((INT32_MAX-17)/2 vs (INT32_MAX-19)/2) and the CPU `threads` argument.
+// names); the two families differ in the rd_max limit ((INT32_MAX-17)/2 vs
+// (INT32_MAX-19)/2), the CPU `threads` argument, and the precise-AFK
+// selection rule and row limits (SlimeCore.cpp:571-629 vs
+// SlimeCoreGPU.cu:3875-3912).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -14,6 +16,7 @@
 
 #include "../../include/slimefinder.h"
 #include "engine.h"
+#include "refine.h"
 #include "slime_math.h"
 #include "sweep_engine.h"
 
@@ -74,6 +77,48 @@ int64_t search_common(int64_t seed, int64_t rd_max, int32_t min_size, int32_t ma
     return slime::run_search(spec, reinterpret_cast<slime::ResultRow*>(out), max_results);
 }
 
+slime::RefineSpec refine_spec(int64_t seed, int32_t scan_xz, int32_t rule, int32_t pack) {
+    slime::RefineSpec r;
+    r.seed = seed;
+    r.scan_xz = scan_xz;
+    r.levels = slime::y_levels(g_yscan.enabled, g_yscan.platform_y, g_yscan.y_min, g_yscan.y_max,
+                               g_yscan.y_step);
+    r.rule = rule;
+    r.pack = pack;
+    return r;
+}
+
+bool cancelled() { return control().cancel.load(std::memory_order_relaxed) != 0; }
+
+// precise_afk != 0 of the CPU names (SlimeCore.cpp:571-629): the first 100
+// rows (y scan) or 500 rows are refined with the keep-center rule; with the y
+// scan every row is packed, the unrefined ones as (0, platform_y). A device
+// error leaves the rows as ranked.
+void precise_cpu(int64_t seed, ExtChunkResult* out, int64_t total, int32_t max_results) {
+    const int32_t n = (int32_t)std::min<int64_t>(total, max_results);
+    if (n <= 0 || cancelled()) return;
+    const int32_t scan_y = g_yscan.enabled;
+    const int32_t limit = std::min(n, scan_y ? 100 : 500);
+    auto* rows = reinterpret_cast<slime::ResultRow*>(out);
+    if (slime::refine_rows(refine_spec(seed, 1, slime::kRuleKeepCenter, scan_y), rows, limit) != 0)
+        return;
+    if (scan_y)
+        for (int32_t i = limit; i < n; ++i)
+            out[i].obs_count = (int32_t)((uint32_t)(g_yscan.platform_y + 1024) << 20);
+}
+
+// precise_afk != 0 of the GPU names (SlimeCoreGPU.cu:3875-3912): the first
+// 100 rows; with the y scan the block kernel's first-in-order rule, packed;
+// else the thread kernel's keep-center rule, unpacked.
+void precise_gpu(int64_t seed, ExtChunkResult* out, int64_t total, int32_t max_results) {
+    const int32_t n = (int32_t)std::min<int64_t>(total, max_results);
+    if (n <= 0 || cancelled()) return;
+    const int32_t scan_y = g_yscan.enabled;
+    (void)slime::refine_rows(
+        refine_spec(seed, 1, scan_y ? slime::kRuleFirstOrder : slime::kRuleKeepCenter, scan_y),
+        reinterpret_cast<slime::ResultRow*>(out), std::min(n, 100));
+}
+
 constexpr int64_t kCpuRdLimit = (INT32_MAX - 17LL) / 2LL;
 constexpr int64_t kGpuRdLimit = (INT32_MAX - 19LL) / 2LL;
 
@@ -86,9 +131,10 @@ int64_t search_slime_clusters_centered(int64_t seed, int64_t rd_max, int32_t min
                                        int32_t search_center_z, ExtChunkResult* results_buffer,
                                        int32_t max_results, int32_t threads, int32_t precise_afk) {
     (void)threads;
-    (void)precise_afk;
-    return search_common(seed, rd_max, min_size, max_size, rd_min, search_center_x,
-                         search_center_z, results_buffer, max_results, kCpuRdLimit);
+    const int64_t total = search_common(seed, rd_max, min_size, max_size, rd_min, search_center_x,
+                                        search_center_z, results_buffer, max_results, kCpuRdLimit);
+    if (precise_afk != 0 && total > 0) precise_cpu(seed, results_buffer, total, max_results);
+    return total;
 }
 
 int64_t search_slime_clusters(int64_t seed, int64_t rd_max, int32_t min_size, int32_t max_size,
@@ -103,9 +149,10 @@ int64_t search_slime_clusters_gpu_v1_centered(int64_t seed, int64_t rd_max, int3
                                               int32_t search_center_x, int32_t search_center_z,
                                               ExtChunkResult* results_buffer, int32_t max_results,
                                               int32_t precise_afk) {
-    (void)precise_afk;
-    return search_common(seed, rd_max, min_size, max_size, rd_min, search_center_x,
-                         search_center_z, results_buffer, max_results, kGpuRdLimit);
+    const int64_t total = search_common(seed, rd_max, min_size, max_size, rd_min, search_center_x,
+                                        search_center_z, results_buffer, max_results, kGpuRdLimit);
+    if (precise_afk != 0 && total > 0) precise_gpu(seed, results_buffer, total, max_results);
+    return total;
 }
 
 int64_t search_slime_clusters_gpu_v1(int64_t seed, int64_t rd_max, int32_t min_size,
@@ -225,6 +272,32 @@ void set_y_scan_config(int32_t enabled, int32_t platform_y, int32_t y_min, int32
     g_yscan.y_step = y_step > 0 ? y_step : 4;
 }
 
+void score_candidates_precise(int64_t seed, ExtChunkResult* results_buffer, int32_t count,
+                              int32_t threads, int32_t scan_xz) {
+    (void)threads;
+    if (!results_buffer || count <= 0) return;
+    control().progress.store(90, std::memory_order_relaxed);
+    if (cancelled()) return;
+    (void)slime::refine_rows(
+        refine_spec(seed, scan_xz ? 1 : 0, slime::kRuleKeepCenter, g_yscan.enabled),
+        reinterpret_cast<slime::ResultRow*>(results_buffer), count);
+    control().progress.store(99, std::memory_order_relaxed);
+}
+
+void refine_candidates_y(int64_t seed, ExtChunkResult* results_buffer, int32_t count,
+                         int32_t threads) {
+    (void)threads;
+    if (!results_buffer || count <= 0) return;
+    control().progress.store(90, std::memory_order_relaxed);
+    (void)slime::refine_rows(refine_spec(seed, 1, slime::kRuleFirstOrder, 1),
+                             reinterpret_cast<slime::ResultRow*>(results_buffer), count);
+    control().progress.store(99, std::memory_order_relaxed);
+}
+
+void slime_refine_stats(int64_t* kernel_ns, int64_t* evaluations) {
+    slime::refine_last_stats(kernel_ns, evaluations);
+}
+
 bool is_slime_chunk_c(int64_t seed, int64_t cx, int64_t cz) {
     return slime::is_slime(seed, (int32_t)cx, (int32_t)cz);
 }
@@ -253,7 +326,10 @@ int32_t get_cuda_device_name(char* buffer, int32_t buffer_size) {
     return (int32_t)n;
 }
 
-void cleanup_gpu_resources(void) { slime::release_engine(); }
+void cleanup_gpu_resources(void) {
+    slime::release_refine();
+    slime::release_engine();
+}
 
 int32_t get_gpu_v1_shape(void) { return g_last_shape; }
 int32_t get_gpu_v1_algorithm(void) { return g_last_algorithm; }
