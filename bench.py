@@ -23,7 +23,10 @@ Two more legs ride in the same line:
            calls of its plain per-chunk kernel; all results must be identical.
            Their top sites (up to 1000 per search) then go through the biome
            filter (Deep Dark / Mushroom Fields under the sites' slime chunks,
-           Minecraft 26.2) in one launch, against one context per world.
+           Minecraft 26.2) in one launch, against one context per world, and
+           through the AFK-Y search (refine_candidates_y, heights -64..320
+           step 4) in one launch, against one call per world and the
+           reference's CPU refinement on a sample.
   served   Poisson arrivals of the served mix (70% radius 256..4096, 25%
            radius 50,000, 5% full world; thresholds 50..65; 10% pause then
            resume, 5% cancel) into one session, against the same arrivals
@@ -155,6 +158,7 @@ def batched_leg(repeats):
     out["speedup_vs_per_search_plain_kernel"] = best / out["per_search_plain_kernel"]["t_centers_per_s"]
     out["identical"] = per == res and plain == res
     out["biome"] = biome_leg(searches, res)
+    out["afk_y"] = afk_y_leg(searches, res)
     return out
 
 
@@ -202,6 +206,101 @@ def biome_leg(searches, res):
                      "flops_per_column": FLOPS_PER_COLUMN, "columns": columns,
                      "note": "the parameter-tree search (integer, divergent) is most of the time"},
     }
+
+
+AFK_Y = (1, -64, -64, 320, 4)   # y scan on, platform -64, heights -64..320 step 4
+REFINE_VALU_PER_COLUMN = 22     # VALU instructions per block column in refine_kernel's loops
+
+
+def refine_columns_per_point(enabled, platform_y, y_min, y_max, y_step):
+    """Block columns one AFK point walks over all heights within reach:
+    2*reach+1 of the outer disk plus 2*reach_in+1 of the inner one."""
+    import math
+    y_step = y_step if y_step > 0 else 4
+    lo, hi = (min(y_min, y_max), max(y_min, y_max)) if enabled else (platform_y, platform_y)
+    step = y_step if enabled else 1
+    cols = 0
+    for y in range(lo, hi + 1, step):
+        dy2 = (platform_y - y) ** 2
+        if dy2 <= 16384:
+            cols += 2 * math.isqrt(16384 - dy2) + 1
+            if dy2 <= 576:
+                cols += 2 * math.isqrt(576 - dy2) + 1
+    return cols
+
+
+def afk_y_leg(searches, res):
+    """The AFK-Y search over the batched top sites: refine_candidates_y
+    (81 AFK points x heights -64..320 step 4 around platform -64) of every
+    world's rows in one launch, against one call per world, and against the
+    reference's CPU refine_candidates_y on a sample of worlds."""
+    import slime_amd
+    seeds = [s[0] for s in searches]
+    rows = [list(r) for _, r in res]
+    n = sum(len(r) for r in rows)
+    slime_amd.set_y_scan_config(*AFK_Y)
+    try:
+        slime_amd.refine_multi(seeds, rows)  # warm-up
+        t0 = time.perf_counter()
+        got = slime_amd.refine_multi(seeds, rows)
+        wall = time.perf_counter() - t0
+        ns, evals = slime_amd.refine_stats()
+        t0 = time.perf_counter()
+        per, per_ns = [], 0
+        for seed, rs in zip(seeds, rows):
+            per.append(slime_amd.refine_candidates_y(seed, rs) if rs else [])
+            per_ns += slime_amd.refine_stats()[0] if rs else 0
+        per_wall = time.perf_counter() - t0
+        cpu = afk_y_cpu(seeds, rows, got)
+    finally:
+        slime_amd.set_y_scan_config(0, -64, -64, 64, 4)
+    info = slime_amd.device_info()
+    cus, clk_khz = (info[0], info[1]) if info else (256, 2400000)
+    peak = cus * 64 * clk_khz * 1e3 / 1e12
+    columns = n * 81 * refine_columns_per_point(*AFK_Y)
+    achieved = columns * REFINE_VALU_PER_COLUMN / (ns / 1e9) / 1e12
+    print(f"afk-y refine: {n} rows in {ns / 1e6:.2f} ms", file=sys.stderr, flush=True)
+    return {
+        "workload": "refine_candidates_y of the batched top sites, y scan -64..320 step 4, "
+                    "platform -64, 81 AFK points",
+        "rows": n, "evaluations": evals,
+        "one_launch": {"kernel_ms": ns / 1e6, "wall_s": wall, "rows_per_s": n / (ns / 1e9)},
+        "per_world_calls": {"kernel_ms": per_ns / 1e6, "wall_s": per_wall},
+        "identical": per == got,
+        "cpu_baseline": cpu,
+        "roofline": {"bound": "valu", "unit": "TOP/s", "peak": peak, "achieved": achieved,
+                     "frac": achieved / peak, "columns": columns,
+                     "valu_per_column": REFINE_VALU_PER_COLUMN,
+                     "note": "issue rate of the compiled loop (VALU instructions per block "
+                             "column x columns / kernel time)"},
+    }
+
+
+def afk_y_cpu(seeds, rows, got):
+    """The compiled reference's refine_candidates_y over the first worlds'
+    rows (about 2,000), on this host's cores."""
+    from oracle import pyoracle as P
+    ref = P.ref_lib()
+    if ref is None:
+        return None
+    threads = max(1, min(16, os.cpu_count() or 1))
+    sample, same, t = 0, True, 0.0
+    for w, (seed, rs) in enumerate(zip(seeds, rows)):
+        if sample >= 2000:
+            break
+        if not rs:
+            continue
+        t0 = time.perf_counter()
+        want = P.ref_refine(ref, "refine_y", seed, rs, AFK_Y, threads=threads)
+        t += time.perf_counter() - t0
+        # The CPU library keeps the center for a row whose counts are all 0
+        # (the GPU library takes point 0): such rows never rank here.
+        same = same and want == got[w]
+        sample += len(rs)
+    ref.set_y_scan_config(0, -64, -64, 64, 4)
+    return {"value": sample / t, "unit": "rows/s", "cores": threads, "kind": "reference",
+            "sample": f"{sample} rows of the first worlds, same heights", "seconds": t,
+            "gpu_matches_sample": same}
 
 
 def percentile(xs, q):

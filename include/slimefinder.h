@@ -112,6 +112,14 @@ void score_candidates_precise(int64_t seed, ExtChunkResult* results_buffer, int3
 void refine_candidates_y(int64_t seed, ExtChunkResult* results_buffer, int32_t count,
                          int32_t threads);
 
+/* Extension with no counterpart in the reference: refine_candidates_y over
+ * the rows of many worlds in one launch. World w has seed world_seeds[w] and
+ * rows [first[w], first[w+1]) of results_buffer (first[0] == 0); each row
+ * gets exactly what refine_candidates_y gives it alone. 0, or -1 on invalid
+ * arguments or a device error (rows unchanged). */
+int32_t refine_candidates_y_multi(int32_t nworlds, const int64_t* world_seeds, const int32_t* first,
+                                  ExtChunkResult* results_buffer);
+
 /* SlimeCoreGPU.cu:3142-3161 (host-side single-chunk test). */
 bool is_slime_chunk_c(int64_t seed, int64_t cx, int64_t cz);
 

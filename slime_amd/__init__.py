@@ -16,7 +16,7 @@ __all__ = [
     "ExtChunkResult", "lib", "build", "search_slime_clusters_centered",
     "search_slime_clusters", "search_slime_clusters_gpu_v1_centered", "scan_rect",
     "EXPORTS", "set_y_scan_config", "score_candidates_precise", "refine_candidates_y",
-    "refine_stats",
+    "refine_stats", "refine_multi",
 ]
 
 # Every symbol include/slimefinder.h declares.
@@ -34,7 +34,7 @@ EXPORTS = (
     "slime_session_submit", "slime_session_pause", "slime_session_resume",
     "slime_session_cancel", "slime_session_poll", "slime_session_wait", "slime_session_result",
     "slime_session_release", "slime_session_stats", "score_candidates_precise",
-    "refine_candidates_y", "slime_refine_stats",
+    "refine_candidates_y", "slime_refine_stats", "refine_candidates_y_multi",
 )
 
 # Every symbol include/mcgpu.h declares (the biome filter).
@@ -100,6 +100,8 @@ def lib():
     L.score_candidates_precise.restype = None
     L.refine_candidates_y.argtypes = [i64, P, i32, i32]
     L.refine_candidates_y.restype = None
+    L.refine_candidates_y_multi.argtypes = [i32, C.POINTER(i64), C.POINTER(i32), P]
+    L.refine_candidates_y_multi.restype = i32
     L.slime_refine_stats.argtypes = [C.POINTER(i64), C.POINTER(i64)]
     L.slime_refine_stats.restype = None
     L.is_slime_chunk_c.argtypes = [i64, i64, i64]
@@ -225,6 +227,22 @@ def refine_candidates_y(seed, rows, threads=0):
     buf = _row_buffer(rows)
     lib().refine_candidates_y(seed, buf, len(rows), threads)
     return _rows(buf, len(rows), len(rows))
+
+
+def refine_multi(seeds, rows_per_world):
+    """refine_candidates_y of each world's rows, all worlds in one launch."""
+    first = [0]
+    for rows in rows_per_world:
+        first.append(first[-1] + len(rows))
+    flat = [r for rows in rows_per_world for r in rows]
+    buf = _row_buffer(flat)
+    n = len(seeds)
+    rc = lib().refine_candidates_y_multi(n, (C.c_int64 * max(1, n))(*seeds),
+                                         (C.c_int32 * (n + 1))(*first), buf)
+    if rc != 0:
+        raise RuntimeError("refine_candidates_y_multi failed")
+    out = _rows(buf, len(flat), len(flat))
+    return [out[first[w]:first[w + 1]] for w in range(n)]
 
 
 def refine_stats():

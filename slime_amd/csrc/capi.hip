@@ -294,6 +294,23 @@ void refine_candidates_y(int64_t seed, ExtChunkResult* results_buffer, int32_t c
     control().progress.store(99, std::memory_order_relaxed);
 }
 
+int32_t refine_candidates_y_multi(int32_t nworlds, const int64_t* world_seeds, const int32_t* first,
+                                  ExtChunkResult* results_buffer) {
+    if (nworlds < 0 || (nworlds > 0 && (!world_seeds || !first || !results_buffer)) ||
+        (nworlds > 0 && first[0] != 0))
+        return -1;
+    if (nworlds == 0) return 0;
+    for (int32_t w = 0; w < nworlds; ++w)
+        if (first[w + 1] < first[w]) return -1;
+    const int32_t count = first[nworlds];
+    if (count == 0) return 0;
+    std::vector<int64_t> seeds((size_t)count);
+    for (int32_t w = 0; w < nworlds; ++w)
+        std::fill(seeds.begin() + first[w], seeds.begin() + first[w + 1], world_seeds[w]);
+    return slime::refine_rows_multi(refine_spec(0, 1, slime::kRuleFirstOrder, 1), seeds.data(),
+                                    reinterpret_cast<slime::ResultRow*>(results_buffer), count);
+}
+
 void slime_refine_stats(int64_t* kernel_ns, int64_t* evaluations) {
     slime::refine_last_stats(kernel_ns, evaluations);
 }

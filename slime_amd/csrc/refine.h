@@ -53,6 +53,11 @@ struct RefineSpec {
 // device error (rows unchanged).
 int refine_rows(const RefineSpec& spec, ResultRow* rows, int32_t count);
 
+// The same with a world seed per row (spec.seed unused): rows of many worlds
+// in one launch.
+int refine_rows_multi(const RefineSpec& spec, const int64_t* row_seeds, ResultRow* rows,
+                      int32_t count);
+
 // Device time (ns, HIP events) of the last refinement kernel, and the number
 // of (row, point, level) evaluations it made.
 void refine_last_stats(int64_t* kernel_ns, int64_t* evaluations);

@@ -8,8 +8,9 @@
 // the ring's z intervals [oz - ro, oz - ri - 1] and [oz + ri + 1, oz + ro]
 // (ro, ri the outer and inner half-widths). A slime-block prefix count F(z)
 // of a chunk column needs one popcount of its mask below z's chunk, so an
-// interval costs two F's; the ring's four offsets per (dy, dx) come from an
-// LDS table built per group of heights. This is the set the reference counts
+// interval costs two F's; the ring's half-widths per (dy, dx) come from an
+// LDS table built per group of heights. The outer disk and the inner disk
+// (|dx| <= 24 only) are separate loops. This is the set the reference counts
 // per slime chunk and column (SlimeCore.cpp:234-331); no weight tables and
 // no per-point chunk caches are needed.
 //
@@ -38,6 +39,7 @@ constexpr int kLevelBits = 9;        // active heights <= 257 < 512
 
 struct RefineArgs {
     ResultRow* rows;
+    const int64_t* row_seeds;  // per-row world seed, or null for `seed`
     int64_t seed;
     int32_t noff;        // kOffsets or 1
     int32_t platform_y;
@@ -58,33 +60,27 @@ __device__ __forceinline__ int32_t isqrt_small(int32_t v) {
     return r;
 }
 
-// Ring offsets of column dx at height difference squared dy2, as four
-// unsigned bytes (u0, u1, u2, u3): count = F(oz + u0) - F(oz - u1)
-// - F(oz + u2) + F(oz - u3). Outer interval [-ro, ro] -> (ro + 1, ro), with
-// ro + 1 up to 129; inner [-ri, ri] -> (ri + 1, ri); an absent interval ->
-// (0, 0), whose two terms cancel.
+// Ring half-widths of column dx at height difference squared dy2: byte 0 =
+// ro = floor(sqrt(16384 - dy2 - dx^2)), byte 1 = ri = floor(sqrt(576 - dy2
+// - dx^2)) where that radicand is >= 0. Columns outside the outer disk are
+// never read (the loops stop at the disk's reach).
 __device__ __forceinline__ uint32_t ring_entry(int32_t dy2, int32_t dx) {
     const int32_t outer2 = 16384 - dy2 - dx * dx;
     if (outer2 < 0) return 0;
-    const uint32_t ro = (uint32_t)isqrt_small(outer2);
     const int32_t inner2 = 576 - dy2 - dx * dx;
-    uint32_t inner = 0;
-    if (inner2 >= 0) {
-        const uint32_t ri = (uint32_t)isqrt_small(inner2);
-        inner = ((ri + 1) << 16) | (ri << 24);
-    }
-    return (ro + 1) | (ro << 8) | inner;
+    const uint32_t ri = inner2 >= 0 ? (uint32_t)isqrt_small(inner2) : 0u;
+    return (uint32_t)isqrt_small(outer2) | (ri << 8);
 }
 
-__device__ __forceinline__ int32_t ubyte(uint32_t e, int shift) {
-    return (int32_t)((e >> shift) & 0xFFu);
-}
-
-// Slime blocks of a chunk column with window z < zr (mask bit j = chunk j).
-__device__ __forceinline__ int32_t prefix(uint32_t mask, int32_t zr) {
-    const uint32_t j = (uint32_t)zr >> 4;
-    const int32_t below = __popc(mask & ((1u << j) - 1u)) << 4;
-    return below + (int32_t)((mask >> j) & 1u) * (zr & 15);
+// Slime blocks of one block column in window z range [lo, hi] (mask bit j =
+// chunk j of the column): F(hi + 1) - F(lo) with F(z) = 16 * (slime chunks
+// below z's chunk) + (z's chunk is slime) * (z mod 16).
+__device__ __forceinline__ int32_t column_count(uint32_t mask, uint32_t lo, uint32_t hi1) {
+    const uint32_t jl = lo >> 4, jh = hi1 >> 4;
+    const uint32_t below = __popc(__builtin_amdgcn_ubfe(mask, 0, jh)) -
+                           __popc(__builtin_amdgcn_ubfe(mask, 0, jl));
+    return (int32_t)((below << 4) + __umul24(__builtin_amdgcn_ubfe(mask, jh, 1), hi1 & 15u) -
+                     __umul24(__builtin_amdgcn_ubfe(mask, jl, 1), lo & 15u));
 }
 
 __device__ __forceinline__ bool better(int32_t obs, int32_t key, int32_t bobs, int32_t bkey) {
@@ -99,12 +95,14 @@ __device__ __forceinline__ void offset_of(int32_t noff, int32_t k, int32_t& dx, 
 __global__ __launch_bounds__(kRefineThreads) void refine_kernel(RefineArgs a) {
     __shared__ uint32_t win[kWin];
     __shared__ uint32_t ring[kMaxGroup * kSpan];
-    __shared__ int32_t reach[kMaxGroup];
+    __shared__ int32_t reach[kMaxGroup];     // outer disk: |dx| <= reach
+    __shared__ int32_t reach_in[kMaxGroup];  // inner disk: |dx| <= reach_in (-1: none)
     __shared__ int32_t red_obs[kRefineThreads / 64];
     __shared__ int32_t red_key[kRefineThreads / 64];
 
     const int tid = threadIdx.x;
     ResultRow& row = a.rows[blockIdx.x];
+    const int64_t seed = a.row_seeds ? a.row_seeds[blockIdx.x] : a.seed;
     const int32_t bx = row.center_x, bz = row.center_z;
     const int32_t ubx = (bx >> 4) - 11, ubz = (bz >> 4) - 11;
 
@@ -112,7 +110,7 @@ __global__ __launch_bounds__(kRefineThreads) void refine_kernel(RefineArgs a) {
     __syncthreads();
     for (int c = tid; c < kWin * kWin; c += kRefineThreads) {
         const int i = c / kWin, j = c - i * kWin;
-        if (is_slime(a.seed, ubx + i, ubz + j)) atomicOr(&win[i], 1u << j);
+        if (is_slime(seed, ubx + i, ubz + j)) atomicOr(&win[i], 1u << j);
     }
 
     // Center relative to the window's first block (160..207 after the offset).
@@ -125,7 +123,10 @@ __global__ __launch_bounds__(kRefineThreads) void refine_kernel(RefineArgs a) {
             const int l = e / kSpan, dx = e - l * kSpan - 128;
             const int32_t dy = a.platform_y - (a.y_first + (g0 + l) * a.y_step);
             ring[e] = ring_entry(dy * dy, dx);
-            if (dx == 0) reach[l] = isqrt_small(16384 - dy * dy);
+            if (dx == 0) {
+                reach[l] = isqrt_small(16384 - dy * dy);
+                reach_in[l] = dy * dy <= 576 ? isqrt_small(576 - dy * dy) : -1;
+            }
         }
         __syncthreads();
         const int items = gn * a.noff;
@@ -135,13 +136,17 @@ __global__ __launch_bounds__(kRefineThreads) void refine_kernel(RefineArgs a) {
             offset_of(a.noff, k, dxo, dzo);
             const int32_t x0 = xb + dxo, z0 = zb + dzo;
             const uint32_t* rt = ring + l * kSpan + 128;
-            const int32_t r = reach[l];
+            // Outer disk z in [z0 - ro, z0 + ro], then minus the inner disk.
             int32_t obs = 0;
-            for (int dx = -r; dx <= r; ++dx) {
-                const uint32_t e = rt[dx];
-                const uint32_t m = win[(uint32_t)(x0 + dx) >> 4];
-                obs += prefix(m, z0 + ubyte(e, 0)) - prefix(m, z0 - ubyte(e, 8)) -
-                       prefix(m, z0 + ubyte(e, 16)) + prefix(m, z0 - ubyte(e, 24));
+            for (int dx = -reach[l]; dx <= reach[l]; ++dx) {
+                const uint32_t ro = rt[dx] & 0xFFu;
+                obs += column_count(win[(uint32_t)(x0 + dx) >> 4], (uint32_t)z0 - ro,
+                                    (uint32_t)z0 + ro + 1u);
+            }
+            for (int dx = -reach_in[l]; dx <= reach_in[l]; ++dx) {
+                const uint32_t ri = (rt[dx] >> 8) & 0xFFu;
+                obs -= column_count(win[(uint32_t)(x0 + dx) >> 4], (uint32_t)z0 - ri,
+                                    (uint32_t)z0 + ri + 1u);
             }
             const int32_t key = (k << kLevelBits) | (g0 + l);
             if (better(obs, key, bobs, bkey)) {
@@ -190,6 +195,7 @@ struct RefineState {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ResultRow* d_rows = nullptr;
+    int64_t* d_seeds = nullptr;
     size_t cap = 0;
     int64_t last_ns = 0, last_evals = 0;
 };
@@ -219,6 +225,11 @@ YLevels y_levels(int32_t enabled, int32_t platform_y, int32_t y_min, int32_t y_m
 }
 
 int refine_rows(const RefineSpec& spec, ResultRow* rows, int32_t count) {
+    return refine_rows_multi(spec, nullptr, rows, count);
+}
+
+int refine_rows_multi(const RefineSpec& spec, const int64_t* row_seeds, ResultRow* rows,
+                      int32_t count) {
     if (count <= 0) return 0;
     if (!rows) return -1;
     const YLevels& L = spec.levels;
@@ -252,18 +263,24 @@ int refine_rows(const RefineSpec& spec, ResultRow* rows, int32_t count) {
     }
     if ((size_t)count > s.cap) {
         if (s.d_rows) (void)hipFree(s.d_rows);
+        if (s.d_seeds) (void)hipFree(s.d_seeds);
         s.d_rows = nullptr;
+        s.d_seeds = nullptr;
         s.cap = 0;
         const size_t cap = std::max<size_t>((size_t)count, 1024);
-        if (hipMalloc(&s.d_rows, cap * sizeof(ResultRow)) != hipSuccess) {
+        if (hipMalloc(&s.d_rows, cap * sizeof(ResultRow)) != hipSuccess ||
+            hipMalloc(&s.d_seeds, cap * sizeof(int64_t)) != hipSuccess) {
             (void)hipGetLastError();
             return -1;
         }
         s.cap = cap;
     }
     a.rows = s.d_rows;
+    a.row_seeds = row_seeds ? s.d_seeds : nullptr;
     const size_t bytes = (size_t)count * sizeof(ResultRow);
-    if (hipMemcpyAsync(s.d_rows, rows, bytes, hipMemcpyHostToDevice, s.stream) != hipSuccess ||
+    if ((row_seeds && hipMemcpyAsync(s.d_seeds, row_seeds, (size_t)count * sizeof(int64_t),
+                                     hipMemcpyHostToDevice, s.stream) != hipSuccess) ||
+        hipMemcpyAsync(s.d_rows, rows, bytes, hipMemcpyHostToDevice, s.stream) != hipSuccess ||
         hipEventRecord(s.ev0, s.stream) != hipSuccess) {
         (void)hipGetLastError();
         return -1;
@@ -298,10 +315,12 @@ void release_refine() {
     RefineState& s = state();
     std::lock_guard<std::mutex> lock(s.mu);
     if (s.d_rows) (void)hipFree(s.d_rows);
+    if (s.d_seeds) (void)hipFree(s.d_seeds);
     if (s.ev0) (void)hipEventDestroy(s.ev0);
     if (s.ev1) (void)hipEventDestroy(s.ev1);
     if (s.stream) (void)hipStreamDestroy(s.stream);
     s.d_rows = nullptr;
+    s.d_seeds = nullptr;
     s.cap = 0;
     s.ev0 = s.ev1 = nullptr;
     s.stream = nullptr;

@@ -131,3 +131,23 @@ def test_refine_many_rows(oracle, yscan):
     assert got == oracle.oracle_refine(SEED, rows, 1, yconf, oracle.RULE_FIRST_ORDER, True)
     ns, evals = slime_amd.refine_stats()
     assert evals == len(rows) * 81 * 33 and ns > 0
+
+
+def test_refine_multi_equals_per_world_calls(oracle, yscan):
+    """Rows of several worlds in one launch, each as refine_candidates_y alone;
+    empty worlds included."""
+    import slime_amd
+    rng = random.Random(12)
+    seeds, per_world = [], []
+    for w in range(6):
+        seed = rng.getrandbits(64) - (1 << 63)
+        cx, cz = rng.randint(-10**6, 10**6), rng.randint(-10**6, 10**6)
+        _, rows = oracle.oracle_search(seed, 400, 38, 221, 0, cx, cz, max_results=20)
+        seeds.append(seed)
+        per_world.append(rows if w != 2 else [])
+    yconf = (1, -64, -64, 320, 4)
+    yscan(*yconf)
+    got = slime_amd.refine_multi(seeds, per_world)
+    for seed, rows, g in zip(seeds, per_world, got):
+        assert g == (slime_amd.refine_candidates_y(seed, rows) if rows else [])
+        assert g == oracle.oracle_refine(seed, rows, 1, yconf, oracle.RULE_FIRST_ORDER, True)
