@@ -7,7 +7,8 @@
  *   xoroshiro128++ and its seeding        :15-70
  *   improved Perlin noise                 :72-190
  *   octave sets, double Perlin, climate   :192-422
- *   peaks and valleys, tree search        :657-805
+ *   peaks and valleys, tree searches      :657-809
+ *   version -> tree                       :1785-1820
  *   depth and biome at a quart            :868-913
  *   the candidate filter                  :945-1074
  * and evaluates the terrain spline (spline_generated.cuh) from the point data
@@ -40,11 +41,38 @@ typedef struct {
     uint32_t loc, der, value;
     int32_t child;
 } BiomeSplinePoint;
+typedef struct {
+    int32_t version, family;
+    const uint64_t* nodes;
+    int32_t len;
+    const int32_t* params;
+    int32_t nparams;
+} BiomeTreeDef;
 #define BIOME_TABLE static const
 #include "../slime_amd/csrc/biome_data.inc"
 #undef BIOME_TABLE
 
-#define TREE_LEN ((int)(sizeof(kTreeNodes) / sizeof(kTreeNodes[0])))
+/* A parameter tree: nodes, intervals and node layout (family). */
+typedef struct {
+    const uint64_t* nodes;
+    const int32_t (*params)[2];
+    int len, family;
+} ob_tree;
+
+#define OB_TREE(sfx, fam) \
+    { kTreeNodes##sfx, kTreeParams##sfx, (int)(sizeof(kTreeNodes##sfx) / sizeof(uint64_t)), fam }
+/* The version selection of MinecraftGPUNoise.cu:1785-1820, trees by name. */
+static const ob_tree kObTrees[7] = {OB_TREE(262, 1), OB_TREE(215, 1), OB_TREE(21wd, 1),
+                                    OB_TREE(20, 1),  OB_TREE(19, 1),  OB_TREE(192, 0),
+                                    OB_TREE(18, 0)};
+static const int kObTreeFirstVersion[7] = {2602, 1215, 1214, 1206, 1194, 1192, 1180};
+
+int ob_tree_for_version(int version) {
+    if (version < 1180) return -1;
+    for (int i = 0; i < 7; ++i)
+        if (version >= kObTreeFirstVersion[i]) return i;
+    return 6;
+}
 
 /* ---- xoroshiro128++ (MinecraftGPUNoise.cu:15-70) ---- */
 
@@ -285,13 +313,14 @@ static float spline_value(int id, const float* v) {
 
 float ob_spline(const float* v4) { return spline_value(0, v4); }
 
-/* ---- parameter tree (:657-805) ---- */
+/* ---- parameter trees (:662-809) ---- */
 
-uint64_t ob_node_distance(const int32_t* np, int idx) {
-    uint64_t node = kTreeNodes[idx], ds = 0;
+uint64_t ob_node_distance(const int32_t* np, int idx, int tree) {
+    const ob_tree* t = &kObTrees[tree];
+    uint64_t node = t->nodes[idx], ds = 0;
     for (int i = 0; i < 6; ++i) {
         int pi = (int)((node >> (8 * i)) & 0xFF);
-        int64_t lo = kTreeParams[pi][0], hi = kTreeParams[pi][1], v = np[i], d = 0;
+        int64_t lo = t->params[pi][0], hi = t->params[pi][1], v = np[i], d = 0;
         if (v > hi) d = v - hi;
         else if (v < lo) d = lo - v;
         ds += (uint64_t)(d * d);
@@ -299,18 +328,25 @@ uint64_t ob_node_distance(const int32_t* np, int idx) {
     return ds;
 }
 
-static int nearest(const int32_t* np, int depth, int idx, int best, uint64_t* best_ds) {
-    if (depth >= 5) return idx;
-    int step = kTreeSteps[depth];
-    if (idx + step >= TREE_LEN) return nearest(np, depth + 1, idx, best, best_ds);
-    int child = (int)((kTreeNodes[idx] >> 48) & 0xFFFF);
+/* Family 1: up to 6 children, steps 1555, 259, 43, 7, 1 (btree20_search);
+ * family 0: up to 10 children, steps 1111, 111, 11, 1 (btree18_search). */
+static const int kSteps1[5] = {1555, 259, 43, 7, 1};
+static const int kSteps0[4] = {1111, 111, 11, 1};
+
+static int nearest(const int32_t* np, int tree, int depth, int idx, int best, uint64_t* best_ds) {
+    const ob_tree* t = &kObTrees[tree];
+    const int depths = t->family ? 5 : 4, order = t->family ? 6 : 10;
+    if (depth >= depths) return idx;
+    int step = t->family ? kSteps1[depth] : kSteps0[depth];
+    if (idx + step >= t->len) return nearest(np, tree, depth + 1, idx, best, best_ds);
+    int child = (int)((t->nodes[idx] >> 48) & 0xFFFF);
     uint64_t ds = *best_ds;
-    for (int k = 0; k < 6 && child < TREE_LEN; ++k, child += step) {
-        uint64_t dc = ob_node_distance(np, child);
+    for (int k = 0; k < order && child < t->len; ++k, child += step) {
+        uint64_t dc = ob_node_distance(np, child, tree);
         if (dc < ds) {
             uint64_t sub = ds;
-            int leaf = nearest(np, depth + 1, child, best, &sub);
-            uint64_t dl = leaf == child ? dc : ob_node_distance(np, leaf);
+            int leaf = nearest(np, tree, depth + 1, child, best, &sub);
+            uint64_t dl = leaf == child ? dc : ob_node_distance(np, leaf, tree);
             if (dl < ds) {
                 ds = dl;
                 best = leaf;
@@ -323,24 +359,25 @@ static int nearest(const int32_t* np, int depth, int idx, int best, uint64_t* be
 
 /* The pruned search's leaf, and an exhaustive one: the first leaf (in table
  * order) at the least distance. */
-int ob_tree_leaf(const int32_t* np) {
+int ob_tree_leaf(const int32_t* np, int tree) {
     uint64_t ds = ~0ull;
-    return nearest(np, 0, 0, 0, &ds);
+    return nearest(np, tree, 0, 0, 0, &ds);
 }
 
 /* The search started from a known leaf with the bound distance(hint) + 1
  * (what the kernels do from one y layer to the next). */
-int ob_tree_leaf_from(const int32_t* np, int hint) {
-    uint64_t ds = ob_node_distance(np, hint) + 1;
-    return nearest(np, 0, 0, hint, &ds);
+int ob_tree_leaf_from(const int32_t* np, int hint, int tree) {
+    uint64_t ds = ob_node_distance(np, hint, tree) + 1;
+    return nearest(np, tree, 0, 0, hint, &ds);
 }
 
-int ob_tree_leaf_exhaustive(const int32_t* np) {
+int ob_tree_leaf_exhaustive(const int32_t* np, int tree) {
+    const ob_tree* t = &kObTrees[tree];
     int best = -1;
     uint64_t bd = ~0ull;
-    for (int i = 0; i < TREE_LEN; ++i) {
-        if ((kTreeNodes[i] >> 56) != 0xFF) continue;
-        uint64_t d = ob_node_distance(np, i);
+    for (int i = 0; i < t->len; ++i) {
+        if ((t->nodes[i] >> 56) != 0xFF) continue;
+        uint64_t d = ob_node_distance(np, i, tree);
         if (d < bd) {
             bd = d;
             best = i;
@@ -349,8 +386,15 @@ int ob_tree_leaf_exhaustive(const int32_t* np) {
     return best;
 }
 
-int ob_leaf_biome(int leaf) { return (int)((kTreeNodes[leaf] >> 48) & 0xFF); }
-int ob_tree_len(void) { return TREE_LEN; }
+int ob_leaf_biome(int leaf, int tree) { return (int)((kObTrees[tree].nodes[leaf] >> 48) & 0xFF); }
+int ob_tree_len(int tree) { return kObTrees[tree].len; }
+int ob_tree_family(int tree) { return kObTrees[tree].family; }
+/* Interval i of a tree (lo, hi) and node i, for the table checks. */
+void ob_tree_param(int tree, int i, int32_t* lo_hi) {
+    lo_hi[0] = kObTrees[tree].params[i][0];
+    lo_hi[1] = kObTrees[tree].params[i][1];
+}
+uint64_t ob_tree_node(int tree, int i) { return kObTrees[tree].nodes[i]; }
 
 /* ---- climate of a quart column, biome at a quart (:868-913) ---- */
 
@@ -376,18 +420,51 @@ void ob_column(const ob_climate* c, int32_t xi, int32_t zi, int32_t* out6) {
     memcpy(&out6[5], &off, 4);
 }
 
-static int biome_from_column(const int32_t* col, int32_t yi) {
+static int32_t depth_of(const int32_t* col, int32_t yi) {
     float off;
     memcpy(&off, &col[5], 4);
     double d0 = 1.0 - ((double)(yi * 4)) / 128.0 - 83.0 / 160.0 + (double)off;
-    int32_t np[6] = {col[0], col[1], col[2], col[3], (int32_t)(10000.0F * (float)d0), col[4]};
-    return ob_leaf_biome(ob_tree_leaf(np));
+    return (int32_t)(10000.0F * (float)d0);
 }
 
-int ob_biome_at(const ob_climate* c, int32_t x, int32_t y, int32_t z) {
+static int biome_from_column(const int32_t* col, int32_t yi, int tree) {
+    int32_t np[6] = {col[0], col[1], col[2], col[3], depth_of(col, yi), col[4]};
+    return ob_leaf_biome(ob_tree_leaf(np, tree), tree);
+}
+
+int ob_biome_at(const ob_climate* c, int32_t x, int32_t y, int32_t z, int tree) {
     int32_t col[6];
     ob_column(c, x, z, col);
-    return biome_from_column(col, y);
+    return biome_from_column(col, y, tree);
+}
+
+/* The point-query layouts (:532-655, :811-866). out5: t, h, c, e, w as
+ * doubles at the shifted point of (x, z); out6: the quantized t, h, c, e,
+ * depth at quart y, w. */
+void ob_climate5(const ob_climate* c, double x, double z, double* out5) {
+    double px = x + noise(c, 0, x, 0.0, z) * 4.0;
+    double pz = z + noise(c, 0, z, x, 0.0) * 4.0;
+    out5[0] = noise(c, 1, px, 0.0, pz);
+    out5[1] = noise(c, 2, px, 0.0, pz);
+    out5[2] = noise(c, 3, px, 0.0, pz);
+    out5[3] = noise(c, 4, px, 0.0, pz);
+    out5[4] = noise(c, 5, px, 0.0, pz);
+}
+
+void ob_climate6(const ob_climate* c, int32_t x, int32_t y, int32_t z, int32_t* out6) {
+    int32_t col[6];
+    ob_column(c, x, z, col);
+    out6[0] = col[0];
+    out6[1] = col[1];
+    out6[2] = col[2];
+    out6[3] = col[3];
+    out6[4] = depth_of(col, y);
+    out6[5] = col[4];
+}
+
+/* The continentalness noise alone at (x, 0, z), unshifted (:491-517). */
+double ob_continentalness(const ob_climate* c, double x, double z) {
+    return noise(c, 3, x, 0.0, z);
 }
 
 /* ---- the candidate filter (:945-1074) ---- */
@@ -406,7 +483,8 @@ static int slime(int64_t seed, int32_t x, int32_t z) {
     }
 }
 
-int ob_filter(const ob_climate* c, int64_t world_seed, int32_t cx, int32_t cz, int flags) {
+int ob_filter(const ob_climate* c, int64_t world_seed, int32_t cx, int32_t cz, int flags,
+              int tree) {
     int mask = 0;
     for (int ci = 0; ci < 17 * 17; ++ci) {
         int dx = ci / 17 - 8, dz = ci % 17 - 8;
@@ -418,10 +496,10 @@ int ob_filter(const ob_climate* c, int64_t world_seed, int32_t cx, int32_t cz, i
             int32_t col[6];
             ob_column(c, (int32_t)((uint32_t)x * 4u + (uint32_t)(q / 4)),
                       (int32_t)((uint32_t)z * 4u + (uint32_t)(q % 4)), col);
-            if ((flags & 2) && biome_from_column(col, 16) == 14) mask |= 2;
+            if ((flags & 2) && biome_from_column(col, 16, tree) == 14) mask |= 2;
             if (flags & 1)
                 for (int y = -16; y <= 0; ++y)
-                    if (biome_from_column(col, y) == 183) {
+                    if (biome_from_column(col, y, tree) == 183) {
                         mask |= 1;
                         break;
                     }

@@ -43,6 +43,13 @@ BIOME_EXPORTS = (
     "mcgpu_modern_ctx_create_ex", "mcgpu_modern_ctx_destroy", "mcgpu_modern_ctx_filter_candidates",
     "mcgpu_modern_ctx_query", "mcgpu_modern_recommended_filter_batch", "mcgpu_modern_ctx_columns",
     "mcgpu_modern_ctx_last_filter_ms", "mcgpu_filter_multi", "mcgpu_filter_multi_last_ms",
+    "mcgpu_xoroshiro_nextlong_batch", "mcgpu_xperlin_sample_batch",
+    "mcgpu_xperlin_sample_fixed_seed", "mcgpu_continentalness_noise_fixed_seed",
+    "mcgpu_climate5_fixed_seed", "mcgpu_climate5_quant_fixed_seed",
+    "mcgpu_climate5_i32_fixed_seed", "mcgpu_climate_ctx_create", "mcgpu_climate_ctx_query_i32",
+    "mcgpu_climate_ctx_destroy", "mcgpu_climate6_i32_fixed_seed",
+    "mcgpu_climate6_i32_fixed_seed_ex", "mcgpu_biome20_i32_fixed_seed", "mcgpu_biome20_ctx_create",
+    "mcgpu_biome20_ctx_query", "mcgpu_biome20_ctx_destroy",
 )
 
 
@@ -161,9 +168,37 @@ def lib():
     L.mcgpu_modern_recommended_filter_batch.restype = i32
     L.mcgpu_modern_ctx_last_filter_ms.argtypes = [VP]
     L.mcgpu_modern_ctx_last_filter_ms.restype = C.c_float
-    L.mcgpu_filter_multi.argtypes = [i32, C.POINTER(C.c_uint64), PI32, PI32, PI32, i32, i32, PI32]
+    U64, PU64, PD = C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_double)
+    L.mcgpu_filter_multi.argtypes = [i32, i32, PU64, PI32, PI32, PI32, i32, i32, PI32]
     L.mcgpu_filter_multi.restype = C.c_int
     L.mcgpu_filter_multi_last_ms.restype = C.c_float
+    L.mcgpu_xoroshiro_nextlong_batch.argtypes = [PU64, i32, i32, PU64]
+    L.mcgpu_xperlin_sample_batch.argtypes = [PU64, PD, PD, PD, i32, PD]
+    L.mcgpu_xperlin_sample_fixed_seed.argtypes = [U64, PD, PD, PD, i32, PD]
+    L.mcgpu_continentalness_noise_fixed_seed.argtypes = [U64, PD, PD, i32, PD]
+    L.mcgpu_climate5_fixed_seed.argtypes = [U64, PD, PD, i32, PD]
+    L.mcgpu_climate5_quant_fixed_seed.argtypes = [U64, PD, PD, i32, PI32]
+    L.mcgpu_climate5_i32_fixed_seed.argtypes = [U64, PI32, PI32, i32, PI32]
+    L.mcgpu_climate_ctx_create.argtypes = [U64, i32]
+    L.mcgpu_climate_ctx_create.restype = VP
+    L.mcgpu_climate_ctx_query_i32.argtypes = [VP, PI32, PI32, i32, PI32]
+    L.mcgpu_climate_ctx_destroy.argtypes = [VP]
+    L.mcgpu_climate_ctx_destroy.restype = None
+    L.mcgpu_climate6_i32_fixed_seed.argtypes = [U64, PI32, PI32, PI32, i32, PI32]
+    L.mcgpu_climate6_i32_fixed_seed_ex.argtypes = [U64, i32, PI32, PI32, PI32, i32, PI32]
+    L.mcgpu_biome20_i32_fixed_seed.argtypes = [U64, PI32, PI32, PI32, i32, PI32]
+    L.mcgpu_biome20_ctx_create.argtypes = [U64, i32]
+    L.mcgpu_biome20_ctx_create.restype = VP
+    L.mcgpu_biome20_ctx_query.argtypes = [VP, PI32, PI32, PI32, i32, PI32]
+    L.mcgpu_biome20_ctx_destroy.argtypes = [VP]
+    L.mcgpu_biome20_ctx_destroy.restype = None
+    for name in ("mcgpu_xoroshiro_nextlong_batch", "mcgpu_xperlin_sample_batch",
+                 "mcgpu_xperlin_sample_fixed_seed", "mcgpu_continentalness_noise_fixed_seed",
+                 "mcgpu_climate5_fixed_seed", "mcgpu_climate5_quant_fixed_seed",
+                 "mcgpu_climate5_i32_fixed_seed", "mcgpu_climate_ctx_query_i32",
+                 "mcgpu_climate6_i32_fixed_seed", "mcgpu_climate6_i32_fixed_seed_ex",
+                 "mcgpu_biome20_i32_fixed_seed", "mcgpu_biome20_ctx_query"):
+        getattr(L, name).restype = C.c_int
     _lib = L
     return L
 
@@ -474,7 +509,8 @@ class BiomeContext:
         return lib().mcgpu_modern_ctx_last_filter_ms(self._h)
 
 
-def filter_multi(world_seeds, candidates, flags=DEEP_DARK | MUSHROOM_FIELDS, large_biomes=False):
+def filter_multi(world_seeds, candidates, flags=DEEP_DARK | MUSHROOM_FIELDS, large_biomes=False,
+                 version=2602):
     """Biome filter of candidates of many worlds in one launch: candidates[w]
     is a list of (center_chunk_x, center_chunk_z) of world world_seeds[w].
     Returns the masks per world and the device time in ms."""
@@ -489,8 +525,94 @@ def filter_multi(world_seeds, candidates, flags=DEEP_DARK | MUSHROOM_FIELDS, lar
     xs = (C.c_int32 * max(1, total))(*[c[0] for c in flat])
     zs = (C.c_int32 * max(1, total))(*[c[1] for c in flat])
     out = (C.c_int32 * max(1, total))()
-    rc = lib().mcgpu_filter_multi(n, seeds, f, xs, zs, flags, 1 if large_biomes else 0, out)
+    rc = lib().mcgpu_filter_multi(n, version, seeds, f, xs, zs, flags, 1 if large_biomes else 0,
+                                  out)
     if rc != 0:
         raise RuntimeError(f"mcgpu_filter_multi failed ({rc})")
     masks = [list(out[first[w]:first[w + 1]]) for w in range(n)]
     return masks, lib().mcgpu_filter_multi_last_ms()
+
+
+# ---- biome primitives and point queries (mcgpu_* of the reference) -----------------
+
+def _u64(v):
+    return (C.c_uint64 * max(1, len(v)))(*[x & ((1 << 64) - 1) for x in v])
+
+
+def _f64(v):
+    return (C.c_double * max(1, len(v)))(*v)
+
+
+def _i32(v):
+    return (C.c_int32 * max(1, len(v)))(*v)
+
+
+def _check(rc, name):
+    if rc != 0:
+        raise RuntimeError(f"{name} failed ({rc})")
+
+
+def xoroshiro_nextlong(seeds, steps):
+    """mcgpu_xoroshiro_nextlong_batch: steps outputs per seed."""
+    out = (C.c_uint64 * max(1, len(seeds) * steps))()
+    _check(lib().mcgpu_xoroshiro_nextlong_batch(_u64(seeds), len(seeds), steps, out), "xoroshiro")
+    return [list(out[i * steps:(i + 1) * steps]) for i in range(len(seeds))]
+
+
+def perlin_sample(seeds, xs, ys, zs):
+    """mcgpu_xperlin_sample_batch (one seed per point) or, for an int seed,
+    mcgpu_xperlin_sample_fixed_seed."""
+    n = len(xs)
+    out = (C.c_double * max(1, n))()
+    if isinstance(seeds, int):
+        rc = lib().mcgpu_xperlin_sample_fixed_seed(seeds & ((1 << 64) - 1), _f64(xs), _f64(ys),
+                                                   _f64(zs), n, out)
+    else:
+        rc = lib().mcgpu_xperlin_sample_batch(_u64(seeds), _f64(xs), _f64(ys), _f64(zs), n, out)
+    _check(rc, "xperlin")
+    return list(out)[:n]
+
+
+def continentalness(world_seed, xs, zs):
+    n = len(xs)
+    out = (C.c_double * max(1, n))()
+    _check(lib().mcgpu_continentalness_noise_fixed_seed(world_seed & ((1 << 64) - 1), _f64(xs),
+                                                        _f64(zs), n, out), "continentalness")
+    return list(out)[:n]
+
+
+def climate5(world_seed, xs, zs, quantized=False, integer=False):
+    """Five climate values (t, h, c, e, w) per point: doubles, quantized at
+    double coordinates, or quantized at int32 coordinates (integer=True)."""
+    n = len(xs)
+    s = world_seed & ((1 << 64) - 1)
+    if integer:
+        out = (C.c_int32 * max(1, 5 * n))()
+        rc = lib().mcgpu_climate5_i32_fixed_seed(s, _i32(xs), _i32(zs), n, out)
+    elif quantized:
+        out = (C.c_int32 * max(1, 5 * n))()
+        rc = lib().mcgpu_climate5_quant_fixed_seed(s, _f64(xs), _f64(zs), n, out)
+    else:
+        out = (C.c_double * max(1, 5 * n))()
+        rc = lib().mcgpu_climate5_fixed_seed(s, _f64(xs), _f64(zs), n, out)
+    _check(rc, "climate5")
+    return [tuple(out[5 * i:5 * i + 5]) for i in range(n)]
+
+
+def climate6(world_seed, xs, ys, zs, large_biomes=False):
+    """Six quantized values (t, h, c, e, depth, w) per quart point."""
+    n = len(xs)
+    out = (C.c_int32 * max(1, 6 * n))()
+    _check(lib().mcgpu_climate6_i32_fixed_seed_ex(world_seed & ((1 << 64) - 1),
+                                                  1 if large_biomes else 0, _i32(xs), _i32(ys),
+                                                  _i32(zs), n, out), "climate6")
+    return [tuple(out[6 * i:6 * i + 6]) for i in range(n)]
+
+
+def biome20(world_seed, xs, ys, zs):
+    """mcgpu_biome20_i32_fixed_seed: biome ids with the 1.20 tree."""
+    n = len(xs)
+    out = (C.c_int32 * max(1, n))()
+    _check(lib().mcgpu_biome20_i32_fixed_seed(world_seed & ((1 << 64) - 1), _i32(xs), _i32(ys),
+                                              _i32(zs), n, out), "biome20")
+    return list(out)[:n]

@@ -1,5 +1,5 @@
 // Minecraft's overworld multi-noise biomes (the 1.18+ climate sampler with
-// the 26.2 parameter tree), as host and device code.
+// the parameter trees of 1.18 to 26.2), as host and device code.
 //
 // A biome at quart coordinates (x, y, z) (blocks / 4) is the leaf of the
 // parameter search tree nearest to the point
@@ -36,6 +36,14 @@ struct BiomeSplinePoint {
     uint32_t loc, der, value;  // binary32 bits
     int32_t child;             // sub-spline giving the value, or -1
 };
+struct BiomeTreeDef {
+    int32_t version;        // first version code (e.g. 1206 = 1.20.6) using the tree
+    int32_t family;         // node layout: 0 (1.18, 1.19.2) or 1 (1.19.4 on)
+    const uint64_t* nodes;
+    int32_t len;
+    const int32_t* params;  // (lo, hi) pairs
+    int32_t nparams;
+};
 
 // Tables that device code reads live in constant memory on the device.
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -46,8 +54,8 @@ struct BiomeSplinePoint {
 #include "biome_data.inc"
 #undef BIOME_TABLE
 
-constexpr int kTreeOrder = 6;          // children per inner node
-constexpr int kTreeDepth = 5;          // levels below the root (steps 1555..1)
+constexpr int kTreeCount = (int)(sizeof(kTreeDefs) / sizeof(kTreeDefs[0]));
+constexpr int kTreeMaxParams = BIOME_TREE_MAX_PARAMS;
 constexpr int kClimateNoises = 6;      // shift, temperature, humidity, continentalness, erosion, weirdness
 constexpr int kClimateOctaves = 46;    // octaves of all six double Perlin noises
 constexpr int kDeepDark = 183;         // biome ids of the filter
@@ -125,7 +133,7 @@ SLIME_HD double grad(uint8_t h, double x, double y, double z) {
 }
 
 // Origins from three doubles, then a Fisher-Yates shuffle of 0..255.
-inline void perlin_init(Perlin& p, Xoro xr) {
+SLIME_HD void perlin_init(Perlin& p, Xoro xr) {
     p.ox = xoro_double(xr) * 256.0;
     p.oy = xoro_double(xr) * 256.0;
     p.oz = xoro_double(xr) * 256.0;
@@ -310,11 +318,21 @@ SLIME_HD float spline_eval(int id, const float* v) {
     return value(last) + bits_float(pt[last].der) * (f - bits_float(pt[last].loc));
 }
 
+// The point the climate noises are sampled at: (x, z) moved by the shift
+// noise (MinecraftGPUNoise.cu:891-894).
+template <typename ClimateRef>
+SLIME_HD void shifted_point(const ClimateRef& c, double x, double z, double& px, double& pz) {
+    px = x + noise_sample(c, kShift, x, 0.0, z) * 4.0;
+    pz = z + noise_sample(c, kShift, z, x, 0.0) * 4.0;
+}
+
+// Climate value as the tree compares it: (int32)(10000 * (float)v).
+SLIME_HD int32_t quantize(double v) { return (int32_t)(10000.0F * (float)v); }
+
 template <typename ClimateRef>
 SLIME_HD Column column_at(const ClimateRef& c, int32_t xi, int32_t zi) {
-    const double x = (double)xi, z = (double)zi;
-    const double px = x + noise_sample(c, kShift, x, 0.0, z) * 4.0;
-    const double pz = z + noise_sample(c, kShift, z, x, 0.0) * 4.0;
+    double px, pz;
+    shifted_point(c, (double)xi, (double)zi, px, pz);
     const float cf = (float)noise_sample(c, kContinentalness, px, 0.0, pz);
     const float ef = (float)noise_sample(c, kErosion, px, 0.0, pz);
     const float wf = (float)noise_sample(c, kWeirdness, px, 0.0, pz);
@@ -337,13 +355,40 @@ SLIME_HD int32_t depth_at(float offset, int32_t yi) {
     return (int32_t)(10000.0F * (float)d);
 }
 
-// ---- parameter tree --------------------------------------------------------------
+// ---- parameter trees ---------------------------------------------------------------
 //
 // Node (64 bits): bytes 0..5 index the parameter interval of each climate
 // dimension; bits 48..63 are the first child's index (children are `step`
 // apart) or, with the top byte 0xFF, a leaf whose byte 6 is the biome id.
+// Two layouts (MinecraftGPUNoise.cu:662-790): family 0 (1.18, 1.19.2) has up
+// to 10 children per node and steps 1111, 111, 11, 1; family 1 (1.19.4 on)
+// up to 6 and steps 1555, 259, 43, 7, 1.
 
-// params: the intervals, flattened (lo, hi) pairs (kTreeParams, or a copy).
+template <int F>
+struct TreeShape;
+template <>
+struct TreeShape<0> {
+    static constexpr int order = 10, depth = 4;
+    static constexpr int step(int d) { return d == 0 ? 1111 : d == 1 ? 111 : d == 2 ? 11 : 1; }
+};
+template <>
+struct TreeShape<1> {
+    static constexpr int order = 6, depth = 5;
+    static constexpr int step(int d) {
+        return d == 0 ? 1555 : d == 1 ? 259 : d == 2 ? 43 : d == 3 ? 7 : 1;
+    }
+};
+
+// The tree of a version code (MinecraftGPUNoise.cu:1785-1820), or -1 below
+// 1.18 (code 1180).
+inline int tree_for_version(int32_t version_code) {
+    if (version_code < 1180) return -1;
+    for (int i = 0; i < kTreeCount; ++i)
+        if (version_code >= kTreeDefs[i].version) return i;
+    return kTreeCount - 1;
+}
+
+// params: the intervals, flattened (lo, hi) pairs.
 SLIME_HD uint64_t node_distance(const int32_t np[6], const uint64_t* nodes, const int32_t* params,
                                 int idx) {
     const uint64_t node = nodes[idx];
@@ -358,23 +403,24 @@ SLIME_HD uint64_t node_distance(const int32_t np[6], const uint64_t* nodes, cons
     return ds;
 }
 
-// Nearest leaf below idx (the search of MinecraftGPUNoise.cu:680-733): visit
+// Nearest leaf below idx (the search of MinecraftGPUNoise.cu:680-790): visit
 // the children whose box is closer than the best leaf so far, in order, and
 // keep a leaf only if strictly closer. Returns the leaf index.
-template <int D>
+template <int F, int D>
 SLIME_HD int tree_nearest(const int32_t np[6], const uint64_t* nodes, const int32_t* params,
                           int len, int idx, int best, uint64_t best_ds) {
-    if constexpr (D >= kTreeDepth) {
+    if constexpr (D >= TreeShape<F>::depth) {
         return idx;
     } else {
-        const int step = kTreeSteps[D];
+        constexpr int step = TreeShape<F>::step(D);
         if (idx + step >= len)
-            return tree_nearest<D + 1>(np, nodes, params, len, idx, best, best_ds);
+            return tree_nearest<F, D + 1>(np, nodes, params, len, idx, best, best_ds);
         int child = (int)((nodes[idx] >> 48) & 0xFFFFu);
-        for (int k = 0; k < kTreeOrder && child < len; ++k, child += step) {
+        for (int k = 0; k < TreeShape<F>::order && child < len; ++k, child += step) {
             const uint64_t dc = node_distance(np, nodes, params, child);
             if (dc < best_ds) {
-                const int leaf = tree_nearest<D + 1>(np, nodes, params, len, child, best, best_ds);
+                const int leaf =
+                    tree_nearest<F, D + 1>(np, nodes, params, len, child, best, best_ds);
                 const uint64_t dl = leaf == child ? dc : node_distance(np, nodes, params, leaf);
                 if (dl < best_ds) {
                     best_ds = dl;
@@ -392,11 +438,12 @@ SLIME_HD int tree_nearest(const int32_t np[6], const uint64_t* nodes, const int3
 // with the bound distance(hint) + 1 prunes more and returns the same leaf:
 // the bound still exceeds the least distance, and only strictly closer leaves
 // are taken. Returns the leaf; hint < 0 starts without one.
+template <int F>
 SLIME_HD int tree_leaf(const int32_t np[6], const uint64_t* nodes, const int32_t* params, int len,
                        int hint) {
-    if (hint < 0) return tree_nearest<0>(np, nodes, params, len, 0, 0, ~0ull);
+    if (hint < 0) return tree_nearest<F, 0>(np, nodes, params, len, 0, 0, ~0ull);
     const uint64_t bound = node_distance(np, nodes, params, hint) + 1;
-    return tree_nearest<0>(np, nodes, params, len, 0, hint, bound);
+    return tree_nearest<F, 0>(np, nodes, params, len, 0, hint, bound);
 }
 
 SLIME_HD int leaf_biome(const uint64_t* nodes, int leaf) {
@@ -405,15 +452,13 @@ SLIME_HD int leaf_biome(const uint64_t* nodes, int leaf) {
 
 // Biome at quart layer yi of a column; `hint` carries the last leaf found
 // (a start bound for the next search) and is updated.
+template <int F>
 SLIME_HD int biome_at_column(const Column& col, int32_t yi, const uint64_t* nodes,
                              const int32_t* params, int len, int& hint) {
     const int32_t np[6] = {col.t, col.h, col.c, col.e, depth_at(col.offset, yi), col.w};
-    hint = tree_leaf(np, nodes, params, len, hint);
+    hint = tree_leaf<F>(np, nodes, params, len, hint);
     return leaf_biome(nodes, hint);
 }
-
-constexpr int kTreeLen = (int)(sizeof(kTreeNodes) / sizeof(kTreeNodes[0]));
-constexpr int kTreeParamCount = (int)(sizeof(kTreeParams) / sizeof(kTreeParams[0]));
 
 }  // namespace biome
 }  // namespace slime

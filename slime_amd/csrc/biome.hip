@@ -1,6 +1,6 @@
-// Biome filter of slime-cluster candidates and biome point queries on the
-// GPU, behind the reference's mcgpu_modern_ctx_* C ABI
-// (MinecraftGPUNoise.cu:978-1105, 1837-1986).
+// Biome filter of slime-cluster candidates and biome / climate point queries
+// on the GPU, behind the reference's mcgpu_* C ABI (MinecraftGPUNoise.cu:
+// 424-1105 kernels and primitives, 1405-2018 contexts and queries).
 //
 // Filter: a candidate is a center chunk; its disk's slime chunks (221-cell
 // disk, Java-exact test) each hold 4 x 4 quart columns; flags bit 0 asks for
@@ -9,15 +9,17 @@
 //   1. its threads test the 17 x 17 chunks and compact the slime chunks in LDS;
 //   2. each thread takes quart columns: the five climate noises (52 double
 //      Perlin octave samples) once per column, then per requested y layer the
-//      depth and the nearest-leaf search of the parameter tree;
+//      depth and the nearest-leaf search of the version's parameter tree;
 //   3. a found biome sets the candidate's mask bit in LDS; columns stop once
 //      every requested bit is set.
 // The climate state (46 octaves, 14.9 KB) is built on the host from the world
-// seed and copied into each workgroup's LDS once.
+// seed and copied into each workgroup's LDS once; so are the tree's intervals.
+// Kernels that search a tree are instantiated for both node layouts.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -38,30 +40,51 @@ __device__ __forceinline__ void load_climate(Climate& dst, const Climate* src) {
 }
 
 static_assert(sizeof(Climate) % 4 == 0, "climate copied as words");
+static_assert(sizeof(Perlin) % 4 == 0, "Perlin state copied as words");
+
+// A parameter tree on the device.
+struct TreeArgs {
+    const uint64_t* nodes;
+    const int32_t* params;  // (lo, hi) pairs
+    int32_t len;
+    int32_t nparams;
+};
+
+__device__ __forceinline__ void load_params(int32_t* dst, const TreeArgs& t) {
+    for (int i = threadIdx.x; i < 2 * t.nparams; i += blockDim.x) dst[i] = t.params[i];
+}
 
 // LDS of a filtering workgroup besides its climate.
 struct FilterLds {
-    int32_t params[2 * kTreeParamCount];  // the tree's intervals (read with divergent indices)
+    int32_t params[2 * kTreeMaxParams];  // the tree's intervals (read with divergent indices)
     uint16_t chunks[kDiskCells];  // slime chunks of the disk (index 17 * (dx+8) + dz+8)
     int nchunks;
     unsigned int mask;
 };
 
 // The leaves of the filter's two biomes (host: found in the tree once).
+constexpr int kMaxTargetLeaves = 4;
 struct TargetLeaves {
-    int32_t deep[4], ndeep;
-    int32_t mush[4], nmush;
+    int32_t deep[kMaxTargetLeaves], ndeep;
+    int32_t mush[kMaxTargetLeaves], nmush;
 };
 
-TargetLeaves find_targets() {
-    TargetLeaves t{};
-    for (int i = 0; i < kTreeLen; ++i) {
-        if ((kTreeNodes[i] >> 56) != 0xFFu) continue;
-        const int b = (int)((kTreeNodes[i] >> 48) & 0xFFu);
-        if (b == kDeepDark && t.ndeep < 4) t.deep[t.ndeep++] = i;
-        if (b == kMushroomFields && t.nmush < 4) t.mush[t.nmush++] = i;
+// Returns false if a biome has more leaves than the list holds.
+bool find_targets(const BiomeTreeDef& tree, TargetLeaves& t) {
+    t = TargetLeaves{};
+    for (int i = 0; i < tree.len; ++i) {
+        if ((tree.nodes[i] >> 56) != 0xFFu) continue;
+        const int b = (int)((tree.nodes[i] >> 48) & 0xFFu);
+        if (b == kDeepDark) {
+            if (t.ndeep == kMaxTargetLeaves) return false;
+            t.deep[t.ndeep++] = i;
+        }
+        if (b == kMushroomFields) {
+            if (t.nmush == kMaxTargetLeaves) return false;
+            t.mush[t.nmush++] = i;
+        }
     }
-    return t;
+    return true;
 }
 
 // Whether the search at np can return one of `leaves`: not if every one of
@@ -80,25 +103,21 @@ __device__ __forceinline__ bool may_be(const int32_t np[6], const uint64_t* node
 
 // Whether layer yi of a column is the biome of `leaves`, searching only when
 // a target leaf could be the nearest; hint carries the last leaf found.
+template <int F>
 __device__ __forceinline__ bool layer_is(const Column& col, int32_t yi, const uint64_t* nodes,
                                         const int32_t* params, int32_t len, int& hint,
                                         const int32_t* leaves, int nleaves, int biome) {
     const int32_t np[6] = {col.t, col.h, col.c, col.e, depth_at(col.offset, yi), col.w};
     if (!may_be(np, nodes, params, hint, leaves, nleaves)) return false;
-    hint = tree_leaf(np, nodes, params, len, hint);
+    hint = tree_leaf<F>(np, nodes, params, len, hint);
     return leaf_biome(nodes, hint) == biome;
-}
-
-__device__ __forceinline__ void load_params(int32_t* dst) {
-    for (int i = threadIdx.x; i < 2 * kTreeParamCount; i += blockDim.x)
-        dst[i] = kTreeParams[i >> 1][i & 1];
 }
 
 // The mask of one candidate, by the whole workgroup (every thread calls it;
 // the caller synchronises before the first call when the LDS was in use).
+template <int F>
 __device__ __forceinline__ unsigned int filter_candidate(const Climate& st, FilterLds& f,
-                                                         const uint64_t* nodes,
-                                                         const int32_t* params, int32_t len,
+                                                         const uint64_t* nodes, int32_t len,
                                                          const TargetLeaves& tl,
                                                          int64_t world_seed, int32_t cx,
                                                          int32_t cz, int32_t flags, int& hint) {
@@ -119,6 +138,7 @@ __device__ __forceinline__ unsigned int filter_candidate(const Climate& st, Filt
     __syncthreads();
     const int columns = f.nchunks * 16;
     const unsigned int want = (unsigned int)flags;
+    const int32_t* params = f.params;
     for (int item = threadIdx.x; item < columns; item += blockDim.x) {
         if ((__atomic_load_n(&f.mask, __ATOMIC_RELAXED) & want) == want) break;
         const int ci = f.chunks[item >> 4], q = item & 15;
@@ -128,12 +148,13 @@ __device__ __forceinline__ unsigned int filter_candidate(const Climate& st, Filt
         const Column col = column_at(st, qx, qz);
         unsigned int found = 0;
         // hint: the thread's last leaf, a bound for every later layer and column
-        if ((want & 2u) && layer_is(col, 16, nodes, params, len, hint, tl.mush, tl.nmush,
-                                    kMushroomFields))
+        if ((want & 2u) && layer_is<F>(col, 16, nodes, params, len, hint, tl.mush, tl.nmush,
+                                       kMushroomFields))
             found |= 2u;
         if (want & 1u) {
             for (int32_t y = 0; y >= -16; --y) {
-                if (layer_is(col, y, nodes, params, len, hint, tl.deep, tl.ndeep, kDeepDark)) {
+                if (layer_is<F>(col, y, nodes, params, len, hint, tl.deep, tl.ndeep,
+                                kDeepDark)) {
                     found |= 1u;
                     break;
                 }
@@ -145,8 +166,8 @@ __device__ __forceinline__ unsigned int filter_candidate(const Climate& st, Filt
     return f.mask;
 }
 
-__global__ __launch_bounds__(kThreads) void filter_kernel(const Climate* climate,
-                                                          const uint64_t* nodes, int32_t len,
+template <int F>
+__global__ __launch_bounds__(kThreads) void filter_kernel(const Climate* climate, TreeArgs tree,
                                                           TargetLeaves tl, int64_t world_seed,
                                                           const int32_t* ccx, const int32_t* ccz,
                                                           int32_t count, int32_t flags,
@@ -154,12 +175,12 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(const Climate* climate
     __shared__ Climate st;
     __shared__ FilterLds f;
     load_climate(st, climate);
-    load_params(f.params);
+    load_params(f.params, tree);
     int hint = -1;
     for (int cand = blockIdx.x; cand < count; cand += gridDim.x) {
         __syncthreads();  // the previous candidate's LDS is no longer read
-        const unsigned int m = filter_candidate(st, f, nodes, f.params, len, tl, world_seed,
-                                                ccx[cand], ccz[cand], flags, hint);
+        const unsigned int m = filter_candidate<F>(st, f, tree.nodes, tree.len, tl, world_seed,
+                                                   ccx[cand], ccz[cand], flags, hint);
         if (threadIdx.x == 0) mask_out[cand] = (int32_t)m;
     }
 }
@@ -167,13 +188,14 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(const Climate* climate
 // Candidates of many worlds: world w's candidates are [first[w], first[w+1]),
 // its climate climates[w]. A workgroup takes a contiguous run of candidates,
 // so it reloads its LDS climate only where the run crosses into a new world.
+template <int F>
 __global__ __launch_bounds__(kThreads) void filter_multi_kernel(
     const Climate* climates, const int64_t* seeds, const int32_t* first, int32_t nworlds,
-    const uint64_t* nodes, int32_t len, TargetLeaves tl, const int32_t* ccx, const int32_t* ccz,
-    int32_t count, int32_t per_block, int32_t flags, int32_t* mask_out) {
+    TreeArgs tree, TargetLeaves tl, const int32_t* ccx, const int32_t* ccz, int32_t count,
+    int32_t per_block, int32_t flags, int32_t* mask_out) {
     __shared__ Climate st;
     __shared__ FilterLds f;
-    load_params(f.params);
+    load_params(f.params, tree);
     const int begin = blockIdx.x * per_block;
     const int end = min(count, begin + per_block);
     int world = -1;
@@ -186,26 +208,27 @@ __global__ __launch_bounds__(kThreads) void filter_multi_kernel(
             load_climate(st, climates + w);
             world = w;
         }
-        const unsigned int m = filter_candidate(st, f, nodes, f.params, len, tl, seeds[w],
-                                                ccx[cand], ccz[cand], flags, hint);
+        const unsigned int m = filter_candidate<F>(st, f, tree.nodes, tree.len, tl, seeds[w],
+                                                   ccx[cand], ccz[cand], flags, hint);
         if (threadIdx.x == 0) mask_out[cand] = (int32_t)m;
     }
 }
 
-__global__ __launch_bounds__(kThreads) void query_kernel(const Climate* climate,
-                                                         const uint64_t* nodes, int32_t len,
+// Biome ids at quart points (MinecraftGPUNoise.cu:868-976).
+template <int F>
+__global__ __launch_bounds__(kThreads) void query_kernel(const Climate* climate, TreeArgs tree,
                                                          const int32_t* xs, const int32_t* ys,
                                                          const int32_t* zs, int32_t count,
                                                          int32_t* out) {
     __shared__ Climate st;
-    __shared__ int32_t params[2 * kTreeParamCount];
+    __shared__ int32_t params[2 * kTreeMaxParams];
     load_climate(st, climate);
-    load_params(params);
+    load_params(params, tree);
     __syncthreads();
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
         const Column col = column_at(st, xs[i], zs[i]);
         int hint = -1;
-        out[i] = biome_at_column(col, ys[i], nodes, params, len, hint);
+        out[i] = biome_at_column<F>(col, ys[i], tree.nodes, params, tree.len, hint);
     }
 }
 
@@ -234,73 +257,259 @@ __global__ __launch_bounds__(kThreads) void column_kernel(const Climate* climate
     }
 }
 
+// Climate at points, the reference's point-query layouts
+// (MinecraftGPUNoise.cu:532-655, 811-866):
+//   kRaw5    five doubles (t, h, c, e, w) at double coordinates;
+//   kQuant5  the same quantized to int32;
+//   kQuant6  six int32 (t, h, c, e, depth at quart y, w).
+enum ClimateOut { kRaw5 = 0, kQuant5 = 1, kQuant6 = 2 };
+
+template <typename Coord, int Mode>
+__global__ __launch_bounds__(kThreads) void climate_kernel(const Climate* climate,
+                                                           const Coord* xs, const int32_t* ys,
+                                                           const Coord* zs, int32_t count,
+                                                           void* out) {
+    __shared__ Climate st;
+    load_climate(st, climate);
+    __syncthreads();
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        double px, pz;
+        shifted_point(st, (double)xs[i], (double)zs[i], px, pz);
+        const double c = noise_sample(st, kContinentalness, px, 0.0, pz);
+        const double e = noise_sample(st, kErosion, px, 0.0, pz);
+        const double w = noise_sample(st, kWeirdness, px, 0.0, pz);
+        const double t = noise_sample(st, kTemperature, px, 0.0, pz);
+        const double h = noise_sample(st, kHumidity, px, 0.0, pz);
+        if constexpr (Mode == kRaw5) {
+            double* o = static_cast<double*>(out) + (size_t)i * 5;
+            o[0] = t;
+            o[1] = h;
+            o[2] = c;
+            o[3] = e;
+            o[4] = w;
+        } else if constexpr (Mode == kQuant5) {
+            int32_t* o = static_cast<int32_t*>(out) + (size_t)i * 5;
+            o[0] = quantize(t);
+            o[1] = quantize(h);
+            o[2] = quantize(c);
+            o[3] = quantize(e);
+            o[4] = quantize(w);
+        } else {
+            int32_t* o = static_cast<int32_t*>(out) + (size_t)i * 6;
+            const float cf = (float)c, ef = (float)e, wf = (float)w;
+            const float sv[4] = {cf, ef, peaks_and_valleys(wf), wf};
+            o[0] = quantize(t);
+            o[1] = quantize(h);
+            o[2] = quantize(c);
+            o[3] = quantize(e);
+            o[4] = depth_at(spline_eval<0>(0, sv) + 0.015F, ys[i]);
+            o[5] = quantize(w);
+        }
+    }
+}
+
+// The continentalness noise alone, unquantized, at (x, 0, z) without the
+// shift (MinecraftGPUNoise.cu:491-517).
+__global__ __launch_bounds__(kThreads) void continentalness_kernel(const Climate* climate,
+                                                                   const double* xs,
+                                                                   const double* zs,
+                                                                   int32_t count, double* out) {
+    __shared__ Climate st;
+    load_climate(st, climate);
+    __syncthreads();
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
+        out[i] = noise_sample(st, kContinentalness, xs[i], 0.0, zs[i]);
+}
+
+// `steps` xoroshiro128++ outputs of each seed (MinecraftGPUNoise.cu:424-443).
+__global__ __launch_bounds__(kThreads) void xoro_kernel(const uint64_t* seeds, int32_t count,
+                                                        int32_t steps, uint64_t* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    Xoro s = xoro_from_seed(seeds[i]);
+    uint64_t* o = out + (size_t)i * (size_t)steps;
+    for (int32_t j = 0; j < steps; ++j) o[j] = xoro_next(s);
+}
+
+// One Perlin noise per query, seeded from its own seed
+// (MinecraftGPUNoise.cu:445-461): the state is built in the thread.
+__global__ __launch_bounds__(kThreads) void perlin_batch_kernel(const uint64_t* seeds,
+                                                                const double* xs,
+                                                                const double* ys,
+                                                                const double* zs, int32_t count,
+                                                                double* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    Perlin p;
+    perlin_init(p, xoro_from_seed(seeds[i]));
+    out[i] = perlin_sample(p, xs[i], ys[i], zs[i]);
+}
+
+// One Perlin noise for every query (MinecraftGPUNoise.cu:463-489).
+__global__ __launch_bounds__(kThreads) void perlin_fixed_kernel(const Perlin* state,
+                                                                const double* xs,
+                                                                const double* ys,
+                                                                const double* zs, int32_t count,
+                                                                double* out) {
+    __shared__ Perlin p;
+    const uint32_t* s = reinterpret_cast<const uint32_t*>(state);
+    uint32_t* d = reinterpret_cast<uint32_t*>(&p);
+    for (int k = threadIdx.x; k < (int)(sizeof(Perlin) / 4); k += blockDim.x) d[k] = s[k];
+    __syncthreads();
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
+        out[i] = perlin_sample(p, xs[i], ys[i], zs[i]);
+}
+
+int device_cus() {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        n <= 0) {
+        (void)hipGetLastError();
+        return 256;
+    }
+    return n;
+}
+
+int grid_for(int32_t count, int cus) {
+    return (int)std::max<int64_t>(
+        1, std::min<int64_t>((count + kThreads - 1) / kThreads, (int64_t)cus * 4));
+}
+
+// Device copies of the trees, per device, made on first use and kept until
+// the process ends (a few hundred KB).
+struct DeviceTrees {
+    int device = -1;
+    uint64_t* nodes[kTreeCount] = {};
+    int32_t* params[kTreeCount] = {};
+};
+
+std::mutex g_tree_mu;
+std::vector<DeviceTrees> g_trees;
+
+bool device_tree(int tree, TreeArgs& out) {
+    int dev = 0;
+    if (tree < 0 || tree >= kTreeCount || hipGetDevice(&dev) != hipSuccess) return false;
+    std::lock_guard<std::mutex> lock(g_tree_mu);
+    DeviceTrees* dt = nullptr;
+    for (auto& d : g_trees)
+        if (d.device == dev) dt = &d;
+    if (!dt) {
+        g_trees.emplace_back();
+        dt = &g_trees.back();
+        dt->device = dev;
+    }
+    const BiomeTreeDef& def = kTreeDefs[tree];
+    if (!dt->nodes[tree]) {
+        uint64_t* n = nullptr;
+        int32_t* p = nullptr;
+        const size_t nb = sizeof(uint64_t) * (size_t)def.len;
+        const size_t pb = sizeof(int32_t) * 2 * (size_t)def.nparams;
+        if (hipMalloc(&n, nb) != hipSuccess || hipMalloc(&p, pb) != hipSuccess ||
+            hipMemcpy(n, def.nodes, nb, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(p, def.params, pb, hipMemcpyHostToDevice) != hipSuccess) {
+            if (n) (void)hipFree(n);
+            if (p) (void)hipFree(p);
+            return false;
+        }
+        dt->nodes[tree] = n;
+        dt->params[tree] = p;
+    }
+    out.nodes = dt->nodes[tree];
+    out.params = dt->params[tree];
+    out.len = def.len;
+    out.nparams = def.nparams;
+    return true;
+}
+
 }  // namespace
 
-// A world's biome source on one device: climate state, tree and I/O buffers.
+// A world's biome source on one device: climate state, parameter tree (or
+// none for climate-only contexts) and I/O buffers of `capacity` points.
 struct Context {
     int device = -1;
     int32_t capacity = 0;
     int64_t world_seed = 0;
     int cus = 256;
+    int tree = -1;
+    int family = 1;
+    TreeArgs targs{};
+    TargetLeaves targets{};
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     Climate* d_climate = nullptr;
-    uint64_t* d_nodes = nullptr;
+    // d_x / d_z hold an int32 or a double per point, d_out up to five
+    // doubles (ten words) per point.
     int32_t *d_x = nullptr, *d_y = nullptr, *d_z = nullptr, *d_out = nullptr;
     float last_ms = 0.f;
-    TargetLeaves targets = find_targets();
 
     ~Context() {
         if (device >= 0) (void)hipSetDevice(device);
-        for (void* p : {(void*)d_climate, (void*)d_nodes, (void*)d_x, (void*)d_y, (void*)d_z,
-                        (void*)d_out})
+        for (void* p : {(void*)d_climate, (void*)d_x, (void*)d_y, (void*)d_z, (void*)d_out})
             if (p) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
-    bool init(uint64_t seed, bool large, int32_t cap) {
-        if (hipGetDevice(&device) != hipSuccess) return false;
+    // MCGPU_OK, or MCGPU_ERR_ALLOC / MCGPU_ERR_CUDA.
+    int init(uint64_t seed, bool large, int tree_index, int32_t cap) {
+        if (hipGetDevice(&device) != hipSuccess) return MCGPU_ERR_CUDA;
         capacity = cap;
         world_seed = (int64_t)seed;
-        int n = 0;
-        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) ==
-                hipSuccess &&
-            n > 0)
-            cus = n;
+        cus = device_cus();
+        tree = tree_index;
+        if (tree >= 0) {
+            family = kTreeDefs[tree].family;
+            if (!find_targets(kTreeDefs[tree], targets) || !device_tree(tree, targs))
+                return MCGPU_ERR_ALLOC;
+        }
         Climate* host = new (std::nothrow) Climate;
-        if (!host) return false;
+        if (!host) return MCGPU_ERR_ALLOC;
         std::memset(host, 0, sizeof(Climate));
         climate_init(*host, seed, large);
         const size_t ni = sizeof(int32_t) * (size_t)cap;
-        const bool ok =
-            hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) == hipSuccess &&
-            hipEventCreate(&ev0) == hipSuccess && hipEventCreate(&ev1) == hipSuccess &&
-            hipMalloc(&d_climate, sizeof(Climate)) == hipSuccess &&
-            hipMalloc(&d_nodes, sizeof(kTreeNodes)) == hipSuccess &&
-            hipMalloc(&d_x, ni) == hipSuccess && hipMalloc(&d_y, ni) == hipSuccess &&
-            hipMalloc(&d_z, ni) == hipSuccess && hipMalloc(&d_out, 6 * ni) == hipSuccess &&
-            hipMemcpy(d_climate, host, sizeof(Climate), hipMemcpyHostToDevice) == hipSuccess &&
-            hipMemcpy(d_nodes, kTreeNodes, sizeof(kTreeNodes), hipMemcpyHostToDevice) ==
-                hipSuccess;
+        const bool alloc = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) == hipSuccess &&
+                           hipEventCreate(&ev0) == hipSuccess &&
+                           hipEventCreate(&ev1) == hipSuccess &&
+                           hipMalloc(&d_climate, sizeof(Climate)) == hipSuccess &&
+                           hipMalloc(&d_x, 2 * ni) == hipSuccess &&
+                           hipMalloc(&d_y, ni) == hipSuccess &&
+                           hipMalloc(&d_z, 2 * ni) == hipSuccess &&
+                           hipMalloc(&d_out, 10 * ni) == hipSuccess;
+        const bool copied =
+            alloc && hipMemcpy(d_climate, host, sizeof(Climate), hipMemcpyHostToDevice) ==
+                         hipSuccess;
         delete host;
-        return ok;
+        return !alloc ? MCGPU_ERR_ALLOC : copied ? MCGPU_OK : MCGPU_ERR_CUDA;
+    }
+
+    int finish(void* out, size_t bytes) {
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(out, d_out, bytes, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+            hipStreamSynchronize(stream) != hipSuccess)
+            return MCGPU_ERR_CUDA;
+        return MCGPU_OK;
+    }
+
+    bool upload(int32_t* dst, const void* src, size_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream) == hipSuccess;
     }
 
     int filter(const int32_t* cx, const int32_t* cz, int32_t count, int32_t flags, int32_t* out) {
         const size_t ni = sizeof(int32_t) * (size_t)count;
         const int grid = (int)std::min<int64_t>(count, (int64_t)cus * 4);
-        if (hipMemcpyAsync(d_x, cx, ni, hipMemcpyHostToDevice, stream) != hipSuccess ||
-            hipMemcpyAsync(d_z, cz, ni, hipMemcpyHostToDevice, stream) != hipSuccess ||
+        if (!upload(d_x, cx, ni) || !upload(d_z, cz, ni) ||
             hipEventRecord(ev0, stream) != hipSuccess)
             return MCGPU_ERR_CUDA;
-        hipLaunchKernelGGL(filter_kernel, dim3(grid), dim3(kThreads), 0, stream, d_climate,
-                           d_nodes, kTreeLen, targets, world_seed, d_x, d_z, count, flags, d_out);
+        if (family == 0)
+            hipLaunchKernelGGL(filter_kernel<0>, dim3(grid), dim3(kThreads), 0, stream, d_climate,
+                               targs, targets, world_seed, d_x, d_z, count, flags, d_out);
+        else
+            hipLaunchKernelGGL(filter_kernel<1>, dim3(grid), dim3(kThreads), 0, stream, d_climate,
+                               targs, targets, world_seed, d_x, d_z, count, flags, d_out);
         if (hipGetLastError() != hipSuccess || hipEventRecord(ev1, stream) != hipSuccess ||
-            hipMemcpyAsync(out, d_out, ni, hipMemcpyDeviceToHost, stream) != hipSuccess ||
-            hipStreamSynchronize(stream) != hipSuccess ||
-            hipEventElapsedTime(&last_ms, ev0, ev1) != hipSuccess)
+            finish(out, ni) != MCGPU_OK || hipEventElapsedTime(&last_ms, ev0, ev1) != hipSuccess)
             return MCGPU_ERR_CUDA;
         return MCGPU_OK;
     }
@@ -308,44 +517,82 @@ struct Context {
     int query(const int32_t* xs, const int32_t* ys, const int32_t* zs, int32_t count,
               int32_t* out) {
         const size_t ni = sizeof(int32_t) * (size_t)count;
-        const int grid = (int)std::min<int64_t>((count + kThreads - 1) / kThreads, (int64_t)cus * 4);
-        if (hipMemcpyAsync(d_x, xs, ni, hipMemcpyHostToDevice, stream) != hipSuccess ||
-            hipMemcpyAsync(d_y, ys, ni, hipMemcpyHostToDevice, stream) != hipSuccess ||
-            hipMemcpyAsync(d_z, zs, ni, hipMemcpyHostToDevice, stream) != hipSuccess)
+        const int grid = grid_for(count, cus);
+        if (!upload(d_x, xs, ni) || !upload(d_y, ys, ni) || !upload(d_z, zs, ni))
             return MCGPU_ERR_CUDA;
-        hipLaunchKernelGGL(query_kernel, dim3(grid), dim3(kThreads), 0, stream, d_climate, d_nodes,
-                           kTreeLen, d_x, d_y, d_z, count, d_out);
-        if (hipGetLastError() != hipSuccess ||
-            hipMemcpyAsync(out, d_out, ni, hipMemcpyDeviceToHost, stream) != hipSuccess ||
-            hipStreamSynchronize(stream) != hipSuccess)
-            return MCGPU_ERR_CUDA;
-        return MCGPU_OK;
+        if (family == 0)
+            hipLaunchKernelGGL(query_kernel<0>, dim3(grid), dim3(kThreads), 0, stream, d_climate,
+                               targs, d_x, d_y, d_z, count, d_out);
+        else
+            hipLaunchKernelGGL(query_kernel<1>, dim3(grid), dim3(kThreads), 0, stream, d_climate,
+                               targs, d_x, d_y, d_z, count, d_out);
+        return finish(out, ni);
     }
 
     int columns(const int32_t* xs, const int32_t* zs, int32_t count, int32_t* out6) {
         const size_t ni = sizeof(int32_t) * (size_t)count;
-        const int grid = (int)std::min<int64_t>((count + kThreads - 1) / kThreads, (int64_t)cus * 4);
-        if (hipMemcpyAsync(d_x, xs, ni, hipMemcpyHostToDevice, stream) != hipSuccess ||
-            hipMemcpyAsync(d_z, zs, ni, hipMemcpyHostToDevice, stream) != hipSuccess)
+        if (!upload(d_x, xs, ni) || !upload(d_z, zs, ni)) return MCGPU_ERR_CUDA;
+        hipLaunchKernelGGL(column_kernel, dim3(grid_for(count, cus)), dim3(kThreads), 0, stream,
+                           d_climate, d_x, d_z, count, d_out);
+        return finish(out6, 6 * ni);
+    }
+
+    // Climate at int32 points: five quantized values, or six with the depth
+    // at quart y (ys != null).
+    int climate_i32(const int32_t* xs, const int32_t* ys, const int32_t* zs, int32_t count,
+                    int32_t* out) {
+        const size_t ni = sizeof(int32_t) * (size_t)count;
+        if (!upload(d_x, xs, ni) || !upload(d_z, zs, ni) || (ys && !upload(d_y, ys, ni)))
             return MCGPU_ERR_CUDA;
-        hipLaunchKernelGGL(column_kernel, dim3(grid), dim3(kThreads), 0, stream, d_climate, d_x,
-                           d_z, count, d_out);
-        if (hipGetLastError() != hipSuccess ||
-            hipMemcpyAsync(out6, d_out, 6 * ni, hipMemcpyDeviceToHost, stream) != hipSuccess ||
-            hipStreamSynchronize(stream) != hipSuccess)
-            return MCGPU_ERR_CUDA;
-        return MCGPU_OK;
+        if (ys)
+            hipLaunchKernelGGL((climate_kernel<int32_t, kQuant6>), dim3(grid_for(count, cus)),
+                               dim3(kThreads), 0, stream, d_climate, d_x, d_y, d_z, count,
+                               (void*)d_out);
+        else
+            hipLaunchKernelGGL((climate_kernel<int32_t, kQuant5>), dim3(grid_for(count, cus)),
+                               dim3(kThreads), 0, stream, d_climate, d_x, nullptr, d_z, count,
+                               (void*)d_out);
+        return finish(out, (ys ? 6 : 5) * ni);
+    }
+
+    // Climate at double points: five raw doubles, or five quantized int32.
+    int climate_f64(const double* xs, const double* zs, int32_t count, bool quantized, void* out) {
+        const size_t nd = sizeof(double) * (size_t)count;
+        if (!upload(d_x, xs, nd) || !upload(d_z, zs, nd)) return MCGPU_ERR_CUDA;
+        const double* dx = reinterpret_cast<const double*>(d_x);
+        const double* dz = reinterpret_cast<const double*>(d_z);
+        if (quantized)
+            hipLaunchKernelGGL((climate_kernel<double, kQuant5>), dim3(grid_for(count, cus)),
+                               dim3(kThreads), 0, stream, d_climate, dx, nullptr, dz, count,
+                               (void*)d_out);
+        else
+            hipLaunchKernelGGL((climate_kernel<double, kRaw5>), dim3(grid_for(count, cus)),
+                               dim3(kThreads), 0, stream, d_climate, dx, nullptr, dz, count,
+                               (void*)d_out);
+        return finish(out, (quantized ? sizeof(int32_t) : sizeof(double)) * 5 * (size_t)count);
+    }
+
+    int continentalness(const double* xs, const double* zs, int32_t count, double* out) {
+        const size_t nd = sizeof(double) * (size_t)count;
+        if (!upload(d_x, xs, nd) || !upload(d_z, zs, nd)) return MCGPU_ERR_CUDA;
+        hipLaunchKernelGGL(continentalness_kernel, dim3(grid_for(count, cus)), dim3(kThreads), 0,
+                           stream, d_climate, reinterpret_cast<const double*>(d_x),
+                           reinterpret_cast<const double*>(d_z), count,
+                           reinterpret_cast<double*>(d_out));
+        return finish(out, nd);
     }
 };
 
 // Filter of candidates of many worlds in one launch (see filter_multi_kernel).
-int filter_multi(int32_t nworlds, const uint64_t* seeds, const int32_t* first, const int32_t* cx,
-                 const int32_t* cz, int32_t flags, bool large, int32_t* out, float* ms) {
+int filter_multi(int32_t nworlds, int tree, const uint64_t* seeds, const int32_t* first,
+                 const int32_t* cx, const int32_t* cz, int32_t flags, bool large, int32_t* out,
+                 float* ms) {
     const int32_t count = first[nworlds];
-    int dev = 0, cus = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-        return MCGPU_ERR_CUDA;
+    const int cus = device_cus();
+    TreeArgs targs{};
+    TargetLeaves targets{};
+    if (!find_targets(kTreeDefs[tree], targets) || !device_tree(tree, targs))
+        return MCGPU_ERR_ALLOC;
     std::vector<Climate> host((size_t)nworlds);
     std::vector<int64_t> sseed((size_t)nworlds);
     for (int32_t w = 0; w < nworlds; ++w) {
@@ -356,7 +603,6 @@ int filter_multi(int32_t nworlds, const uint64_t* seeds, const int32_t* first, c
     Climate* d_clim = nullptr;
     int64_t* d_seeds = nullptr;
     int32_t *d_first = nullptr, *d_x = nullptr, *d_z = nullptr, *d_out = nullptr;
-    uint64_t* d_nodes = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     const size_t ni = sizeof(int32_t) * (size_t)count;
@@ -369,7 +615,6 @@ int filter_multi(int32_t nworlds, const uint64_t* seeds, const int32_t* first, c
               hipMalloc(&d_clim, sizeof(Climate) * (size_t)nworlds) == hipSuccess &&
               hipMalloc(&d_seeds, sizeof(int64_t) * (size_t)nworlds) == hipSuccess &&
               hipMalloc(&d_first, sizeof(int32_t) * (size_t)(nworlds + 1)) == hipSuccess &&
-              hipMalloc(&d_nodes, sizeof(kTreeNodes)) == hipSuccess &&
               hipMalloc(&d_x, ni) == hipSuccess && hipMalloc(&d_z, ni) == hipSuccess &&
               hipMalloc(&d_out, ni) == hipSuccess &&
               hipMemcpyAsync(d_clim, host.data(), sizeof(Climate) * (size_t)nworlds,
@@ -378,22 +623,25 @@ int filter_multi(int32_t nworlds, const uint64_t* seeds, const int32_t* first, c
                              hipMemcpyHostToDevice, stream) == hipSuccess &&
               hipMemcpyAsync(d_first, first, sizeof(int32_t) * (size_t)(nworlds + 1),
                              hipMemcpyHostToDevice, stream) == hipSuccess &&
-              hipMemcpyAsync(d_nodes, kTreeNodes, sizeof(kTreeNodes), hipMemcpyHostToDevice,
-                             stream) == hipSuccess &&
               hipMemcpyAsync(d_x, cx, ni, hipMemcpyHostToDevice, stream) == hipSuccess &&
               hipMemcpyAsync(d_z, cz, ni, hipMemcpyHostToDevice, stream) == hipSuccess &&
               hipEventRecord(e0, stream) == hipSuccess;
     if (ok) {
-        hipLaunchKernelGGL(filter_multi_kernel, dim3(grid), dim3(kThreads), 0, stream, d_clim,
-                           d_seeds, d_first, nworlds, d_nodes, kTreeLen, find_targets(), d_x, d_z,
-                           count, per_block, flags, d_out);
+        if (kTreeDefs[tree].family == 0)
+            hipLaunchKernelGGL(filter_multi_kernel<0>, dim3(grid), dim3(kThreads), 0, stream,
+                               d_clim, d_seeds, d_first, nworlds, targs, targets, d_x, d_z, count,
+                               per_block, flags, d_out);
+        else
+            hipLaunchKernelGGL(filter_multi_kernel<1>, dim3(grid), dim3(kThreads), 0, stream,
+                               d_clim, d_seeds, d_first, nworlds, targs, targets, d_x, d_z, count,
+                               per_block, flags, d_out);
         ok = hipGetLastError() == hipSuccess && hipEventRecord(e1, stream) == hipSuccess &&
              hipMemcpyAsync(out, d_out, ni, hipMemcpyDeviceToHost, stream) == hipSuccess &&
              hipStreamSynchronize(stream) == hipSuccess &&
              hipEventElapsedTime(ms, e0, e1) == hipSuccess;
     }
-    for (void* p : {(void*)d_clim, (void*)d_seeds, (void*)d_first, (void*)d_nodes, (void*)d_x,
-                    (void*)d_z, (void*)d_out})
+    for (void* p : {(void*)d_clim, (void*)d_seeds, (void*)d_first, (void*)d_x, (void*)d_z,
+                    (void*)d_out})
         if (p) (void)hipFree(p);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -407,8 +655,50 @@ int filter_multi(int32_t nworlds, const uint64_t* seeds, const int32_t* first, c
 using slime::biome::Context;
 
 namespace {
+
 float g_multi_ms = 0.f;
+
+// A new context, or null (the device error, if any, cleared).
+Context* new_context(uint64_t seed, bool large, int tree, int32_t capacity, int* rc = nullptr) {
+    Context* ctx = new (std::nothrow) Context;
+    const int r = ctx ? ctx->init(seed, large, tree, capacity) : MCGPU_ERR_ALLOC;
+    if (r != MCGPU_OK) {
+        delete ctx;
+        ctx = nullptr;
+        (void)hipGetLastError();
+    }
+    if (rc) *rc = r;
+    return ctx;
 }
+
+int cleared(int rc) {
+    if (rc != MCGPU_OK) (void)hipGetLastError();
+    return rc;
+}
+
+// Runs fn on a context made for one call (the reference's *_fixed_seed calls
+// allocate and free per call too).
+template <typename Fn>
+int one_shot(uint64_t seed, bool large, int tree, int32_t count, Fn fn) {
+    int rc = MCGPU_OK;
+    Context* ctx = new_context(seed, large, tree, count, &rc);
+    if (!ctx) return rc;
+    rc = cleared(fn(*ctx));
+    delete ctx;
+    return rc;
+}
+
+// A point count: positive (an int32 count keeps every buffer, at most 48
+// bytes per point, far from size_t overflow).
+bool count_ok(int32_t count) { return count > 0; }
+
+int tree_20() { return slime::biome::tree_for_version(1206); }
+
+unsigned blocks_for(int32_t count) {
+    return (unsigned)((count + slime::biome::kThreads - 1) / slime::biome::kThreads);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -418,16 +708,9 @@ const char* mcgpu_build_name(void) { return "slime_amd biome filter (gfx950, HIP
 
 void* mcgpu_modern_ctx_create_ex(int32_t version_code, uint64_t world_seed, int32_t flags,
                                  int32_t capacity) {
-    // the parameter tree here is the 26.2 one: older versions are refused
-    if (capacity <= 0 || version_code < 2602 || (flags & ~1) != 0) return nullptr;
-    Context* ctx = new (std::nothrow) Context;
-    if (!ctx) return nullptr;
-    if (!ctx->init(world_seed, (flags & 1) != 0, capacity)) {
-        delete ctx;
-        (void)hipGetLastError();
-        return nullptr;
-    }
-    return ctx;
+    const int tree = slime::biome::tree_for_version(version_code);
+    if (capacity <= 0 || tree < 0 || (flags & ~1) != 0) return nullptr;
+    return new_context(world_seed, (flags & 1) != 0, tree, capacity);
 }
 
 void* mcgpu_modern_ctx_create(int32_t version_code, uint64_t world_seed, int32_t capacity) {
@@ -440,58 +723,46 @@ int mcgpu_modern_ctx_filter_candidates(void* handle, const int32_t* center_chunk
                                        const int32_t* center_chunk_z, int32_t count,
                                        int32_t flags, int32_t* mask_out) {
     Context* ctx = static_cast<Context*>(handle);
-    if (!ctx || !center_chunk_x || !center_chunk_z || !mask_out || count <= 0 ||
+    if (!ctx || ctx->tree < 0 || !center_chunk_x || !center_chunk_z || !mask_out || count <= 0 ||
         count > ctx->capacity || (flags & ~3) != 0 || flags == 0)
         return MCGPU_ERR_ARG;
-    const int rc = ctx->filter(center_chunk_x, center_chunk_z, count, flags, mask_out);
-    if (rc != MCGPU_OK) (void)hipGetLastError();
-    return rc;
+    return cleared(ctx->filter(center_chunk_x, center_chunk_z, count, flags, mask_out));
 }
 
 int mcgpu_modern_ctx_query(void* handle, const int32_t* xs, const int32_t* ys, const int32_t* zs,
                            int32_t count, int32_t* biome_out) {
     Context* ctx = static_cast<Context*>(handle);
-    if (!ctx || !xs || !ys || !zs || !biome_out || count <= 0 || count > ctx->capacity)
+    if (!ctx || ctx->tree < 0 || !xs || !ys || !zs || !biome_out || count <= 0 ||
+        count > ctx->capacity)
         return MCGPU_ERR_ARG;
-    const int rc = ctx->query(xs, ys, zs, count, biome_out);
-    if (rc != MCGPU_OK) (void)hipGetLastError();
-    return rc;
+    return cleared(ctx->query(xs, ys, zs, count, biome_out));
 }
 
 int32_t mcgpu_modern_recommended_filter_batch(void) {
-    int dev = 0, cus = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        cus <= 0) {
-        (void)hipGetLastError();
-        return 128;
-    }
-    return cus * 16;  // candidates per call that keep every CU's workgroups busy
+    return slime::biome::device_cus() * 16;  // candidates per call that keep every CU busy
 }
 
 int mcgpu_modern_ctx_columns(void* handle, const int32_t* xs, const int32_t* zs, int32_t count,
                              int32_t* out6) {
     Context* ctx = static_cast<Context*>(handle);
     if (!ctx || !xs || !zs || !out6 || count <= 0 || count > ctx->capacity) return MCGPU_ERR_ARG;
-    const int rc = ctx->columns(xs, zs, count, out6);
-    if (rc != MCGPU_OK) (void)hipGetLastError();
-    return rc;
+    return cleared(ctx->columns(xs, zs, count, out6));
 }
 
-int mcgpu_filter_multi(int32_t nworlds, const uint64_t* world_seeds, const int32_t* first,
-                       const int32_t* center_chunk_x, const int32_t* center_chunk_z,
-                       int32_t flags, int32_t large_biomes, int32_t* mask_out) {
-    if (nworlds <= 0 || !world_seeds || !first || !center_chunk_x || !center_chunk_z ||
-        !mask_out || (flags & ~3) != 0 || flags == 0 || first[0] != 0)
+int mcgpu_filter_multi(int32_t nworlds, int32_t version_code, const uint64_t* world_seeds,
+                       const int32_t* first, const int32_t* center_chunk_x,
+                       const int32_t* center_chunk_z, int32_t flags, int32_t large_biomes,
+                       int32_t* mask_out) {
+    const int tree = slime::biome::tree_for_version(version_code);
+    if (nworlds <= 0 || tree < 0 || !world_seeds || !first || !center_chunk_x ||
+        !center_chunk_z || !mask_out || (flags & ~3) != 0 || flags == 0 || first[0] != 0)
         return MCGPU_ERR_ARG;
     for (int32_t w = 0; w < nworlds; ++w)
         if (first[w + 1] < first[w]) return MCGPU_ERR_ARG;
     if (first[nworlds] == 0) return MCGPU_OK;
-    const int rc = slime::biome::filter_multi(nworlds, world_seeds, first, center_chunk_x,
+    return cleared(slime::biome::filter_multi(nworlds, tree, world_seeds, first, center_chunk_x,
                                               center_chunk_z, flags, large_biomes != 0, mask_out,
-                                              &g_multi_ms);
-    if (rc != MCGPU_OK) (void)hipGetLastError();
-    return rc;
+                                              &g_multi_ms));
 }
 
 float mcgpu_filter_multi_last_ms(void) { return g_multi_ms; }
@@ -500,5 +771,179 @@ float mcgpu_modern_ctx_last_filter_ms(void* handle) {
     Context* ctx = static_cast<Context*>(handle);
     return ctx ? ctx->last_ms : -1.f;
 }
+
+// ---- primitives and point queries ----
+
+int mcgpu_xoroshiro_nextlong_batch(const uint64_t* seeds, int32_t count, int32_t steps,
+                                   uint64_t* out) {
+    if (!seeds || !out || count <= 0 || steps <= 0) return MCGPU_ERR_ARG;
+    const size_t n_out = (size_t)count * (size_t)steps;
+    if (n_out > ((size_t)1 << 60)) return MCGPU_ERR_ARG;
+    uint64_t *d_seeds = nullptr, *d_out = nullptr;
+    if (hipMalloc(&d_seeds, sizeof(uint64_t) * (size_t)count) != hipSuccess ||
+        hipMalloc(&d_out, sizeof(uint64_t) * n_out) != hipSuccess) {
+        if (d_seeds) (void)hipFree(d_seeds);
+        (void)hipGetLastError();
+        return MCGPU_ERR_ALLOC;
+    }
+    int rc = MCGPU_OK;
+    if (hipMemcpy(d_seeds, seeds, sizeof(uint64_t) * (size_t)count, hipMemcpyHostToDevice) !=
+        hipSuccess) {
+        rc = MCGPU_ERR_CUDA;
+    } else {
+        hipLaunchKernelGGL(slime::biome::xoro_kernel, dim3(blocks_for(count)),
+                           dim3(slime::biome::kThreads), 0, nullptr, d_seeds, count, steps, d_out);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+            hipMemcpy(out, d_out, sizeof(uint64_t) * n_out, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = MCGPU_ERR_CUDA;
+    }
+    (void)hipFree(d_out);
+    (void)hipFree(d_seeds);
+    return cleared(rc);
+}
+
+int mcgpu_xperlin_sample_batch(const uint64_t* seeds, const double* xs, const double* ys,
+                               const double* zs, int32_t count, double* out) {
+    if (!seeds || !xs || !ys || !zs || !out || !count_ok(count)) return MCGPU_ERR_ARG;
+    const size_t n8 = sizeof(double) * (size_t)count;
+    uint64_t* d_seeds = nullptr;
+    double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr, *d_out = nullptr;
+    if (hipMalloc(&d_seeds, n8) != hipSuccess || hipMalloc(&d_x, n8) != hipSuccess ||
+        hipMalloc(&d_y, n8) != hipSuccess || hipMalloc(&d_z, n8) != hipSuccess ||
+        hipMalloc(&d_out, n8) != hipSuccess) {
+        for (void* p : {(void*)d_seeds, (void*)d_x, (void*)d_y, (void*)d_z, (void*)d_out})
+            if (p) (void)hipFree(p);
+        (void)hipGetLastError();
+        return MCGPU_ERR_ALLOC;
+    }
+    int rc = MCGPU_OK;
+    if (hipMemcpy(d_seeds, seeds, n8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_x, xs, n8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_y, ys, n8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_z, zs, n8, hipMemcpyHostToDevice) != hipSuccess) {
+        rc = MCGPU_ERR_CUDA;
+    } else {
+        hipLaunchKernelGGL(slime::biome::perlin_batch_kernel, dim3(blocks_for(count)),
+                           dim3(slime::biome::kThreads), 0, nullptr, d_seeds, d_x, d_y, d_z,
+                           count, d_out);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+            hipMemcpy(out, d_out, n8, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = MCGPU_ERR_CUDA;
+    }
+    for (void* p : {(void*)d_seeds, (void*)d_x, (void*)d_y, (void*)d_z, (void*)d_out})
+        (void)hipFree(p);
+    return cleared(rc);
+}
+
+int mcgpu_xperlin_sample_fixed_seed(uint64_t seed, const double* xs, const double* ys,
+                                    const double* zs, int32_t count, double* out) {
+    if (!xs || !ys || !zs || !out || !count_ok(count)) return MCGPU_ERR_ARG;
+    const size_t n8 = sizeof(double) * (size_t)count;
+    slime::biome::Perlin host;
+    std::memset(&host, 0, sizeof(host));
+    slime::biome::perlin_init(host, slime::biome::xoro_from_seed(seed));
+    slime::biome::Perlin* d_state = nullptr;
+    double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr, *d_out = nullptr;
+    if (hipMalloc(&d_state, sizeof(host)) != hipSuccess || hipMalloc(&d_x, n8) != hipSuccess ||
+        hipMalloc(&d_y, n8) != hipSuccess || hipMalloc(&d_z, n8) != hipSuccess ||
+        hipMalloc(&d_out, n8) != hipSuccess) {
+        for (void* p : {(void*)d_state, (void*)d_x, (void*)d_y, (void*)d_z, (void*)d_out})
+            if (p) (void)hipFree(p);
+        (void)hipGetLastError();
+        return MCGPU_ERR_ALLOC;
+    }
+    int rc = MCGPU_OK;
+    if (hipMemcpy(d_state, &host, sizeof(host), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_x, xs, n8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_y, ys, n8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_z, zs, n8, hipMemcpyHostToDevice) != hipSuccess) {
+        rc = MCGPU_ERR_CUDA;
+    } else {
+        const int grid = slime::biome::grid_for(count, slime::biome::device_cus());
+        hipLaunchKernelGGL(slime::biome::perlin_fixed_kernel, dim3(grid),
+                           dim3(slime::biome::kThreads), 0, nullptr, d_state, d_x, d_y, d_z,
+                           count, d_out);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+            hipMemcpy(out, d_out, n8, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = MCGPU_ERR_CUDA;
+    }
+    for (void* p : {(void*)d_state, (void*)d_x, (void*)d_y, (void*)d_z, (void*)d_out})
+        (void)hipFree(p);
+    return cleared(rc);
+}
+
+int mcgpu_continentalness_noise_fixed_seed(uint64_t world_seed, const double* xs,
+                                           const double* zs, int32_t count, double* out) {
+    if (!xs || !zs || !out || !count_ok(count)) return MCGPU_ERR_ARG;
+    return one_shot(world_seed, false, -1, count,
+                    [&](Context& c) { return c.continentalness(xs, zs, count, out); });
+}
+
+int mcgpu_climate5_fixed_seed(uint64_t world_seed, const double* xs, const double* zs,
+                              int32_t count, double* out5) {
+    if (!xs || !zs || !out5 || !count_ok(count)) return MCGPU_ERR_ARG;
+    return one_shot(world_seed, false, -1, count,
+                    [&](Context& c) { return c.climate_f64(xs, zs, count, false, out5); });
+}
+
+int mcgpu_climate5_quant_fixed_seed(uint64_t world_seed, const double* xs, const double* zs,
+                                    int32_t count, int32_t* out5) {
+    if (!xs || !zs || !out5 || !count_ok(count)) return MCGPU_ERR_ARG;
+    return one_shot(world_seed, false, -1, count,
+                    [&](Context& c) { return c.climate_f64(xs, zs, count, true, out5); });
+}
+
+int mcgpu_climate5_i32_fixed_seed(uint64_t world_seed, const int32_t* xs, const int32_t* zs,
+                                  int32_t count, int32_t* out5) {
+    if (!xs || !zs || !out5 || !count_ok(count)) return MCGPU_ERR_ARG;
+    return one_shot(world_seed, false, -1, count,
+                    [&](Context& c) { return c.climate_i32(xs, nullptr, zs, count, out5); });
+}
+
+void* mcgpu_climate_ctx_create(uint64_t world_seed, int32_t capacity) {
+    if (!count_ok(capacity)) return nullptr;
+    return new_context(world_seed, false, -1, capacity);
+}
+
+int mcgpu_climate_ctx_query_i32(void* handle, const int32_t* xs, const int32_t* zs,
+                                int32_t count, int32_t* out5) {
+    Context* ctx = static_cast<Context*>(handle);
+    if (!ctx || !xs || !zs || !out5 || count <= 0 || count > ctx->capacity) return MCGPU_ERR_ARG;
+    return cleared(ctx->climate_i32(xs, nullptr, zs, count, out5));
+}
+
+void mcgpu_climate_ctx_destroy(void* handle) { delete static_cast<Context*>(handle); }
+
+int mcgpu_climate6_i32_fixed_seed_ex(uint64_t world_seed, int32_t large, const int32_t* xs,
+                                     const int32_t* ys, const int32_t* zs, int32_t count,
+                                     int32_t* out6) {
+    if (!xs || !ys || !zs || !out6 || !count_ok(count)) return MCGPU_ERR_ARG;
+    return one_shot(world_seed, large != 0, -1, count,
+                    [&](Context& c) { return c.climate_i32(xs, ys, zs, count, out6); });
+}
+
+int mcgpu_climate6_i32_fixed_seed(uint64_t world_seed, const int32_t* xs, const int32_t* ys,
+                                  const int32_t* zs, int32_t count, int32_t* out6) {
+    return mcgpu_climate6_i32_fixed_seed_ex(world_seed, 0, xs, ys, zs, count, out6);
+}
+
+int mcgpu_biome20_i32_fixed_seed(uint64_t world_seed, const int32_t* xs, const int32_t* ys,
+                                 const int32_t* zs, int32_t count, int32_t* biome_out) {
+    if (!xs || !ys || !zs || !biome_out || !count_ok(count)) return MCGPU_ERR_ARG;
+    return one_shot(world_seed, false, tree_20(), count,
+                    [&](Context& c) { return c.query(xs, ys, zs, count, biome_out); });
+}
+
+void* mcgpu_biome20_ctx_create(uint64_t world_seed, int32_t capacity) {
+    if (!count_ok(capacity)) return nullptr;
+    return new_context(world_seed, false, tree_20(), capacity);
+}
+
+int mcgpu_biome20_ctx_query(void* handle, const int32_t* xs, const int32_t* ys, const int32_t* zs,
+                            int32_t count, int32_t* biome_out) {
+    return mcgpu_modern_ctx_query(handle, xs, ys, zs, count, biome_out);
+}
+
+void mcgpu_biome20_ctx_destroy(void* handle) { delete static_cast<Context*>(handle); }
 
 }  // extern "C"

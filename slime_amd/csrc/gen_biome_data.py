@@ -1,13 +1,14 @@
 #!/usr/bin/env python3
 """Writes biome_data.inc: the data the biome filter needs to reproduce
-Minecraft's multi-noise overworld biomes (version 26.2).
+Minecraft's multi-noise overworld biomes (versions 1.18 to 26.2).
 
   * the seeds of the climate and octave noises: MD5 digests of Minecraft's
     noise names ("minecraft:temperature", "octave_-9", ...), computed here;
-  * the biome parameter search tree and its parameter intervals, and the
-    terrain-offset cubic spline: Minecraft data as encoded by cubiomes (MIT,
-    see gpu_noise/LICENSE.cubiomes of the reference), read from the reference's
-    gpu_noise/tables/btree262.h and gpu_noise/spline_generated.cuh.
+  * the biome parameter search trees of each version and their parameter
+    intervals, and the terrain-offset cubic spline: Minecraft data as encoded
+    by cubiomes (MIT, see gpu_noise/LICENSE.cubiomes of the reference), read
+    from the reference's gpu_noise/tables/btree*.h and
+    gpu_noise/spline_generated.cuh.
 
 Run in a checkout that has the reference next to it:
     python3 slime_amd/csrc/gen_biome_data.py /root/reference
@@ -101,17 +102,31 @@ NOISES = [  # (slot, name, large name, first octave, large first octave, amplitu
 ]
 
 
+# (file suffix, first version code using it, tree shape family): the version
+# selection of MinecraftGPUNoise.cu:1785-1820. Family 0 (1.18, 1.19.2): steps
+# 1111, 111, 11, 1 and 10 children per node; family 1 (1.19.4 on): steps
+# 1555, 259, 43, 7, 1 and 6 children.
+TREES = [("262", 2602, 1), ("215", 1215, 1), ("21wd", 1214, 1), ("20", 1206, 1),
+         ("19", 1194, 1), ("192", 1192, 0), ("18", 1180, 0)]
+FAMILY_STEPS = {0: [1111, 111, 11, 1, 0], 1: [1555, 259, 43, 7, 1, 0]}
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
-    steps, params, nodes = read_tree(os.path.join(ref, "gpu_noise", "tables", "btree262.h"))
+    trees = []
+    for suffix, version, family in TREES:
+        steps, params, nodes = read_tree(
+            os.path.join(ref, "gpu_noise", "tables", f"btree{suffix}.h"))
+        assert steps == FAMILY_STEPS[family], (suffix, steps)
+        trees.append((suffix, version, family, params, nodes))
     splines = read_splines(os.path.join(ref, "gpu_noise", "spline_generated.cuh"))
     L = []
     w = L.append
     w("// Generated by gen_biome_data.py; do not edit. Data of Minecraft's overworld")
-    w("// multi-noise biome source (version 26.2): noise seeds (MD5 of the noise")
-    w("// names), the biome parameter search tree and the terrain-offset spline,")
-    w("// the latter two as encoded by cubiomes (MIT). BIOME_TABLE marks the tables")
-    w("// device code reads (the includer defines it).")
+    w("// multi-noise biome source (versions 1.18 to 26.2): noise seeds (MD5 of the")
+    w("// noise names), the biome parameter search trees and the terrain-offset")
+    w("// spline, the latter two as encoded by cubiomes (MIT). BIOME_TABLE marks the")
+    w("// tables device code reads (the includer defines it).")
     w("")
     w("// Noise definitions: MD5 seed halves of the name (and of the large-biome")
     w("// name), first octave, number of octaves, amplitudes.")
@@ -131,15 +146,25 @@ def main():
         w(f"    {{0x{lo:016x}ull, 0x{hi:016x}ull}},  // octave_{i}")
     w("};")
     w("")
-    w(f"BIOME_TABLE int32_t kTreeSteps[{len(steps)}] = {{{', '.join(map(str, steps))}}};")
-    w(f"BIOME_TABLE int32_t kTreeParams[{len(params)}][2] = {{")
-    for i in range(0, len(params), 6):
-        w("    " + ", ".join(f"{{{a}, {b}}}" for a, b in params[i:i + 6]) + ",")
+    for suffix, version, family, params, nodes in trees:
+        w(f"// Parameter tree of version code >= {version} (family {family}).")
+        w(f"static const int32_t kTreeParams{suffix}[{len(params)}][2] = {{")
+        for i in range(0, len(params), 6):
+            w("    " + ", ".join(f"{{{a}, {b}}}" for a, b in params[i:i + 6]) + ",")
+        w("};")
+        w(f"static const uint64_t kTreeNodes{suffix}[{len(nodes)}] = {{")
+        for i in range(0, len(nodes), 4):
+            w("    " + ", ".join(f"0x{n:016x}ull" for n in nodes[i:i + 4]) + ",")
+        w("};")
+    w("")
+    w("// The trees, newest first: first version code, shape family, nodes, node")
+    w("// count, intervals, interval count.")
+    w(f"static const BiomeTreeDef kTreeDefs[{len(trees)}] = {{")
+    for suffix, version, family, params, nodes in trees:
+        w(f"    {{{version}, {family}, kTreeNodes{suffix}, {len(nodes)}, &kTreeParams{suffix}[0][0], "
+          f"{len(params)}}},")
     w("};")
-    w(f"static const uint64_t kTreeNodes[{len(nodes)}] = {{")
-    for i in range(0, len(nodes), 4):
-        w("    " + ", ".join(f"0x{n:016x}ull" for n in nodes[i:i + 4]) + ",")
-    w("};")
+    w(f"#define BIOME_TREE_MAX_PARAMS {max(len(t[3]) for t in trees)}")
     w("")
     npts = sum(len(p) for _, p in splines)
     w("// Splines: coordinate (0 continentalness, 1 erosion, 2 peaks-and-valleys,")
@@ -158,8 +183,8 @@ def main():
     w("};")
     out = os.path.join(HERE, "biome_data.inc")
     open(out, "w").write("\n".join(L) + "\n")
-    print(f"wrote {out}: {len(nodes)} tree nodes, {len(params)} intervals, {len(splines)} "
-          f"splines / {npts} points")
+    print(f"wrote {out}: {len(trees)} trees ({sum(len(t[4]) for t in trees)} nodes), "
+          f"{len(splines)} splines / {npts} points")
 
 
 if __name__ == "__main__":

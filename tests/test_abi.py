@@ -36,7 +36,7 @@ def test_python_mirror_lists_every_export():
 
 def test_library_exports_every_biome_symbol(native):
     names = declared_functions(BIOME_HEADER)
-    assert len(names) == 12
+    assert len(names) == 28
     for name in names:
         assert hasattr(native, name), name
     assert native.mcgpu_api_version() == 1

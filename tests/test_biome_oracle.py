@@ -6,8 +6,9 @@ reference's CUDA build:
     against independent Python restatements of the published algorithms;
   * the terrain spline: bit-exact against the reference's own spline source
     compiled for the CPU (oracle/_ref, where built);
-  * the parameter tree: the generated table equals the reference's table text;
-    the pruned nearest-leaf search equals an exhaustive search.
+  * the parameter trees of every version: the generated tables equal the
+    reference's table text; the version selection; the pruned nearest-leaf
+    search of both node layouts equals an exhaustive search.
 Biome ids against the reference's CUDA build or cubiomes: parity unpinned.
 """
 import ctypes as C
@@ -23,7 +24,8 @@ import pytest
 from conftest import ROOT
 
 INC = os.path.join(ROOT, "slime_amd", "csrc", "biome_data.inc")
-REF_TREE = "/root/reference/gpu_noise/tables/btree262.h"
+REF_TABLES = "/root/reference/gpu_noise/tables"
+TREE_IDS = list(range(7))
 M64 = (1 << 64) - 1
 
 
@@ -192,22 +194,50 @@ def test_spline_equals_reference_source(bl):
     assert cases == 20000
 
 
-def test_tree_table_equals_reference_text():
-    if not os.path.exists(REF_TREE):
-        pytest.skip("reference tree table not present")
-    ref = open(REF_TREE).read()
+def inc_tree(suffix):
+    """(nodes, intervals) of one tree as generated into biome_data.inc."""
+    text = inc_text()
+    nodes = [int(h, 16) for h in re.findall(
+        r"0x([0-9a-f]{16})ull", text.split(f"kTreeNodes{suffix}[", 1)[1].split("};", 1)[0])]
+    param = [(int(a), int(b)) for a, b in re.findall(
+        r"\{(-?\d+), (-?\d+)\}", text.split(f"kTreeParams{suffix}[", 1)[1].split("};", 1)[0])]
+    return nodes, param
+
+
+@pytest.mark.parametrize("tree", TREE_IDS)
+def test_tree_tables_equal_reference_text(bl, tree):
+    from oracle.pyoracle import BIOME_TREES
+    suffix, _ = BIOME_TREES[tree]
+    path = os.path.join(REF_TABLES, f"btree{suffix}.h")
+    if not os.path.exists(path):
+        pytest.skip("reference tree tables not present")
+    ref = open(path).read()
     ref_nodes = [int(h, 16) for h in re.findall(r"\b0x([0-9A-Fa-f]{16})\b", ref.split("_nodes[]", 1)[1])]
     ref_param = [(int(a), int(b)) for a, b in re.findall(
         r"\{\s*(-?\d+)\s*,\s*(-?\d+)\s*\}", re.sub(r"//[^\n]*", "", ref.split("_param[][2]", 1)[1].split("};", 1)[0]))]
-    text = inc_text()
-    nodes = [int(h, 16) for h in re.findall(r"0x([0-9a-f]{16})ull", text.split("kTreeNodes", 1)[1].split("};", 1)[0])]
-    param = [(int(a), int(b)) for a, b in re.findall(
-        r"\{(-?\d+), (-?\d+)\}", text.split("kTreeParams", 1)[1].split("};", 1)[0])]
-    assert nodes == ref_nodes and len(nodes) == 9113
-    assert param == ref_param and len(param) == 143
+    nodes, param = inc_tree(suffix)
+    assert nodes == ref_nodes and len(nodes) > 8000
+    assert param == ref_param
+    # the oracle reads the same tree under this index
+    assert bl.ob_tree_len(tree) == len(nodes)
+    assert [bl.ob_tree_node(tree, i) for i in (0, 1, len(nodes) // 2, len(nodes) - 1)] == \
+        [nodes[0], nodes[1], nodes[len(nodes) // 2], nodes[-1]]
+    # the reference's steps table: family 1 is 1555..1, family 0 1111..1
+    steps = [int(x) for x in re.search(r"_steps\[\]\s*=\s*\{([^}]*)\}", ref).group(1).split(",") if x.strip()]
+    assert steps == ([1555, 259, 43, 7, 1, 0] if bl.ob_tree_family(tree) else [1111, 111, 11, 1, 0])
 
 
-def test_pruned_tree_search_is_nearest(bl):
+def test_version_selection(bl):
+    """MinecraftGPUNoise.cu:1785-1820: the first tree whose version code is
+    reached; nothing below 1.18."""
+    want = {1179: -1, 1180: 6, 1191: 6, 1192: 5, 1193: 5, 1194: 4, 1205: 4, 1206: 3, 1213: 3,
+            1214: 2, 1215: 1, 2601: 1, 2602: 0, 9999: 0}
+    for v, t in want.items():
+        assert bl.ob_tree_for_version(v) == t, v
+
+
+@pytest.mark.parametrize("tree", TREE_IDS)
+def test_pruned_tree_search_is_nearest(bl, tree):
     rng = random.Random(11)
     ties = 0
     for i in range(3000):
@@ -215,9 +245,9 @@ def test_pruned_tree_search_is_nearest(bl):
         if i % 3 == 0:
             np6[4] = rng.randint(-2000, 12000)  # depth-like values
         a = (C.c_int32 * 6)(*np6)
-        leaf = bl.ob_tree_leaf(a)
-        best = bl.ob_tree_leaf_exhaustive(a)
-        d_leaf, d_best = bl.ob_node_distance(a, leaf), bl.ob_node_distance(a, best)
+        leaf = bl.ob_tree_leaf(a, tree)
+        best = bl.ob_tree_leaf_exhaustive(a, tree)
+        d_leaf, d_best = bl.ob_node_distance(a, leaf, tree), bl.ob_node_distance(a, best, tree)
         assert d_leaf == d_best, np6
         if leaf != best:
             ties += 1
@@ -228,8 +258,7 @@ def test_pruned_tree_search_is_nearest(bl):
 def test_biome_ids_and_filter_on_a_small_world(bl):
     """Every quart of a small area gets a leaf's biome id; the filter of a
     candidate equals the union over its disk's slime chunks' quarts."""
-    text = inc_text()
-    nodes = [int(h, 16) for h in re.findall(r"0x([0-9a-f]{16})ull", text.split("kTreeNodes", 1)[1].split("};", 1)[0])]
+    nodes, _ = inc_tree("262")
     leaf_ids = {(n >> 48) & 0xFF for n in nodes if n >> 56 == 0xFF}
     c = bl.ob_climate_new(1234, 0)
     try:
@@ -238,30 +267,31 @@ def test_biome_ids_and_filter_on_a_small_world(bl):
         for _ in range(400):
             x, z = rng.randint(-60000, 60000), rng.randint(-60000, 60000)
             for y in (-16, -8, 0, 16):
-                b = bl.ob_biome_at(c, x, y, z)
+                b = bl.ob_biome_at(c, x, y, z, 0)
                 assert b in leaf_ids
                 seen.add(b)
         assert len(seen) >= 10
-        m = bl.ob_filter(c, 1234, 3, -7, 3)
+        m = bl.ob_filter(c, 1234, 3, -7, 3, 0)
         assert 0 <= m <= 3
     finally:
         bl.ob_climate_free(c)
 
 
-def test_search_from_any_leaf_equals_search_from_root(bl):
+@pytest.mark.parametrize("tree", TREE_IDS)
+def test_search_from_any_leaf_equals_search_from_root(bl, tree):
     """Starting the search from a known leaf (bound: its distance + 1) returns
     the root search's leaf, for random hints and for the previous layer's leaf
     of real columns."""
+    from oracle.pyoracle import BIOME_TREES
     rng = random.Random(13)
-    text = inc_text()
-    nodes = [int(h, 16) for h in re.findall(r"0x([0-9a-f]{16})ull", text.split("kTreeNodes", 1)[1].split("};", 1)[0])]
+    nodes, _ = inc_tree(BIOME_TREES[tree][0])
     leaves = [i for i, n in enumerate(nodes) if n >> 56 == 0xFF]
     for _ in range(2000):
         np6 = [rng.randint(-12000, 12000) for _ in range(6)]
         a = (C.c_int32 * 6)(*np6)
-        cold = bl.ob_tree_leaf(a)
-        assert bl.ob_tree_leaf_from(a, rng.choice(leaves)) == cold
-        assert bl.ob_tree_leaf_from(a, cold) == cold
+        cold = bl.ob_tree_leaf(a, tree)
+        assert bl.ob_tree_leaf_from(a, rng.choice(leaves), tree) == cold
+        assert bl.ob_tree_leaf_from(a, cold, tree) == cold
     c = bl.ob_climate_new(77, 0)
     col = (C.c_int32 * 6)()
     try:
@@ -272,9 +302,9 @@ def test_search_from_any_leaf_equals_search_from_root(bl):
             for y in [16] + list(range(0, -17, -1)):
                 d = f32(1.0 - (y * 4) / 128.0 - 83.0 / 160.0 + off)
                 np6 = (C.c_int32 * 6)(col[0], col[1], col[2], col[3], int(f32(10000.0 * d)), col[4])
-                cold = bl.ob_tree_leaf(np6)
+                cold = bl.ob_tree_leaf(np6, tree)
                 if hint >= 0:
-                    assert bl.ob_tree_leaf_from(np6, hint) == cold
+                    assert bl.ob_tree_leaf_from(np6, hint, tree) == cold
                 hint = cold
     finally:
         bl.ob_climate_free(c)

@@ -1,6 +1,7 @@
-"""GPU parity of the biome filter (mcgpu_modern_ctx_*) against the CPU
+"""GPU parity of the biome filter and point queries (mcgpu_*) against the CPU
 restatement in oracle/biome_oracle.c: the per-column climate, biome ids at
-quart points, and filter masks of slime-cluster candidates, bit-exact.
+quart points for every version's tree, filter masks of slime-cluster
+candidates, and the noise / climate primitives, bit-exact.
 (Biome ids against the reference's CUDA build: parity unpinned; see
 tests/test_biome_oracle.py for what pins the restatement.)"""
 import ctypes as C
@@ -42,20 +43,20 @@ def test_columns_and_queries_match_oracle(gpu, bl, large):
             want_cols = oracle_columns(bl, clim, xs, zs)
             assert got_cols == want_cols
             got = ctx.query(xs, ys, zs)
-            want = [bl.ob_biome_at(clim, x, y, z) for x, y, z in zip(xs, ys, zs)]
+            want = [bl.ob_biome_at(clim, x, y, z, 0) for x, y, z in zip(xs, ys, zs)]
             assert got == want
             assert len(set(got)) >= 10
     finally:
         bl.ob_climate_free(clim)
 
 
-def find_biome_chunk(bl, clim, seed, biome, ys, rng, tries=20000):
+def find_biome_chunk(bl, clim, seed, biome, ys, rng, tries=20000, tree=0):
     """A slime chunk holding a quart of the biome at one of the layers ys."""
     from oracle import pyoracle
     ol = pyoracle.oracle_lib()
     for _ in range(tries):
         qx, qz = rng.randint(-4_000_000, 4_000_000), rng.randint(-4_000_000, 4_000_000)
-        if any(bl.ob_biome_at(clim, qx, y, qz) == biome for y in ys):
+        if any(bl.ob_biome_at(clim, qx, y, qz, tree) == biome for y in ys):
             cx, cz = qx >> 2, qz >> 2
             if ol.oracle_is_slime(seed, cx, cz):
                 return cx, cz
@@ -82,7 +83,7 @@ def test_filter_matches_oracle(gpu, bl):
         with slime_amd.BiomeContext(seed, capacity=len(cands)) as ctx:
             for flags in (3, 1, 2):
                 got = ctx.filter(cx, cz, flags)
-                want = [bl.ob_filter(clim, seed, x, z, flags) for x, z in cands]
+                want = [bl.ob_filter(clim, seed, x, z, flags, 0) for x, z in cands]
                 print("flags", flags, got, flush=True)
                 assert got == want
             if dd:
@@ -94,7 +95,7 @@ def test_filter_matches_oracle(gpu, bl):
 def test_filter_rejects_bad_arguments(gpu):
     import slime_amd
     L = slime_amd.lib()
-    assert L.mcgpu_modern_ctx_create(2601, 1, 16) is None  # older trees are not carried
+    assert L.mcgpu_modern_ctx_create(1179, 1, 16) is None  # before 1.18
     assert L.mcgpu_modern_ctx_create(2602, 1, 0) is None
     with slime_amd.BiomeContext(1, capacity=4) as ctx:
         a = (C.c_int32 * 8)()
@@ -121,7 +122,140 @@ def test_filter_multi_equals_per_world_contexts(gpu, bl):
             assert got == ctx.filter([c[0] for c in cs], [c[1] for c in cs], 3)
         clim = bl.ob_climate_new(seed, 0)
         try:
-            assert got == [bl.ob_filter(clim, seed - (1 << 64) if seed >> 63 else seed, x, z, 3)
+            assert got == [bl.ob_filter(clim, seed - (1 << 64) if seed >> 63 else seed, x, z, 3, 0)
                            for x, z in cs]
         finally:
             bl.ob_climate_free(clim)
+
+
+VERSIONS = [(2602, 0), (1215, 1), (1214, 2), (1206, 3), (1194, 4), (1192, 5), (1180, 6)]
+
+
+@pytest.mark.parametrize("version,tree", VERSIONS)
+def test_every_version_tree(gpu, bl, version, tree):
+    """Queries and filter masks with each version's tree (both node layouts)
+    against the oracle; 1.18 has no Deep Dark leaf."""
+    import slime_amd
+    seed = 0x0DDBA11 + version
+    rng = random.Random(version)
+    n = 1500
+    xs = [rng.randint(-3_000_000, 3_000_000) for _ in range(n)]
+    zs = [rng.randint(-3_000_000, 3_000_000) for _ in range(n)]
+    ys = [rng.randint(-20, 20) for _ in range(n)]
+    clim = bl.ob_climate_new(seed, 0)
+    try:
+        with slime_amd.BiomeContext(seed, capacity=n, version=version) as ctx:
+            got = ctx.query(xs, ys, zs)
+            assert got == [bl.ob_biome_at(clim, x, y, z, tree) for x, y, z in zip(xs, ys, zs)]
+            cands = [(rng.randint(-500_000, 500_000), rng.randint(-500_000, 500_000))
+                     for _ in range(12)]
+            mf = find_biome_chunk(bl, clim, seed, 14, [16], rng, tries=40000, tree=tree)
+            if mf:
+                cands.append((mf[0] - 1, mf[1] + 4))
+            cx, cz = [c[0] for c in cands], [c[1] for c in cands]
+            print("version", version, "planted", mf, flush=True)
+            got = ctx.filter(cx, cz, 3)
+            assert got == [bl.ob_filter(clim, seed, x, z, 3, tree) for x, z in cands]
+            if mf:
+                assert got[-1] & 2
+            if version < 1190:
+                assert all(m & 1 == 0 for m in got)
+        masks, _ = slime_amd.filter_multi([seed], [cands], 3, version=version)
+        assert masks == [got]
+    finally:
+        bl.ob_climate_free(clim)
+
+
+def quant(v):
+    import numpy as np
+    return int(np.float32(10000.0) * np.float32(v))
+
+
+def test_noise_primitives(gpu, bl):
+    """xoroshiro128++ streams and Perlin noise (per-query seeds and one fixed
+    seed) against the restatement."""
+    import slime_amd
+    rng = random.Random(44)
+    seeds = [rng.getrandbits(64) for _ in range(300)] + [0, (1 << 64) - 1]
+    got = slime_amd.xoroshiro_nextlong(seeds, 9)
+    buf = (C.c_uint64 * 9)()
+    for s_, g in zip(seeds, got):
+        bl.ob_xoro_stream(s_, 9, buf)
+        assert g == list(buf)
+    n = 2000
+    xs = [rng.uniform(-3e6, 3e6) for _ in range(n)]
+    ys = [rng.choice([0.0, rng.uniform(-600, 600)]) for _ in range(n)]
+    zs = [rng.uniform(-3e6, 3e6) for _ in range(n)]
+    ps = [rng.getrandbits(64) for _ in range(n)]
+    assert slime_amd.perlin_sample(ps, xs, ys, zs) == \
+        [bl.ob_perlin_sample_seeded(p, x, y, z) for p, x, y, z in zip(ps, xs, ys, zs)]
+    fixed = 0xC0FFEE123
+    assert slime_amd.perlin_sample(fixed, xs, ys, zs) == \
+        [bl.ob_perlin_sample_seeded(fixed, x, y, z) for x, y, z in zip(xs, ys, zs)]
+
+
+@pytest.mark.parametrize("large", [False, True])
+def test_climate_point_queries(gpu, bl, large):
+    """climate5 (doubles, quantized, int32 points), climate6 (with depth; large
+    biomes through the _ex form), the climate context, the continentalness
+    noise and the 1.20 biome queries, against the restatement."""
+    import slime_amd
+    seed = -2 if large else 0x5EED1E55
+    rng = random.Random(7 + large)
+    n = 1200
+    fx = [rng.uniform(-2e6, 2e6) for _ in range(n)]
+    fz = [rng.uniform(-2e6, 2e6) for _ in range(n)]
+    ix = [rng.randint(-2_000_000, 2_000_000) for _ in range(n)]
+    iz = [rng.randint(-2_000_000, 2_000_000) for _ in range(n)]
+    iy = [rng.randint(-20, 90) for _ in range(n)]
+    clim = bl.ob_climate_new(seed & ((1 << 64) - 1), 1 if large else 0)
+    o5 = (C.c_double * 5)()
+    o6 = (C.c_int32 * 6)()
+    try:
+        got6 = slime_amd.climate6(seed, ix, iy, iz, large_biomes=large)
+        want6 = []
+        for x, y, z in zip(ix, iy, iz):
+            bl.ob_climate6(clim, x, y, z, o6)
+            want6.append(tuple(o6))
+        assert got6 == want6
+        if large:
+            return
+        want5 = []
+        for x, z in zip(fx, fz):
+            bl.ob_climate5(clim, x, z, o5)
+            want5.append(tuple(o5))
+        assert slime_amd.climate5(seed, fx, fz) == want5
+        assert slime_amd.climate5(seed, fx, fz, quantized=True) == \
+            [tuple(quant(v) for v in w) for w in want5]
+        wi = []
+        for x, z in zip(ix, iz):
+            bl.ob_climate5(clim, float(x), float(z), o5)
+            wi.append(tuple(quant(v) for v in o5))
+        assert slime_amd.climate5(seed, ix, iz, integer=True) == wi
+        L = slime_amd.lib()
+        h = L.mcgpu_climate_ctx_create(seed, n)
+        assert h
+        try:
+            out = (C.c_int32 * (5 * n))()
+            assert L.mcgpu_climate_ctx_query_i32(h, (C.c_int32 * n)(*ix), (C.c_int32 * n)(*iz), n,
+                                                 out) == 0
+            assert [tuple(out[5 * i:5 * i + 5]) for i in range(n)] == wi
+            assert L.mcgpu_climate_ctx_query_i32(h, (C.c_int32 * n)(*ix), (C.c_int32 * n)(*iz),
+                                                 n + 1, out) == -1
+        finally:
+            L.mcgpu_climate_ctx_destroy(h)
+        assert slime_amd.continentalness(seed, fx, fz) == \
+            [bl.ob_continentalness(clim, x, z) for x, z in zip(fx, fz)]
+        want_b = [bl.ob_biome_at(clim, x, y, z, 3) for x, y, z in zip(ix, iy, iz)]
+        assert slime_amd.biome20(seed, ix, iy, iz) == want_b
+        h = L.mcgpu_biome20_ctx_create(seed, n)
+        assert h
+        try:
+            out = (C.c_int32 * n)()
+            assert L.mcgpu_biome20_ctx_query(h, (C.c_int32 * n)(*ix), (C.c_int32 * n)(*iy),
+                                             (C.c_int32 * n)(*iz), n, out) == 0
+            assert list(out) == want_b
+        finally:
+            L.mcgpu_biome20_ctx_destroy(h)
+    finally:
+        bl.ob_climate_free(clim)
