@@ -127,8 +127,17 @@ bool is_slime_chunk_c(int64_t seed, int64_t cx, int64_t cz);
 int32_t get_cuda_device_count(void);
 int32_t get_cuda_device_name(char* buffer, int32_t buffer_size);
 void cleanup_gpu_resources(void);
-int32_t get_gpu_v1_shape(void);     /* last tile shape id, -1 before any scan */
-int32_t get_gpu_v1_algorithm(void); /* last scan kernel id, -1 before any scan */
+/* Last tile shape in the reference's numbering (1 = 128x8, 2 = 256x4,
+ * 3 = 256x8, 4 = 512x4; 0 before any scan): 2 for the plain kernel, 3 for
+ * the banded kernel. */
+int32_t get_gpu_v1_shape(void);
+/* The kernel that decided most centers of the last search, -1 before any:
+ * 16 plain per-chunk, 17 banded one x-phase, 18 banded two x-phases (the
+ * reference's ids 0-11 name its CUDA kernels). */
+int32_t get_gpu_v1_algorithm(void);
+/* The reference selects among three LCG formulations (0 native, 1 32-bit
+ * limbs, 2 truncated); every kernel here computes the same exact test, so
+ * these report the override set (0 when none) and change nothing else. */
 int32_t get_gpu_v1_rng(void);
 void set_gpu_v1_rng_override(int32_t value);
 int32_t get_gpu_v34_shape(void);

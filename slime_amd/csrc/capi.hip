@@ -26,8 +26,6 @@ static_assert(sizeof(ExtChunkResult) == sizeof(slime::ResultRow), "row layout");
 
 namespace {
 
-int32_t g_last_shape = -1;
-int32_t g_last_algorithm = -1;
 int32_t g_rng_override = -1;
 
 struct YScan {
@@ -72,8 +70,6 @@ int64_t search_common(int64_t seed, int64_t rd_max, int32_t min_size, int32_t ma
         return -1;
     control().progress.store(1, std::memory_order_relaxed);
     const slime::SearchSpec spec = square_spec(seed, rd_max, min_size, max_size, rd_min, cx, cz);
-    g_last_shape = 0;
-    g_last_algorithm = 0;
     return slime::run_search(spec, reinterpret_cast<slime::ResultRow*>(out), max_results);
 }
 
@@ -348,8 +344,15 @@ void cleanup_gpu_resources(void) {
     slime::release_engine();
 }
 
-int32_t get_gpu_v1_shape(void) { return g_last_shape; }
-int32_t get_gpu_v1_algorithm(void) { return g_last_algorithm; }
+// The reference's shape numbering (SlimeCoreGPU.cu:31: 1 = 128x8, 2 = 256x4,
+// 3 = 256x8, 4 = 512x4 threads x cells per thread; 0 before any scan): the
+// plain kernel runs 256 threads x 4 columns, the banded kernel 256 threads x
+// 8 chunks.
+int32_t get_gpu_v1_shape(void) {
+    const int k = slime::last_scan_kernel();
+    return k < 0 ? 0 : k == 16 ? 2 : 3;
+}
+int32_t get_gpu_v1_algorithm(void) { return slime::last_scan_kernel(); }
 int32_t get_gpu_v1_rng(void) { return g_rng_override >= 0 ? g_rng_override : 0; }
 void set_gpu_v1_rng_override(int32_t value) {
     g_rng_override = (value >= 0 && value <= 2) ? value : -1;

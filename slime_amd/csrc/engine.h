@@ -71,6 +71,11 @@ void kernel_stats(int kind, int64_t* launches, int64_t* kernel_ns, int64_t* cent
 // Releases every device buffer the engine holds.
 void release_engine();
 
+// The scan kernel that decided most centers of the last search: 16 the plain
+// per-chunk kernel, 17 the banded kernel with one x-phase, 18 with two; -1
+// before any search.
+int last_scan_kernel();
+
 // Ranking order of the reference: size desc, then center_x asc, center_z asc
 // compared on 64-bit block coordinates (SlimeCore.cpp:191-206).
 inline bool ranks_before(const Hit& a, const Hit& b) {

@@ -429,6 +429,10 @@ void kernel_stats(int kind, int64_t* launches, int64_t* kernel_ns, int64_t* cent
             for (int i = 0; i < 3; ++i) g_kstat[k][i].store(0);
 }
 
+std::atomic<int> g_last_kernel{-1};
+
+int last_scan_kernel() { return g_last_kernel.load(std::memory_order_relaxed); }
+
 void release_engine() {
     std::lock_guard<std::mutex> lock(g_mutex);
     free_buffers();
@@ -464,7 +468,14 @@ int64_t run_search(const SearchSpec& spec, ResultRow* out, int32_t max_results,
         plan_plain_rows(spec, spec.x0, spec.w, works);
 
     int64_t total_centers = 0, done_centers = 0;
-    for (const Work& w : works) total_centers += w.centers;
+    int64_t by_kernel[3] = {0, 0, 0};  // plain, banded one phase, banded two phases
+    for (const Work& w : works) {
+        total_centers += w.centers;
+        by_kernel[w.banded ? (w.ph == 1 ? 1 : 2) : 0] += w.centers;
+    }
+    if (!works.empty())
+        g_last_kernel.store(16 + (int)(std::max_element(by_kernel, by_kernel + 3) - by_kernel),
+                            std::memory_order_relaxed);
     bool cancelled = false;
     while (!works.empty()) {
         wait_while_paused();

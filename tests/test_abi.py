@@ -80,7 +80,10 @@ def test_control_plane(native):
     native.request_cancel()
     native.reset_cancel()
     native.set_y_scan_config(1, -64, -64, 64, 0)
-    assert native.get_gpu_v1_algorithm() in (-1, 0, 1, 2, 3)
+    native.set_y_scan_config(0, -64, -64, 64, 4)
+    # no scan has run in this process: the reference's initial values
+    assert native.get_gpu_v1_algorithm() == -1
+    assert native.get_gpu_v1_shape() == 0 and native.get_gpu_v34_shape() == 0
     native.set_gpu_v1_rng_override(1)
     assert native.get_gpu_v1_rng() == 1
     native.set_gpu_v1_rng_override(9)
