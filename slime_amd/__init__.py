@@ -305,7 +305,7 @@ def kernel_stats(kind=KERNEL_ALL, reset=False):
     return n.value, ns.value, c.value
 
 
-MODE_AUTO, MODE_PLAIN, MODE_BANDED = 0, 1, 2
+MODE_AUTO, MODE_PLAIN, MODE_BANDED, MODE_BANDED_SQUARE, MODE_BANDED_UNION = 0, 1, 2, 3, 4
 
 
 def scan_bands(seed, x0, z0, w, h, min_size, max_size=221, band_rows=0, mode=MODE_AUTO,

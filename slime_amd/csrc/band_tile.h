@@ -142,10 +142,16 @@ constexpr bool check_tables() {
 static_assert(check_tables(), "disk and union row tables");
 
 // Row r of a group's union of disks: bits [8 - h, 11 + h] of the window.
-__device__ __forceinline__ uint32_t union_row_mask(int r) {
-    const int h = union_half_width(r);
-    return ((1u << (4 + 2 * h)) - 1u) << (8 - h);
+__host__ __device__ constexpr uint32_t union_row_mask(int r) {
+    return ((1u << (4 + 2 * union_half_width(r))) - 1u) << (8 - union_half_width(r));
 }
+
+// The window bits of row r outside the union (non-zero for r <= 5, r >= 14;
+// the rolling bound trims rows 0..3 and 16..19, 60 of the 68 cells).
+__host__ __device__ constexpr uint32_t corner_mask(int r) { return ~union_row_mask(r) & 0xFFFFFu; }
+static_assert(corner_mask(6) == 0 && corner_mask(13) == 0 && corner_mask(5) != 0 &&
+                  corner_mask(14) != 0,
+              "corner rows");
 
 // A group of one band-phase, as a lane hands it to its wave:
 // (gbit << 8) | (b << 1) | ph, gbit = 4 * group.
@@ -166,16 +172,53 @@ __device__ __forceinline__ uint32_t group_row(const uint32_t* ring, int first, u
 
 // The wave refines the groups its lanes found (todo: a bit per owned
 // band-phase q, coded by code_of(q)). All 64 lanes must be active.
-//  1. Up to three groups at a time: lanes 20s..20s+19 count row r of group s
-//     under the union-of-disks mask; the five bits of each count are summed
-//     per group with ballots (a scalar popcount each).
-//  2. A group whose union count reaches min_size: lane 4c + part sums the
-//     disk rows dz = -8 + part + 4t of center c (4 lanes per center), a quad
-//     reduction gives the exact 221-cell count, and lane 4c emits it.
-template <int NB, typename CodeOf, typename EmitCenter>
+//  1. (square bound, UB false) Up to three groups at a time: lanes
+//     20s..20s+19 count row r of group s under the union-of-disks mask; the
+//     five bits of each count are summed per group with ballots (a scalar
+//     popcount each).
+//  2. A group whose union count reaches min_size (with UB, every group the
+//     lanes found): exact counts (exact_group).
+// Exact 221-cell counts of the 16 centers of one coded group by the whole
+// wave: lane 4c + part sums the disk rows dz = -8 + part + 4t of center c, a
+// quad reduction gives the count, and lane 4c emits it.
+template <int NB, typename EmitCenter>
+__device__ __forceinline__ void exact_group(uint32_t code, const uint32_t* ring, int first,
+                                            const EmitCenter& emit) {
+    const int lane = threadIdx.x & 63;
+    const int center = lane >> 2, part = lane & 3;
+    const int i = center & 3, jr = center >> 2;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int t = 0; t < 5; ++t) {
+        const int dz = -8 + part + 4 * t;
+        if (dz <= 8) {
+            const int hw = half_width_of(dz < 0 ? -dz : dz);
+            const uint32_t w = group_row<NB>(ring, first, code, 8 + jr + dz);
+            cnt += __popc(w & (((1u << (2 * hw + 1)) - 1u) << (8 + i - hw)));
+        }
+    }
+    cnt += __shfl_xor(cnt, 1);
+    cnt += __shfl_xor(cnt, 2);
+    if (part == 0) emit(code, i, jr, cnt);
+}
+
+template <int NB, bool UB, typename CodeOf, typename EmitCenter>
 __device__ __forceinline__ void refine_wave(uint32_t todo, const CodeOf& code_of,
                                             const uint32_t* ring, int first, int32_t min_size,
                                             const EmitCenter& emit) {
+    if constexpr (UB) {
+        // the lanes' union bounds already passed: exact counts, a group at a time
+        for (;;) {
+            const uint64_t act = __ballot(todo != 0);
+            if (act == 0) break;
+            const int l = __builtin_ctzll(act);
+            const uint32_t mine = todo ? code_of(__builtin_ctz(todo)) : 0u;
+            const uint32_t code = __builtin_amdgcn_readlane(mine, l);
+            if ((threadIdx.x & 63) == l) todo &= todo - 1;
+            exact_group<NB>(code, ring, first, emit);
+        }
+        return;
+    }
     const int lane = threadIdx.x & 63;
     const int slot = lane < 60 ? lane / 20 : 3;
     const int r = lane - 20 * slot;
@@ -209,22 +252,7 @@ __device__ __forceinline__ void refine_wave(uint32_t todo, const CodeOf& code_of
 #pragma unroll
             for (int j = 0; j < 5; ++j) sum += (uint32_t)__popcll(bits[j] & m) << j;
             if (sum < (uint32_t)min_size) continue;
-            // exact counts of the group's 16 centers
-            const int center = lane >> 2, part = lane & 3;
-            const int i = center & 3, jr = center >> 2;
-            uint32_t cnt = 0;
-#pragma unroll
-            for (int t = 0; t < 5; ++t) {
-                const int dz = -8 + part + 4 * t;
-                if (dz <= 8) {
-                    const int hw = half_width_of(dz < 0 ? -dz : dz);
-                    const uint32_t w = group_row<NB>(ring, first, code[s], 8 + jr + dz);
-                    cnt += __popc(w & (((1u << (2 * hw + 1)) - 1u) << (8 + i - hw)));
-                }
-            }
-            cnt += __shfl_xor(cnt, 1);
-            cnt += __shfl_xor(cnt, 2);
-            if (part == 0) emit(code[s], i, jr, cnt);
+            exact_group<NB>(code[s], ring, first, emit);
         }
     }
 }
@@ -232,7 +260,14 @@ __device__ __forceinline__ void refine_wave(uint32_t todo, const CodeOf& code_of
 // Scans tile `tile` of the job with the workgroup's LDS; scores accumulate in
 // lds.lhist. The caller synchronises the workgroup before the call when the
 // LDS was in use.
-template <int NB, int PH>
+// UB selects the bound that sends a group to refinement: false, the rolling
+// 20x20 square (then the union of disks is counted in refine_wave); true,
+// the square minus the corner cells (corner_mask) of the window's first and
+// last four rows, 340 cells that hold the union of disks. A new row's corner
+// count at offset 16 + r is taken now and at offset r (four steps later) is
+// queued a byte per step. Two masked popcounts per row and band-phase, and no
+// refinement of the many groups whose square passes at thresholds near 50.
+template <int NB, int PH, bool UB>
 __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min, int tile,
                                           TileLds<NB>& lds) {
     static_assert(PH == 1 || PH == 2, "one or two x-phases");
@@ -280,8 +315,12 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
     // Rolling state per owned band-phase q = kPhases*(b - b_lo) + (ph - ph_lo):
     // sq = square | (strip leaving next) << 16; hist = last four strips, a byte each.
     uint32_t sq[C::kOwned], hist[C::kOwned];
+    // UB: corner counts due 1..4 steps ahead, a byte each (byte 0 next step)
+    uint32_t pend[UB ? C::kOwned : 1];
 #pragma unroll
     for (int q = 0; q < C::kOwned; ++q) sq[q] = hist[q] = 0;
+#pragma unroll
+    for (int q = 0; q < (UB ? C::kOwned : 1); ++q) pend[q] = 0;
 
     const int32_t tile_s0 = p.s_begin + ts * p.tile_h;
     const int nbatch = p.tile_h / 4 + 4;
@@ -337,8 +376,13 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
         uint32_t pass = 0;  // owned band-phases whose square reaches min_size
         if (counts) {
             uint32_t c4[C::kOwned];
+            // UB: this strip's corner cells at offsets 16+r (now) and r (in
+            // four steps)
+            uint32_t k0[UB ? C::kOwned : 1], k4[UB ? C::kOwned : 1];
 #pragma unroll
             for (int q = 0; q < C::kOwned; ++q) c4[q] = 0;
+#pragma unroll
+            for (int q = 0; q < (UB ? C::kOwned : 1); ++q) k0[q] = k4[q] = 0;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const uint32_t* rowp = ring + ((4 * k + r) % kRing) * R::kRowDw;
@@ -353,7 +397,13 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
                     for (int pp = 0; pp < C::kPhases; ++pp) {
                         // phase ph of band b: where e ^ (b & 1) == ph
                         const bool use_e = ((b ^ (ph_lo + pp)) & 1) != 0;
-                        c4[C::kPhases * hb + pp] += __popc(hw & (use_e ? ew : e0));
+                        const uint32_t w = hw & (use_e ? ew : e0);
+                        const int q = C::kPhases * hb + pp;
+                        c4[q] += __popc(w);
+                        if constexpr (UB) {
+                            k0[q] += __popc(w & corner_mask(16 + r));
+                            k4[q] += __popc(w & corner_mask(r));
+                        }
                     }
                 }
             }
@@ -362,7 +412,12 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
                 const uint32_t square = (sq[q] & 0xFFFFu) + c4[q] - (sq[q] >> 16);
                 sq[q] = square | ((hist[q] >> 24) << 16);
                 hist[q] = (hist[q] << 8) | c4[q];
-                pass |= (square >= (uint32_t)p.min_size ? 1u : 0u) << q;
+                uint32_t bound = square;
+                if constexpr (UB) {
+                    bound = square - (k0[q] + (pend[q] & 0xFFu));
+                    pend[q] = (pend[q] >> 8) | (k4[q] << 24);
+                }
+                pass |= (bound >= (uint32_t)p.min_size ? 1u : 0u) << q;
             }
             if (m < 0 || col0 + gbit >= p.phase_cols || s_g >= p.s_end) pass = 0;
             // keep the band-phases with a center inside this job
@@ -399,7 +454,7 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
                 if (idx < p.hit_cap) p.hits[idx] = Hit{(int32_t)c, cx, cz, p.tag};
             }
         };
-        refine_wave<NB>(pass, code_of, ring, 4 * m, p.min_size, emit);
+        refine_wave<NB, UB>(pass, code_of, ring, 4 * m, p.min_size, emit);
     }
 }
 

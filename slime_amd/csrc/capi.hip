@@ -199,7 +199,7 @@ int64_t slime_scan_bands(int64_t seed, int32_t x0, int32_t z0, int64_t w, int64_
                          int32_t band_rows, int32_t mode, int32_t min_size, int32_t max_size,
                          ExtChunkResult* results_buffer, int32_t max_results, int64_t* hist222) {
     if (!results_buffer || max_results <= 0 || w <= 0 || h <= 0 || w > INT32_MAX ||
-        h > INT32_MAX || band_rows < 0 || mode < 0 || mode > 2 || min_size < 0 ||
+        h > INT32_MAX || band_rows < 0 || mode < 0 || mode > 4 || min_size < 0 ||
         min_size > 221 || max_size < min_size)
         return -1;
     slime::SearchSpec spec;

@@ -24,10 +24,22 @@ static_assert(sizeof(ResultRow) == 24, "ABI row is 24 bytes");
 // One search: a square of radius rd_max around (center_x, center_z), or any
 // rectangle of centers, with the reference's per-center filters.
 enum class ScanMode : int32_t {
-    kAuto = 0,    // banded scan (two x-phases above 2^17 columns, one below)
-    kPlain = 1,   // independent per-chunk scan everywhere
-    kBanded = 2,  // banded scan (same as kAuto)
+    kAuto = 0,          // banded scan (two x-phases above 2^17 columns, one below)
+    kPlain = 1,         // independent per-chunk scan everywhere
+    kBanded = 2,        // banded scan (same as kAuto)
+    kBandedSquare = 3,  // banded scan, square bound before refinement
+    kBandedUnion = 4,   // banded scan, rolling union-of-disks bound
 };
+
+// The bound the banded scan uses for a threshold when the mode does not force
+// one: the union of disks at thresholds up to kUnionBoundMaxMin, where many
+// 4x4 groups pass the square.
+constexpr int32_t kUnionBoundMaxMin = 54;
+inline bool union_bound_for(ScanMode mode, int32_t min_size) {
+    if (mode == ScanMode::kBandedSquare) return false;
+    if (mode == ScanMode::kBandedUnion) return true;
+    return min_size <= kUnionBoundMaxMin;
+}
 
 struct SearchSpec {
     int64_t seed = 0;

@@ -184,6 +184,7 @@ struct Work {
     ScanParams plain{};
     BandParams band{};
     int nb = 0, ph = 2;
+    bool union_bound = false;
     int64_t centers = 0;  // candidate centers this launch decides
 };
 
@@ -203,7 +204,7 @@ RunResult run_work(Work& wk, Ranker& rk) {
             p.hit_cap = kHitCap;
             p.emitted = g_buf.d_counters;
             p.hist = g_buf.d_counters + 1;
-            launch_err = launch_scan_band(p, wk.nb, wk.ph, g_buf.stream);
+            launch_err = launch_scan_band(p, wk.nb, wk.ph, wk.union_bound, g_buf.stream);
         } else {
             ScanParams& p = wk.plain;
             p.emit_min = rk.emit_min;
@@ -361,6 +362,7 @@ void plan_banded(const SearchSpec& spec, std::deque<Work>& out) {
         wk.banded = true;
         wk.nb = j.nb;
         wk.ph = j.ph;
+        wk.union_bound = union_bound_for(spec.mode, spec.min_size);
         wk.band = j.p;
         wk.band.tag = 0;
         wk.centers = j.centers;

@@ -79,8 +79,12 @@ struct BandParams {
     unsigned long long* emitted;
     unsigned long long* hist;
 };
-// One workgroup per tile; nb bands (1..kMaxBands), ph x-phases (1 or 2).
-hipError_t launch_scan_band(const BandParams& p, int nb, int ph, hipStream_t stream);
+// One workgroup per tile; nb bands (1..kMaxBands), ph x-phases (1 or 2);
+// union_bound selects the rolling union-of-disks bound (for nb <=
+// kUnionBoundMaxBands; the square bound otherwise).
+hipError_t launch_scan_band(const BandParams& p, int nb, int ph, bool union_bound,
+                            hipStream_t stream);
+constexpr int kUnionBoundMaxBands = 8;  // more bands spill with the union state
 
 // ---- multi-search sweep -----------------------------------------------------
 //
@@ -102,12 +106,24 @@ inline int sweep_class_of(int nb, int ph) {
         if (kSweepClass[c].nb == nb && kSweepClass[c].ph == ph) return c;
     return -1;
 }
+// Kernels: one per class with the square bound (index = class), and one per
+// class of at most kUnionBoundMaxBands bands with the rolling union bound
+// (index = kSweepClasses + class).
+constexpr int kSweepUnionClasses = 10;
+constexpr int kSweepKernels = kSweepClasses + kSweepUnionClasses;
+static_assert(kSweepClass[kSweepUnionClasses - 1].nb <= kUnionBoundMaxBands &&
+                  kSweepClass[kSweepUnionClasses].nb > kUnionBoundMaxBands,
+              "union-bound sweep kernels: the classes within the band limit");
+inline int sweep_kernel_of(int cls, bool union_bound) {
+    return union_bound && cls < kSweepUnionClasses ? kSweepClasses + cls : cls;
+}
+constexpr int sweep_class_of_kernel(int k) { return k < kSweepClasses ? k : k - kSweepClasses; }
 
 constexpr int32_t kReqActive = 1;  // request status: tiles may be claimed
 
 struct SweepJob {
     BandParams p;
-    int32_t cls;  // index into kSweepClass
+    int32_t cls;  // kernel index (sweep_kernel_of)
     int32_t req;  // request slot whose status gates the job
 };
 struct SweepArgs {
@@ -121,9 +137,9 @@ struct SweepArgs {
     uint32_t budget;        // claim attempts allowed in this launch
 };
 // Workgroups of class cls's sweep kernel that fit one CU at once.
-int sweep_blocks_per_cu(int cls);
+int sweep_blocks_per_cu(int kernel);
 // The persistent kernel of class cls over a.order (jobs of that class only).
-hipError_t launch_sweep(const SweepArgs& a, int cls, int grid, hipStream_t stream);
+hipError_t launch_sweep(const SweepArgs& a, int kernel, int grid, hipStream_t stream);
 // Sets n words to v with a kernel (usable inside a captured graph).
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t stream);
 

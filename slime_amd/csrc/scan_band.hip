@@ -5,13 +5,14 @@
 namespace slime {
 
 namespace band {
-template hipError_t launch_phases<2>(const BandParams&, int, hipStream_t);
+template hipError_t launch_phases<2>(const BandParams&, int, bool, hipStream_t);
 }  // namespace band
 
-hipError_t launch_scan_band(const BandParams& p, int nb, int ph, hipStream_t stream) {
+hipError_t launch_scan_band(const BandParams& p, int nb, int ph, bool union_bound,
+                            hipStream_t stream) {
     if (p.tiles_total <= 0) return hipSuccess;
-    if (ph == 2) return band::launch_phases<2>(p, nb, stream);
-    if (ph == 1) return band::launch_phases<1>(p, nb, stream);
+    if (ph == 2) return band::launch_phases<2>(p, nb, union_bound, stream);
+    if (ph == 1) return band::launch_phases<1>(p, nb, union_bound, stream);
     return hipErrorInvalidValue;
 }
 

@@ -4,6 +4,6 @@
 
 namespace slime {
 namespace band {
-template hipError_t launch_phases<1>(const BandParams&, int, hipStream_t);
+template hipError_t launch_phases<1>(const BandParams&, int, bool, hipStream_t);
 }  // namespace band
 }  // namespace slime

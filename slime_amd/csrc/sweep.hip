@@ -57,11 +57,16 @@ __device__ bool claim_tile(const SweepArgs& a, int& pos, int& job, int& tile) {
 // (a runtime switch over all classes in one kernel spills).
 // Waves per SIMD each class is built for: the occupancy its per-search kernel
 // reaches with the same tile code (fewer bands need fewer registers).
-constexpr int kSweepWaves[kSweepClasses] = {6, 5, 5, 5, 5, 5, 4, 4, 4, 3, 3};
+// The union-bound kernels (index kSweepClasses + class) keep their class's
+// occupancy where the extra rolling state fits.
+constexpr int kSweepWaves[kSweepKernels] = {6, 5, 5, 5, 5, 5, 4, 4, 4, 3, 3,
+                                            6, 5, 4, 4, 5, 5, 4, 4, 3, 3};
 
-template <int C>
-__global__ __launch_bounds__(kThreads, kSweepWaves[C]) void sweep_kernel(SweepArgs a) {
+template <int K>
+__global__ __launch_bounds__(kThreads, kSweepWaves[K]) void sweep_kernel(SweepArgs a) {
+    constexpr int C = sweep_class_of_kernel(K);
     constexpr int NB = kSweepClass[C].nb, PH = kSweepClass[C].ph;
+    constexpr bool UB = K >= kSweepClasses;
     __shared__ band::TileLds<NB> lds;
     __shared__ int s_job, s_tile;
     for (int i = threadIdx.x; i < kHistBins; i += kThreads) lds.lhist[i] = 0;
@@ -84,7 +89,7 @@ __global__ __launch_bounds__(kThreads, kSweepWaves[C]) void sweep_kernel(SweepAr
         if (job < 0) break;
         // read in place (scalar loads as needed) rather than copied into registers
         const BandParams& p = a.jobs[job].p;
-        band::band_tile<NB, PH>(p, p.emit_min, tile, lds);
+        band::band_tile<NB, PH, UB>(p, p.emit_min, tile, lds);
         band::flush_hist(lds.lhist, p.hist);
         if (threadIdx.x == 0) atomicAdd(&a.done[job], 1u);
         __syncthreads();
@@ -115,8 +120,8 @@ __global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v) {
 
 }  // namespace
 
-int sweep_blocks_per_cu(int cls) {
-    switch (cls) {
+int sweep_blocks_per_cu(int kernel) {
+    switch (kernel) {
         case 0: return blocks_per_cu<0>();
         case 1: return blocks_per_cu<1>();
         case 2: return blocks_per_cu<2>();
@@ -128,13 +133,23 @@ int sweep_blocks_per_cu(int cls) {
         case 8: return blocks_per_cu<8>();
         case 9: return blocks_per_cu<9>();
         case 10: return blocks_per_cu<10>();
+        case 11: return blocks_per_cu<11>();
+        case 12: return blocks_per_cu<12>();
+        case 13: return blocks_per_cu<13>();
+        case 14: return blocks_per_cu<14>();
+        case 15: return blocks_per_cu<15>();
+        case 16: return blocks_per_cu<16>();
+        case 17: return blocks_per_cu<17>();
+        case 18: return blocks_per_cu<18>();
+        case 19: return blocks_per_cu<19>();
+        case 20: return blocks_per_cu<20>();
         default: return 1;
     }
 }
 
-hipError_t launch_sweep(const SweepArgs& a, int cls, int grid, hipStream_t stream) {
+hipError_t launch_sweep(const SweepArgs& a, int kernel, int grid, hipStream_t stream) {
     if (a.norder <= 0 || grid <= 0) return hipSuccess;
-    switch (cls) {
+    switch (kernel) {
         case 0: return launch_class<0>(a, grid, stream);
         case 1: return launch_class<1>(a, grid, stream);
         case 2: return launch_class<2>(a, grid, stream);
@@ -146,10 +161,20 @@ hipError_t launch_sweep(const SweepArgs& a, int cls, int grid, hipStream_t strea
         case 8: return launch_class<8>(a, grid, stream);
         case 9: return launch_class<9>(a, grid, stream);
         case 10: return launch_class<10>(a, grid, stream);
+        case 11: return launch_class<11>(a, grid, stream);
+        case 12: return launch_class<12>(a, grid, stream);
+        case 13: return launch_class<13>(a, grid, stream);
+        case 14: return launch_class<14>(a, grid, stream);
+        case 15: return launch_class<15>(a, grid, stream);
+        case 16: return launch_class<16>(a, grid, stream);
+        case 17: return launch_class<17>(a, grid, stream);
+        case 18: return launch_class<18>(a, grid, stream);
+        case 19: return launch_class<19>(a, grid, stream);
+        case 20: return launch_class<20>(a, grid, stream);
         default: return hipErrorInvalidValue;
     }
 }
-static_assert(kSweepClasses == 11, "one case per class");
+static_assert(kSweepKernels == 21, "one case per kernel");
 
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t stream) {
     if (n == 0) return hipSuccess;

@@ -33,7 +33,7 @@ struct SweepRequest {
     int32_t keep = 0;  // max_results
     // plan
     std::vector<BandJob> jobs;
-    std::vector<int32_t> job_cls;
+    std::vector<int32_t> job_cls;  // kernel index of each job (sweep_kernel_of)
     int64_t centers = 0;
     double est_ns = 0;   // modelled device time of the whole request
     int64_t hit_cap = 0;
@@ -98,7 +98,7 @@ class Sweeper {
 
     int device_ = -1;
     int cus_ = 256;
-    int grid_[kSweepClasses] = {};
+    int grid_[kSweepKernels] = {};
     hipStream_t stream_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     int32_t req_cap_ = 0;

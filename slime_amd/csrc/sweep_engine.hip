@@ -164,7 +164,7 @@ bool Sweeper::init(int32_t req_cap, int64_t job_cap, int64_t hit_cap) {
         hipMalloc(&d_hist_, sizeof(unsigned long long) * kHistBins * req_cap) == hipSuccess &&
         hipMalloc(&d_emitted_, sizeof(unsigned long long) * req_cap) == hipSuccess &&
         hipMalloc(&d_hits_, sizeof(Hit) * hit_cap) == hipSuccess &&
-        hipMalloc(&d_claims_, sizeof(uint32_t) * kSweepClasses) == hipSuccess &&
+        hipMalloc(&d_claims_, sizeof(uint32_t) * kSweepKernels) == hipSuccess &&
         hipMalloc(&d_table_, sizeof(uint64_t) * table.size()) == hipSuccess &&
         hipMemcpy(d_table_, table.data(), sizeof(uint64_t) * table.size(),
                   hipMemcpyHostToDevice) == hipSuccess &&
@@ -183,7 +183,7 @@ bool Sweeper::init(int32_t req_cap, int64_t job_cap, int64_t hit_cap) {
     hits_alloc_.reset(hit_cap);
     free_slots_.clear();
     for (int32_t s = req_cap - 1; s >= 0; --s) free_slots_.push_back(s);
-    for (int c = 0; c < kSweepClasses; ++c) grid_[c] = cus_ * sweep_blocks_per_cu(c);
+    for (int k = 0; k < kSweepKernels; ++k) grid_[k] = cus_ * sweep_blocks_per_cu(k);
     return true;
 }
 
@@ -241,7 +241,7 @@ bool Sweeper::calibrate() {
                     hipSuccess ||
                 hipMemsetAsync(d_cursor_, 0, sizeof(uint32_t), stream_) != hipSuccess ||
                 hipMemsetAsync(d_done_, 0, sizeof(uint32_t), stream_) != hipSuccess ||
-                hipMemsetAsync(d_claims_, 0, sizeof(uint32_t) * kSweepClasses, stream_) !=
+                hipMemsetAsync(d_claims_, 0, sizeof(uint32_t) * kSweepKernels, stream_) !=
                     hipSuccess ||
                 hipMemcpyAsync(d_order_, &order0, sizeof(int32_t), hipMemcpyHostToDevice,
                                stream_) != hipSuccess ||
@@ -275,7 +275,10 @@ bool Sweeper::calibrate() {
     return true;
 }
 
-double Sweeper::tile_ns(int cls, int32_t tile_h) const {
+// Costs are measured per class (square-bound kernels); a union-bound kernel
+// is costed as its class.
+double Sweeper::tile_ns(int kernel, int32_t tile_h) const {
+    const int cls = sweep_class_of_kernel(kernel);
     return costs_.ns_row[cls] * ((double)tile_h + kTileHalo + costs_.extra_rows[cls]);
 }
 
@@ -376,10 +379,11 @@ bool Sweeper::plan(SweepRequest& r) const {
                            int64_t(1) << 62, r.jobs);
     r.job_cls.clear();
     r.centers = 0;
+    const bool ub = union_bound_for(spec.mode, spec.min_size);
     for (const BandJob& j : r.jobs) {
         const int c = sweep_class_of(j.nb, j.ph);
         if (c < 0) return false;
-        r.job_cls.push_back(c);
+        r.job_cls.push_back(sweep_kernel_of(c, ub));
         r.centers += j.centers;
     }
     r.est_ns = best;
@@ -556,7 +560,7 @@ bool Sweeper::round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& p
         int64_t left;
         int32_t tile_h;
     };
-    std::vector<Item> items[kSweepClasses];
+    std::vector<Item> items[kSweepKernels];
     int64_t seq = 0;
     for (const SweepRequest* r : reqs) {
         if (r->status != kReqActive || r->complete) continue;
@@ -579,7 +583,7 @@ bool Sweeper::round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& p
         double min_key;
     };
     std::vector<Launch> launches;
-    for (int c = 0; c < kSweepClasses; ++c) {
+    for (int c = 0; c < kSweepKernels; ++c) {
         std::vector<Item>& v = items[c];
         if (v.empty()) continue;
         std::sort(v.begin(), v.end(), [](const Item& a, const Item& b) {
@@ -616,7 +620,7 @@ bool Sweeper::round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& p
         return false;
 
     auto enqueue = [&]() -> bool {
-        if (launch_fill_u32(d_claims_, kSweepClasses, 0u, stream_) != hipSuccess) return false;
+        if (launch_fill_u32(d_claims_, kSweepKernels, 0u, stream_) != hipSuccess) return false;
         for (const Launch& l : launches) {
             SweepArgs a{d_jobs_, d_order_ + l.off, l.n, d_cursor_, d_done_, d_status_,
                         d_claims_ + l.cls, l.budget};

@@ -171,3 +171,45 @@ def test_full_width_search_hits_recount(gpu, oracle):
     p = slime_amd.scan_bands(seed, 1000 - rd, -2000 - rd, 2 * rd + 1, 2 * rd + 1, thr, 221, 0,
                              slime_amd.MODE_PLAIN, 100)
     assert p[0] == total and p[1] == rows
+
+
+@pytest.mark.parametrize("radius,thr", [(3000, 38), (70000, 50), (140000, 52), (200000, 55),
+                                        (150000, 60)])
+def test_union_and_square_bounds_agree(gpu, radius, thr):
+    """The rolling union-of-disks bound and the square bound send different
+    groups to refinement but must give identical results (and equal the plain
+    kernel on the smaller squares)."""
+    import slime_amd
+    seed = s64(0x0BADC0DE * (radius + thr))
+    x0 = z0 = -radius
+    w = h = 2 * radius + 1
+    print("bounds", radius, thr, flush=True)
+    u = slime_amd.scan_bands(seed, x0, z0, w, h, thr, 221, 0, slime_amd.MODE_BANDED_UNION, 3000)
+    s = slime_amd.scan_bands(seed, x0, z0, w, h, thr, 221, 0, slime_amd.MODE_BANDED_SQUARE, 3000)
+    assert u[0] == s[0] and u[1] == s[1] and u[2] == s[2]
+    assert u[0] > 0 or thr >= 56  # high thresholds: hits are rare at these sizes
+    if radius <= 70000:
+        p = slime_amd.scan_bands(seed, x0, z0, w, h, thr, 221, 0, slime_amd.MODE_PLAIN, 3000)
+        assert u[0] == p[0] and u[1] == p[1] and u[2] == p[2]
+
+
+@pytest.mark.parametrize("nb", [1, 2, 4, 8, 9])
+def test_union_bound_band_sets_vs_oracle(gpu, oracle, nb):
+    """Strided band sets through the union-bound kernel (9 bands: past the
+    union's band limit, the square bound runs) against the oracle."""
+    import slime_amd
+    rng = random.Random(500 + nb)
+    seed = s64(rng.getrandbits(64))
+    band_rows = 16
+    w = 2 * P + 333
+    x0 = rng.randint(-10 ** 8, 10 ** 8)
+    z0 = rng.randint(-10 ** 8, 10 ** 8)
+    h = (nb - 1) * P + band_rows
+    thr, k = 35, 400
+    print("union bands", nb, seed, x0, z0, flush=True)
+    total, rows, hist = slime_amd.scan_bands(seed, x0, z0, w, h, thr, 221, band_rows,
+                                             slime_amd.MODE_BANDED_UNION, k)
+    want_total, want_hist, want_rows = oracle_bands(oracle, seed, x0, z0, w, h, band_rows, thr, k)
+    assert total == want_total
+    assert hist == want_hist
+    assert rows == want_rows
