@@ -19,12 +19,14 @@ class ChunkRow(C.Structure):
                 ("obs_count", C.c_int32), ("afk_x", C.c_int32), ("afk_z", C.c_int32)]
 
 
-def build(ref=False):
-    """Compile the C restatement (and the reference, when its tree exists)."""
+def build(ref=False, force=False):
+    """Compile the C restatement (and the reference, when its tree exists);
+    force recompiles what is up to date."""
     targets = ["all"]
     if ref and os.path.isdir(os.environ.get("REF_DIR", "/root/reference")):
         targets.append("ref")
-    subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
+    subprocess.run(["make", "-s"] + (["-B"] if force else []) + ["-C", HERE] + targets,
+                   check=True)
 
 
 _oracle = None
