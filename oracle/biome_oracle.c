@@ -364,11 +364,50 @@ int ob_tree_leaf(const int32_t* np, int tree) {
     return nearest(np, tree, 0, 0, 0, &ds);
 }
 
-/* The search started from a known leaf with the bound distance(hint) + 1
- * (what the kernels do from one y layer to the next). */
+/* The search bounded by a known leaf (what the kernels do from one y layer
+ * to the next): no leaf kept at the start, the bound distance(hint) + 1, and
+ * a child's leaf kept only when it differs from the current one (a strictly
+ * nearer leaf was found below it). */
+static int bounded(const int32_t* np, int tree, int depth, int idx, int best, uint64_t* best_ds) {
+    const ob_tree* t = &kObTrees[tree];
+    const int depths = t->family ? 5 : 4, order = t->family ? 6 : 10;
+    if (depth >= depths) return idx;
+    int step = t->family ? kSteps1[depth] : kSteps0[depth];
+    if (idx + step >= t->len) return bounded(np, tree, depth + 1, idx, best, best_ds);
+    int child = (int)((t->nodes[idx] >> 48) & 0xFFFF);
+    for (int k = 0; k < order && child < t->len; ++k, child += step) {
+        uint64_t dc = ob_node_distance(np, child, tree);
+        if (dc < *best_ds) {
+            uint64_t sub = *best_ds;
+            int leaf = bounded(np, tree, depth + 1, child, best, &sub);
+            if (leaf != best) {
+                *best_ds = leaf == child ? dc : ob_node_distance(np, leaf, tree);
+                best = leaf;
+            }
+        }
+    }
+    return best;
+}
+
 int ob_tree_leaf_from(const int32_t* np, int hint, int tree) {
     uint64_t ds = ob_node_distance(np, hint, tree) + 1;
-    return nearest(np, tree, 0, 0, hint, &ds);
+    return bounded(np, tree, 0, 0, -1, &ds);
+}
+
+/* The first descent of the reference's search (first child at every level)
+ * ends on a leaf: returns that node, so a test can check it is a leaf. */
+int ob_tree_first_descent(int tree) {
+    const ob_tree* t = &kObTrees[tree];
+    const int depths = t->family ? 5 : 4;
+    int idx = 0;
+    for (int depth = 0; depth < depths; ++depth) {
+        int step = t->family ? kSteps1[depth] : kSteps0[depth];
+        if (idx + step >= t->len) continue;
+        int child = (int)((t->nodes[idx] >> 48) & 0xFFFF);
+        if (child >= t->len) return -1;
+        idx = child;
+    }
+    return idx;
 }
 
 int ob_tree_leaf_exhaustive(const int32_t* np, int tree) {

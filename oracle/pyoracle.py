@@ -245,6 +245,8 @@ def biome_lib():
         L.ob_tree_leaf.restype = ci
         L.ob_tree_leaf_from.argtypes = [P32, ci, ci]
         L.ob_tree_leaf_from.restype = ci
+        L.ob_tree_first_descent.argtypes = [ci]
+        L.ob_tree_first_descent.restype = ci
         L.ob_tree_leaf_exhaustive.argtypes = [P32, ci]
         L.ob_tree_leaf_exhaustive.restype = ci
         L.ob_node_distance.argtypes = [P32, ci, ci]

@@ -404,27 +404,46 @@ SLIME_HD uint64_t node_distance(const int32_t np[6], const uint64_t* nodes, cons
 }
 
 // Nearest leaf below idx (the search of MinecraftGPUNoise.cu:680-790): visit
-// the children whose box is closer than the best leaf so far, in order, and
-// keep a leaf only if strictly closer. Returns the leaf index.
-template <int F, int D>
+// the children whose box is nearer than best_ds, in order, and keep a leaf
+// only if strictly nearer. `best` is the leaf kept so far or -1: a child's
+// search returns a different leaf only when it kept a strictly nearer one, so
+// the result is the first leaf in visiting order at the least distance below
+// the bound (boxes never lie farther than the leaves below them). The
+// reference starts from node 0 with an unbounded distance and takes a
+// returned leaf when its distance is below the bound; that differs from this
+// only if its first descent (always the first child of every node) ended
+// without a leaf, which no tree has (checked by the tests). `stop(leaf)`
+// ends the search early once it returns true for a kept leaf (then `stopped`
+// is set and that leaf returned).
+struct NoStop {
+    SLIME_HD bool operator()(int) const { return false; }
+};
+
+template <int F, int D, typename Stop = NoStop>
 SLIME_HD int tree_nearest(const int32_t np[6], const uint64_t* nodes, const int32_t* params,
-                          int len, int idx, int best, uint64_t best_ds) {
+                          int len, int idx, int best, uint64_t best_ds, const Stop& stop,
+                          bool& stopped) {
     if constexpr (D >= TreeShape<F>::depth) {
         return idx;
     } else {
         constexpr int step = TreeShape<F>::step(D);
         if (idx + step >= len)
-            return tree_nearest<F, D + 1>(np, nodes, params, len, idx, best, best_ds);
+            return tree_nearest<F, D + 1>(np, nodes, params, len, idx, best, best_ds, stop,
+                                          stopped);
         int child = (int)((nodes[idx] >> 48) & 0xFFFFu);
         for (int k = 0; k < TreeShape<F>::order && child < len; ++k, child += step) {
             const uint64_t dc = node_distance(np, nodes, params, child);
             if (dc < best_ds) {
-                const int leaf =
-                    tree_nearest<F, D + 1>(np, nodes, params, len, child, best, best_ds);
-                const uint64_t dl = leaf == child ? dc : node_distance(np, nodes, params, leaf);
-                if (dl < best_ds) {
-                    best_ds = dl;
+                const int leaf = tree_nearest<F, D + 1>(np, nodes, params, len, child, best,
+                                                        best_ds, stop, stopped);
+                if (stopped) return leaf;
+                if (leaf != best) {  // a strictly nearer leaf below the child
+                    best_ds = leaf == child ? dc : node_distance(np, nodes, params, leaf);
                     best = leaf;
+                    if (stop(leaf)) {
+                        stopped = true;
+                        return leaf;
+                    }
                 }
             }
         }
@@ -433,17 +452,15 @@ SLIME_HD int tree_nearest(const int32_t np[6], const uint64_t* nodes, const int3
 }
 
 // The leaf the search from the root finds: the first leaf in visiting order
-// at the least distance (a box never lies farther than the leaves below it,
-// so that leaf's branch is always entered). Starting from a known leaf `hint`
-// with the bound distance(hint) + 1 prunes more and returns the same leaf:
-// the bound still exceeds the least distance, and only strictly closer leaves
-// are taken. Returns the leaf; hint < 0 starts without one.
+// at the least distance. A known leaf `hint` (hint >= 0) only bounds the
+// search (distance(hint) + 1, above the least distance); it is not taken until
+// the search reaches it, so the result is the same leaf. Returns the leaf.
 template <int F>
 SLIME_HD int tree_leaf(const int32_t np[6], const uint64_t* nodes, const int32_t* params, int len,
                        int hint) {
-    if (hint < 0) return tree_nearest<F, 0>(np, nodes, params, len, 0, 0, ~0ull);
-    const uint64_t bound = node_distance(np, nodes, params, hint) + 1;
-    return tree_nearest<F, 0>(np, nodes, params, len, 0, hint, bound);
+    bool stopped = false;
+    const uint64_t bound = hint < 0 ? ~0ull : node_distance(np, nodes, params, hint) + 1;
+    return tree_nearest<F, 0>(np, nodes, params, len, 0, -1, bound, NoStop{}, stopped);
 }
 
 SLIME_HD int leaf_biome(const uint64_t* nodes, int leaf) {

@@ -87,30 +87,51 @@ bool find_targets(const BiomeTreeDef& tree, TargetLeaves& t) {
     return true;
 }
 
-// Whether the search at np can return one of `leaves`: not if every one of
-// them lies strictly farther than the known leaf `hint` (the nearest leaf is
-// no farther than hint). A biome with no leaf in the tree is never found.
-__device__ __forceinline__ bool may_be(const int32_t np[6], const uint64_t* nodes,
-                                       const int32_t* params, int hint, const int32_t* leaves,
-                                       int nleaves) {
-    if (nleaves == 0) return false;
-    if (hint < 0) return true;
-    const uint64_t known = node_distance(np, nodes, params, hint);
-    for (int i = 0; i < nleaves; ++i)
-        if (node_distance(np, nodes, params, leaves[i]) <= known) return true;
-    return false;
-}
+// Stops a search at the first kept leaf that is not one of the targets (the
+// leaves held by value: constant indices keep them in registers).
+struct NotTarget {
+    int32_t leaf[kMaxTargetLeaves];
+    int n;
+    __device__ bool operator()(int l) const {
+        bool hit = false;
+#pragma unroll
+        for (int i = 0; i < kMaxTargetLeaves; ++i) hit |= i < n && leaf[i] == l;
+        return !hit;
+    }
+};
 
-// Whether layer yi of a column is the biome of `leaves`, searching only when
-// a target leaf could be the nearest; hint carries the last leaf found.
+// Whether the search from the root at layer yi of a column returns one of
+// `leaves` (the biome's leaves). Not if every one of them lies strictly
+// farther than `hint`, a leaf found before (the nearest leaf is no farther).
+// Otherwise the search runs with the bound (nearest target's distance + 1)
+// and stops at the first kept leaf that is not a target: that leaf is no
+// farther than every target and comes first in visiting order among the
+// leaves that near, so the result cannot be a target. If the search ends
+// without stopping, every leaf it kept was a target. `hint` gets the last
+// leaf kept (a bound for the next layers).
 template <int F>
 __device__ __forceinline__ bool layer_is(const Column& col, int32_t yi, const uint64_t* nodes,
                                         const int32_t* params, int32_t len, int& hint,
-                                        const int32_t* leaves, int nleaves, int biome) {
+                                        const int32_t* leaves, int nleaves) {
+    if (nleaves == 0) return false;
     const int32_t np[6] = {col.t, col.h, col.c, col.e, depth_at(col.offset, yi), col.w};
-    if (!may_be(np, nodes, params, hint, leaves, nleaves)) return false;
-    hint = tree_leaf<F>(np, nodes, params, len, hint);
-    return leaf_biome(nodes, hint) == biome;
+    NotTarget other;
+    other.n = nleaves;
+    uint64_t target = ~0ull;
+#pragma unroll
+    for (int i = 0; i < kMaxTargetLeaves; ++i) {
+        other.leaf[i] = leaves[i];
+        if (i < nleaves) {
+            const uint64_t d = node_distance(np, nodes, params, leaves[i]);
+            target = d < target ? d : target;
+        }
+    }
+    if (hint >= 0 && target > node_distance(np, nodes, params, hint)) return false;
+    bool stopped = false;
+    const int leaf =
+        tree_nearest<F, 0>(np, nodes, params, len, 0, -1, target + 1, other, stopped);
+    if (leaf >= 0) hint = leaf;
+    return !stopped && leaf >= 0;
 }
 
 // The mask of one candidate, by the whole workgroup (every thread calls it;
@@ -148,13 +169,11 @@ __device__ __forceinline__ unsigned int filter_candidate(const Climate& st, Filt
         const Column col = column_at(st, qx, qz);
         unsigned int found = 0;
         // hint: the thread's last leaf, a bound for every later layer and column
-        if ((want & 2u) && layer_is<F>(col, 16, nodes, params, len, hint, tl.mush, tl.nmush,
-                                       kMushroomFields))
+        if ((want & 2u) && layer_is<F>(col, 16, nodes, params, len, hint, tl.mush, tl.nmush))
             found |= 2u;
         if (want & 1u) {
             for (int32_t y = 0; y >= -16; --y) {
-                if (layer_is<F>(col, y, nodes, params, len, hint, tl.deep, tl.ndeep,
-                                kDeepDark)) {
+                if (layer_is<F>(col, y, nodes, params, len, hint, tl.deep, tl.ndeep)) {
                     found |= 1u;
                     break;
                 }

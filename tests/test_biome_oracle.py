@@ -308,3 +308,36 @@ def test_search_from_any_leaf_equals_search_from_root(bl, tree):
                 hint = cold
     finally:
         bl.ob_climate_free(c)
+
+
+@pytest.mark.parametrize("tree", TREE_IDS)
+def test_bounded_search_on_distance_ties(bl, tree):
+    """Points with several leaves at the least distance: the search bounded by
+    any of them (the kernels' start from the last leaf) returns the root
+    search's leaf, the first of them in visiting order."""
+    rng = random.Random(77 + tree)
+    n = bl.ob_tree_len(tree)
+    leaves = [i for i in range(n) if (bl.ob_tree_node(tree, i) >> 56) == 0xFF]
+    ties = 0
+    for it in range(400):
+        np6 = [rng.randint(-12000, 12000) for _ in range(6)]
+        if it % 2:
+            np6[4] = rng.randint(-2000, 12000)
+        a = (C.c_int32 * 6)(*np6)
+        cold = bl.ob_tree_leaf(a, tree)
+        d = bl.ob_node_distance(a, cold, tree)
+        mins = [l for l in leaves if bl.ob_node_distance(a, l, tree) == d]
+        if len(mins) > 1:
+            ties += 1
+        for h in mins:
+            assert bl.ob_tree_leaf_from(a, h, tree) == cold, (np6, h)
+    assert ties >= 1
+
+
+@pytest.mark.parametrize("tree", TREE_IDS)
+def test_first_descent_reaches_a_leaf(bl, tree):
+    """The reference's search takes the first child at every level before it
+    has found a leaf; that path ends on a leaf in every tree (so starting with
+    no leaf instead of node 0 changes nothing)."""
+    leaf = bl.ob_tree_first_descent(tree)
+    assert leaf >= 0 and (bl.ob_tree_node(tree, leaf) >> 56) == 0xFF

@@ -259,3 +259,23 @@ def test_climate_point_queries(gpu, bl, large):
             L.mcgpu_biome20_ctx_destroy(h)
     finally:
         bl.ob_climate_free(clim)
+
+
+@pytest.mark.parametrize("version,tree", [(2602, 0), (1206, 3), (1192, 5)])
+def test_filter_many_candidates(gpu, bl, version, tree):
+    """A few hundred candidates per tree: masks equal the oracle's searches
+    from the root (the kernel bounds each search by leaves it found before)."""
+    import slime_amd
+    seed = 0x5EED0000 + version
+    rng = random.Random(3 * version)
+    cands = [(rng.randint(-600_000, 600_000), rng.randint(-600_000, 600_000)) for _ in range(250)]
+    clim = bl.ob_climate_new(seed, 0)
+    try:
+        with slime_amd.BiomeContext(seed, capacity=len(cands), version=version) as ctx:
+            got = ctx.filter([c[0] for c in cands], [c[1] for c in cands], 3)
+        want = [bl.ob_filter(clim, seed, x, z, 3, tree) for x, z in cands]
+        assert got == want
+        print("version", version, "deep dark", sum(m & 1 for m in got), "mushroom",
+              sum(m >> 1 & 1 for m in got), flush=True)
+    finally:
+        bl.ob_climate_free(clim)
