@@ -161,7 +161,9 @@ __device__ __forceinline__ unsigned int filter_candidate(const Climate& st, Filt
     const unsigned int want = (unsigned int)flags;
     const int32_t* params = f.params;
     for (int item = threadIdx.x; item < columns; item += blockDim.x) {
-        if ((__atomic_load_n(&f.mask, __ATOMIC_RELAXED) & want) == want) break;
+        // bits another column already found need no more checks
+        const unsigned int need = want & ~__atomic_load_n(&f.mask, __ATOMIC_RELAXED);
+        if (need == 0) break;
         const int ci = f.chunks[item >> 4], q = item & 15;
         const int dx = ci / 17 - 8, dz = ci % 17 - 8;
         const int32_t qx = (int32_t)(((uint32_t)cx + (uint32_t)dx) * 4u + (uint32_t)(q >> 2));
@@ -169,9 +171,9 @@ __device__ __forceinline__ unsigned int filter_candidate(const Climate& st, Filt
         const Column col = column_at(st, qx, qz);
         unsigned int found = 0;
         // hint: the thread's last leaf, a bound for every later layer and column
-        if ((want & 2u) && layer_is<F>(col, 16, nodes, params, len, hint, tl.mush, tl.nmush))
+        if ((need & 2u) && layer_is<F>(col, 16, nodes, params, len, hint, tl.mush, tl.nmush))
             found |= 2u;
-        if (want & 1u) {
+        if (need & 1u) {
             for (int32_t y = 0; y >= -16; --y) {
                 if (layer_is<F>(col, y, nodes, params, len, hint, tl.deep, tl.ndeep)) {
                     found |= 1u;
