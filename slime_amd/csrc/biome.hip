@@ -174,7 +174,7 @@ __device__ __forceinline__ unsigned int filter_candidate(const Climate& st, Filt
         if ((need & 2u) && layer_is<F>(col, 16, nodes, params, len, hint, tl.mush, tl.nmush))
             found |= 2u;
         if (need & 1u) {
-            for (int32_t y = 0; y >= -16; --y) {
+            for (int32_t y = -16; y <= 0; ++y) {  // deepest first: Deep Dark lies deep
                 if (layer_is<F>(col, y, nodes, params, len, hint, tl.deep, tl.ndeep)) {
                     found |= 1u;
                     break;
