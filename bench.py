@@ -164,6 +164,7 @@ def batched_leg(repeats):
 
 FP64_PEAK_TF = 78.6          # MI355X dense FP64 vector peak
 FLOPS_PER_COLUMN = 52 * 60   # 52 double Perlin octave samples of ~60 FP64 ops
+EXTENDED_LEVELS = (-64, 320, 4)  # block heights of the wider biome sweep
 
 
 def biome_leg(searches, res):
@@ -193,6 +194,17 @@ def biome_leg(searches, res):
     columns = 16 * sum(sizes)  # quart columns under the sites' slime chunks (before early exits)
     achieved = columns * FLOPS_PER_COLUMN / (ms / 1e3) / 1e12
     print(f"biome filter: {n} sites in {ms:.1f} ms", file=sys.stderr, flush=True)
+    # the wider sweep: both biomes at every height -64..320 step 4, over the
+    # disk's slime chunks and over every chunk of the disk
+    extended = {"heights": "-64..320 step 4 (quart y -16..80), both biomes at every height"}
+    for name, flags in (("slime_chunks", 3), ("all_chunks", 3 | slime_amd.ALL_CHUNKS)):
+        m_ext, ms_ext = slime_amd.filter_multi(seeds, cands, flags, levels=EXTENDED_LEVELS)
+        fl = [m for ws in m_ext for m in ws]
+        extended[name] = {"kernel_ms": ms_ext, "sites_per_s": n / (ms_ext / 1e3),
+                          "deep_dark": sum(1 for m in fl if m & 1),
+                          "mushroom_fields": sum(1 for m in fl if m & 2)}
+        print(f"biome filter, heights -64..320, {name}: {ms_ext:.1f} ms", file=sys.stderr,
+              flush=True)
     return {
         "workload": "top sites (<= 1000) of each batched search, flags Deep Dark | Mushroom Fields, "
                     "version 26.2",
@@ -201,6 +213,7 @@ def biome_leg(searches, res):
         "one_launch": {"kernel_ms": ms, "wall_s": wall, "sites_per_s": n / (ms / 1e3)},
         "per_world_contexts": {"kernel_ms": per_ms, "wall_s": per_wall},
         "identical": per == masks,
+        "extended": extended,
         "roofline": {"bound": "fp64", "unit": "TFLOP/s", "peak": FP64_PEAK_TF,
                      "achieved": achieved, "frac": achieved / FP64_PEAK_TF,
                      "flops_per_column": FLOPS_PER_COLUMN, "columns": columns,

@@ -41,6 +41,21 @@ int mcgpu_modern_ctx_filter_candidates(void* ctx, const int32_t* center_chunk_x,
                                        const int32_t* center_chunk_z, int32_t count,
                                        int32_t flags, int32_t* mask_out);
 
+/* Extension with no counterpart in the reference: the filter over a range of
+ * heights, for the wider sweep (y -64..320 step 4) next to the reference's
+ * fixed set. Each biome requested in flags (bit 0 Deep Dark, bit 1 Mushroom
+ * Fields) is looked for at block heights y_min, y_min + y_step, ... <= y_max,
+ * sampled at quart y floor(y / 4), in the 16 quart columns of each sampled
+ * chunk; flags bit 2 samples every chunk of the 221-chunk disk instead of its
+ * slime chunks. MCGPU_ERR_ARG unless y_step >= 1, y_min <= y_max, both within
+ * +-2^24 and at most 4096 heights. With heights -64..0 step 4 (flags 1) or 64
+ * alone (flags 2) it gives the masks of mcgpu_modern_ctx_filter_candidates. */
+enum { MCGPU_FILTER_ALL_CHUNKS = 4 };
+int mcgpu_modern_ctx_filter_candidates_levels(void* ctx, const int32_t* center_chunk_x,
+                                              const int32_t* center_chunk_z, int32_t count,
+                                              int32_t flags, int32_t y_min, int32_t y_max,
+                                              int32_t y_step, int32_t* mask_out);
+
 /* MinecraftGPUNoise.cu:1882-1908: biome ids at quart coordinates. */
 int mcgpu_modern_ctx_query(void* ctx, const int32_t* xs, const int32_t* ys, const int32_t* zs,
                            int32_t count, int32_t* biome_out);
@@ -69,8 +84,15 @@ int mcgpu_filter_multi(int32_t nworlds, int32_t version_code, const uint64_t* wo
                        const int32_t* first, const int32_t* center_chunk_x,
                        const int32_t* center_chunk_z, int32_t flags, int32_t large_biomes,
                        int32_t* mask_out);
+/* Extension with no counterpart in the reference: mcgpu_filter_multi over a
+ * range of heights, flags and heights as for
+ * mcgpu_modern_ctx_filter_candidates_levels. */
+int mcgpu_filter_multi_levels(int32_t nworlds, int32_t version_code, const uint64_t* world_seeds,
+                              const int32_t* first, const int32_t* center_chunk_x,
+                              const int32_t* center_chunk_z, int32_t flags, int32_t large_biomes,
+                              int32_t y_min, int32_t y_max, int32_t y_step, int32_t* mask_out);
 /* Extension with no counterpart in the reference: device time (ms, HIP events)
- * of the last mcgpu_filter_multi launch. */
+ * of the last mcgpu_filter_multi or mcgpu_filter_multi_levels launch. */
 float mcgpu_filter_multi_last_ms(void);
 
 /* ---- primitives and point queries (MinecraftGPUNoise.cu:1107-1774, 1988-2018)

@@ -546,3 +546,35 @@ int ob_filter(const ob_climate* c, int64_t world_seed, int32_t cx, int32_t cz, i
     }
     return mask;
 }
+
+/* The filter over a level range, an extension of the reference's sampling
+ * set (the wider sweep of y -64..320 step 4): flags bit 0 Deep Dark, bit 1
+ * Mushroom Fields, bit 2 every chunk of the disk instead of its slime chunks.
+ * Each requested biome is looked for at block heights y_min, y_min + y_step,
+ * ... <= y_max, sampled at quart y floor(y / 4), in the 16 quart columns of
+ * each sampled chunk. With levels -64..0 (flags 1) or 64 alone (flags 2) over
+ * slime chunks it is the reference's filter. */
+static int32_t floor_quart(int32_t y) { return y >= 0 ? y / 4 : -((3 - y) / 4); }
+
+int ob_filter_levels(const ob_climate* c, int64_t world_seed, int32_t cx, int32_t cz, int flags,
+                     int32_t y_min, int32_t y_max, int32_t y_step, int tree) {
+    int want = flags & 3, mask = 0;
+    for (int ci = 0; ci < 17 * 17; ++ci) {
+        int dx = ci / 17 - 8, dz = ci % 17 - 8;
+        if (dx * dx + dz * dz > 68) continue;
+        int32_t x = (int32_t)((uint32_t)cx + (uint32_t)dx), z = (int32_t)((uint32_t)cz + (uint32_t)dz);
+        if (!(flags & 4) && !slime(world_seed, x, z)) continue;
+        for (int q = 0; q < 16; ++q) {
+            if ((mask & want) == want) return mask;
+            int32_t col[6];
+            ob_column(c, (int32_t)((uint32_t)x * 4u + (uint32_t)(q / 4)),
+                      (int32_t)((uint32_t)z * 4u + (uint32_t)(q % 4)), col);
+            for (int64_t y = y_min; y <= y_max; y += y_step) {
+                int b = biome_from_column(col, floor_quart((int32_t)y), tree);
+                if ((want & 1) && b == 183) mask |= 1;
+                if ((want & 2) && b == 14) mask |= 2;
+            }
+        }
+    }
+    return mask;
+}

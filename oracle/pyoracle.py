@@ -237,6 +237,8 @@ def biome_lib():
         L.ob_biome_at.restype = ci
         L.ob_filter.argtypes = [C.c_void_p, i64, i32, i32, ci, ci]
         L.ob_filter.restype = ci
+        L.ob_filter_levels.argtypes = [C.c_void_p, i64, i32, i32, ci, i32, i32, i32, ci]
+        L.ob_filter_levels.restype = ci
         L.ob_spline.argtypes = [C.POINTER(C.c_float)]
         L.ob_spline.restype = C.c_float
         L.ob_tree_for_version.argtypes = [ci]

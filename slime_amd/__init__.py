@@ -50,6 +50,7 @@ BIOME_EXPORTS = (
     "mcgpu_climate_ctx_destroy", "mcgpu_climate6_i32_fixed_seed",
     "mcgpu_climate6_i32_fixed_seed_ex", "mcgpu_biome20_i32_fixed_seed", "mcgpu_biome20_ctx_create",
     "mcgpu_biome20_ctx_query", "mcgpu_biome20_ctx_destroy",
+    "mcgpu_modern_ctx_filter_candidates_levels", "mcgpu_filter_multi_levels",
 )
 
 
@@ -172,6 +173,12 @@ def lib():
     L.mcgpu_filter_multi.argtypes = [i32, i32, PU64, PI32, PI32, PI32, i32, i32, PI32]
     L.mcgpu_filter_multi.restype = C.c_int
     L.mcgpu_filter_multi_last_ms.restype = C.c_float
+    L.mcgpu_modern_ctx_filter_candidates_levels.argtypes = [VP, PI32, PI32, i32, i32, i32, i32,
+                                                            i32, PI32]
+    L.mcgpu_modern_ctx_filter_candidates_levels.restype = C.c_int
+    L.mcgpu_filter_multi_levels.argtypes = [i32, i32, PU64, PI32, PI32, PI32, i32, i32, i32, i32,
+                                            i32, PI32]
+    L.mcgpu_filter_multi_levels.restype = C.c_int
     L.mcgpu_xoroshiro_nextlong_batch.argtypes = [PU64, i32, i32, PU64]
     L.mcgpu_xperlin_sample_batch.argtypes = [PU64, PD, PD, PD, i32, PD]
     L.mcgpu_xperlin_sample_fixed_seed.argtypes = [U64, PD, PD, PD, i32, PD]
@@ -450,6 +457,7 @@ class Session:
 
 
 DEEP_DARK, MUSHROOM_FIELDS = 1, 2  # filter flags / mask bits
+ALL_CHUNKS = 4  # level-range filter: every chunk of the disk, not only its slime chunks
 
 
 class BiomeContext:
@@ -488,6 +496,18 @@ class BiomeContext:
             raise RuntimeError(f"filter failed ({rc})")
         return list(out)[:n]
 
+    def filter_levels(self, center_x, center_z, y_min, y_max, y_step,
+                      flags=DEEP_DARK | MUSHROOM_FIELDS):
+        """The filter over block heights y_min..y_max step y_step (both biomes
+        at every height; flags | ALL_CHUNKS samples the whole disk)."""
+        n = len(center_x)
+        out = (C.c_int32 * max(1, n))()
+        rc = lib().mcgpu_modern_ctx_filter_candidates_levels(
+            self._h, self._arr(center_x), self._arr(center_z), n, flags, y_min, y_max, y_step, out)
+        if rc != 0:
+            raise RuntimeError(f"filter_levels failed ({rc})")
+        return list(out)[:n]
+
     def query(self, xs, ys, zs):
         n = len(xs)
         out = (C.c_int32 * max(1, n))()
@@ -510,10 +530,11 @@ class BiomeContext:
 
 
 def filter_multi(world_seeds, candidates, flags=DEEP_DARK | MUSHROOM_FIELDS, large_biomes=False,
-                 version=2602):
+                 version=2602, levels=None):
     """Biome filter of candidates of many worlds in one launch: candidates[w]
-    is a list of (center_chunk_x, center_chunk_z) of world world_seeds[w].
-    Returns the masks per world and the device time in ms."""
+    is a list of (center_chunk_x, center_chunk_z) of world world_seeds[w];
+    levels = (y_min, y_max, y_step) selects the level-range form. Returns the
+    masks per world and the device time in ms."""
     n = len(world_seeds)
     first = [0]
     for c in candidates:
@@ -525,8 +546,12 @@ def filter_multi(world_seeds, candidates, flags=DEEP_DARK | MUSHROOM_FIELDS, lar
     xs = (C.c_int32 * max(1, total))(*[c[0] for c in flat])
     zs = (C.c_int32 * max(1, total))(*[c[1] for c in flat])
     out = (C.c_int32 * max(1, total))()
-    rc = lib().mcgpu_filter_multi(n, version, seeds, f, xs, zs, flags, 1 if large_biomes else 0,
-                                  out)
+    large = 1 if large_biomes else 0
+    if levels is None:
+        rc = lib().mcgpu_filter_multi(n, version, seeds, f, xs, zs, flags, large, out)
+    else:
+        rc = lib().mcgpu_filter_multi_levels(n, version, seeds, f, xs, zs, flags, large, *levels,
+                                             out)
     if rc != 0:
         raise RuntimeError(f"mcgpu_filter_multi failed ({rc})")
     masks = [list(out[first[w]:first[w + 1]]) for w in range(n)]

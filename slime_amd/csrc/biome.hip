@@ -12,6 +12,10 @@
 //      depth and the nearest-leaf search of the version's parameter tree;
 //   3. a found biome sets the candidate's mask bit in LDS; columns stop once
 //      every requested bit is set.
+// Besides the reference's heights, the filter takes a range of block heights
+// (the wider sweep y -64..320 step 4, say) at which both biomes are looked
+// for, one tree search per height, and optionally every chunk of the disk
+// (flags bit 2) instead of its slime chunks.
 // The climate state (46 octaves, 14.9 KB) is built on the host from the world
 // seed and copied into each workgroup's LDS once; so are the tree's intervals.
 // Kernels that search a tree are instantiated for both node layouts.
@@ -88,81 +92,103 @@ bool find_targets(const BiomeTreeDef& tree, TargetLeaves& t) {
 }
 
 // Stops a search at the first kept leaf that is not one of the targets (the
-// leaves held by value: constant indices keep them in registers).
+// leaves held by value: constant indices keep them in registers; -1 marks an
+// unused entry).
+template <int N>
 struct NotTarget {
-    int32_t leaf[kMaxTargetLeaves];
-    int n;
+    int32_t leaf[N];
     __device__ bool operator()(int l) const {
         bool hit = false;
 #pragma unroll
-        for (int i = 0; i < kMaxTargetLeaves; ++i) hit |= i < n && leaf[i] == l;
+        for (int i = 0; i < N; ++i) hit |= leaf[i] == l;
         return !hit;
     }
 };
 
-// Whether the search from the root at layer yi of a column returns one of
-// `leaves` (the biome's leaves). Not if every one of them lies strictly
-// farther than `hint`, a leaf found before (the nearest leaf is no farther).
-// Otherwise the search runs with the bound (nearest target's distance + 1)
-// and stops at the first kept leaf that is not a target: that leaf is no
-// farther than every target and comes first in visiting order among the
-// leaves that near, so the result cannot be a target. If the search ends
-// without stopping, every leaf it kept was a target. `hint` gets the last
-// leaf kept (a bound for the next layers).
-template <int F>
-__device__ __forceinline__ bool layer_is(const Column& col, int32_t yi, const uint64_t* nodes,
-                                        const int32_t* params, int32_t len, int& hint,
-                                        const int32_t* leaves, int nleaves) {
-    if (nleaves == 0) return false;
+template <int N>
+__device__ __forceinline__ NotTarget<N> targets_of(const int32_t* leaves, int n) {
+    NotTarget<N> t;
+#pragma unroll
+    for (int i = 0; i < N; ++i) t.leaf[i] = i < n ? leaves[i] : -1;
+    return t;
+}
+
+// The leaf the search from the root at layer yi of a column returns if it is
+// one of the targets, else -1. Not a target if every one of them lies
+// strictly farther than `hint`, a leaf found before (the nearest leaf is no
+// farther). Otherwise the search runs with the bound (nearest target's
+// distance + 1) and stops at the first kept leaf that is not a target: that
+// leaf is no farther than every target and comes first in visiting order
+// among the leaves that near, so the result cannot be a target. If the search
+// ends without stopping, every leaf it kept was a target and the last is the
+// result. `hint` gets the last leaf kept (a bound for the next layers).
+template <int F, int N>
+__device__ __forceinline__ int layer_target(const Column& col, int32_t yi, const uint64_t* nodes,
+                                            const int32_t* params, int32_t len, int& hint,
+                                            const NotTarget<N>& other) {
     const int32_t np[6] = {col.t, col.h, col.c, col.e, depth_at(col.offset, yi), col.w};
-    NotTarget other;
-    other.n = nleaves;
     uint64_t target = ~0ull;
 #pragma unroll
-    for (int i = 0; i < kMaxTargetLeaves; ++i) {
-        other.leaf[i] = leaves[i];
-        if (i < nleaves) {
-            const uint64_t d = node_distance(np, nodes, params, leaves[i]);
+    for (int i = 0; i < N; ++i) {
+        if (other.leaf[i] >= 0) {
+            const uint64_t d = node_distance(np, nodes, params, other.leaf[i]);
             target = d < target ? d : target;
         }
     }
-    if (hint >= 0 && target > node_distance(np, nodes, params, hint)) return false;
+    if (target == ~0ull) return -1;
+    if (hint >= 0 && target > node_distance(np, nodes, params, hint)) return -1;
     bool stopped = false;
     const int leaf =
         tree_nearest<F, 0>(np, nodes, params, len, 0, -1, target + 1, other, stopped);
     if (leaf >= 0) hint = leaf;
-    return !stopped && leaf >= 0;
+    return stopped ? -1 : leaf;
 }
+
+// The heights a filter samples. n == 0: the reference's set (Deep Dark at
+// quart y -16..0, Mushroom Fields at quart y 16). Otherwise every requested
+// biome at block heights y0, y0 + step, ... (n of them), sampled at quart y
+// floor(y / 4), one search per height for all biomes still wanted.
+struct Levels {
+    int32_t y0, step, n;
+};
+
+__device__ __forceinline__ int32_t floor_quart(int32_t y) { return y >> 2; }
 
 // The mask of one candidate, by the whole workgroup (every thread calls it;
 // the caller synchronises before the first call when the LDS was in use).
-template <int F>
+// L: the level-range form (lv.n > 0), a kernel of its own so that the
+// reference's form keeps its register budget.
+template <int F, bool L>
 __device__ __forceinline__ unsigned int filter_candidate(const Climate& st, FilterLds& f,
                                                          const uint64_t* nodes, int32_t len,
                                                          const TargetLeaves& tl,
                                                          int64_t world_seed, int32_t cx,
-                                                         int32_t cz, int32_t flags, int& hint) {
+                                                         int32_t cz, int32_t flags,
+                                                         const Levels& lv, int& hint) {
     if (threadIdx.x == 0) {
         f.nchunks = 0;
         f.mask = 0u;
     }
     __syncthreads();
+    const bool all_chunks = (flags & 4) != 0;
     for (int ci = threadIdx.x; ci < 17 * 17; ci += blockDim.x) {
         const int dx = ci / 17 - 8, dz = ci % 17 - 8;
         if (dx * dx + dz * dz <= 68 &&
-            is_slime(world_seed, (int32_t)((uint32_t)cx + (uint32_t)dx),
-                     (int32_t)((uint32_t)cz + (uint32_t)dz))) {
+            (all_chunks || is_slime(world_seed, (int32_t)((uint32_t)cx + (uint32_t)dx),
+                                    (int32_t)((uint32_t)cz + (uint32_t)dz)))) {
             const int slot = atomicAdd(&f.nchunks, 1);
             f.chunks[slot] = (uint16_t)ci;
         }
     }
     __syncthreads();
     const int columns = f.nchunks * 16;
-    const unsigned int want = (unsigned int)flags;
+    const unsigned int want = (unsigned int)flags & 3u;
     const int32_t* params = f.params;
+    const NotTarget<kMaxTargetLeaves> deep = targets_of<kMaxTargetLeaves>(tl.deep, tl.ndeep);
+    const NotTarget<kMaxTargetLeaves> mush = targets_of<kMaxTargetLeaves>(tl.mush, tl.nmush);
     for (int item = threadIdx.x; item < columns; item += blockDim.x) {
         // bits another column already found need no more checks
-        const unsigned int need = want & ~__atomic_load_n(&f.mask, __ATOMIC_RELAXED);
+        unsigned int need = want & ~__atomic_load_n(&f.mask, __ATOMIC_RELAXED);
         if (need == 0) break;
         const int ci = f.chunks[item >> 4], q = item & 15;
         const int dx = ci / 17 - 8, dz = ci % 17 - 8;
@@ -171,13 +197,35 @@ __device__ __forceinline__ unsigned int filter_candidate(const Climate& st, Filt
         const Column col = column_at(st, qx, qz);
         unsigned int found = 0;
         // hint: the thread's last leaf, a bound for every later layer and column
-        if ((need & 2u) && layer_is<F>(col, 16, nodes, params, len, hint, tl.mush, tl.nmush))
-            found |= 2u;
-        if (need & 1u) {
-            for (int32_t y = -16; y <= 0; ++y) {  // deepest first: Deep Dark lies deep
-                if (layer_is<F>(col, y, nodes, params, len, hint, tl.deep, tl.ndeep)) {
-                    found |= 1u;
-                    break;
+        if constexpr (!L) {
+            if ((need & 2u) && layer_target<F>(col, 16, nodes, params, len, hint, mush) >= 0)
+                found |= 2u;
+            if (need & 1u) {
+                for (int32_t y = -16; y <= 0; ++y) {  // deepest first: Deep Dark lies deep
+                    if (layer_target<F>(col, y, nodes, params, len, hint, deep) >= 0) {
+                        found |= 1u;
+                        break;
+                    }
+                }
+            }
+        } else {
+            int32_t prev = 0;
+            for (int32_t i = 0; i < lv.n && need != 0; ++i) {
+                const int32_t yi = floor_quart(lv.y0 + i * lv.step);
+                if (i > 0 && yi == prev) continue;  // steps below 4 blocks share quarts
+                prev = yi;
+                NotTarget<2 * kMaxTargetLeaves> both;
+#pragma unroll
+                for (int k = 0; k < kMaxTargetLeaves; ++k) {
+                    both.leaf[k] = (need & 1u) ? deep.leaf[k] : -1;
+                    both.leaf[kMaxTargetLeaves + k] = (need & 2u) ? mush.leaf[k] : -1;
+                }
+                const int l = layer_target<F>(col, yi, nodes, params, len, hint, both);
+                if (l >= 0) {
+                    const bool is_deep = !deep(l);  // deep(l) is true for other leaves
+                    const unsigned int bit = is_deep ? 1u : 2u;
+                    found |= bit;
+                    need &= ~bit;
                 }
             }
         }
@@ -187,12 +235,12 @@ __device__ __forceinline__ unsigned int filter_candidate(const Climate& st, Filt
     return f.mask;
 }
 
-template <int F>
+template <int F, bool L>
 __global__ __launch_bounds__(kThreads) void filter_kernel(const Climate* climate, TreeArgs tree,
                                                           TargetLeaves tl, int64_t world_seed,
                                                           const int32_t* ccx, const int32_t* ccz,
                                                           int32_t count, int32_t flags,
-                                                          int32_t* mask_out) {
+                                                          Levels lv, int32_t* mask_out) {
     __shared__ Climate st;
     __shared__ FilterLds f;
     load_climate(st, climate);
@@ -200,8 +248,8 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(const Climate* climate
     int hint = -1;
     for (int cand = blockIdx.x; cand < count; cand += gridDim.x) {
         __syncthreads();  // the previous candidate's LDS is no longer read
-        const unsigned int m = filter_candidate<F>(st, f, tree.nodes, tree.len, tl, world_seed,
-                                                   ccx[cand], ccz[cand], flags, hint);
+        const unsigned int m = filter_candidate<F, L>(st, f, tree.nodes, tree.len, tl, world_seed,
+                                                   ccx[cand], ccz[cand], flags, lv, hint);
         if (threadIdx.x == 0) mask_out[cand] = (int32_t)m;
     }
 }
@@ -209,11 +257,11 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(const Climate* climate
 // Candidates of many worlds: world w's candidates are [first[w], first[w+1]),
 // its climate climates[w]. A workgroup takes a contiguous run of candidates,
 // so it reloads its LDS climate only where the run crosses into a new world.
-template <int F>
+template <int F, bool L>
 __global__ __launch_bounds__(kThreads) void filter_multi_kernel(
     const Climate* climates, const int64_t* seeds, const int32_t* first, int32_t nworlds,
     TreeArgs tree, TargetLeaves tl, const int32_t* ccx, const int32_t* ccz, int32_t count,
-    int32_t per_block, int32_t flags, int32_t* mask_out) {
+    int32_t per_block, int32_t flags, Levels lv, int32_t* mask_out) {
     __shared__ Climate st;
     __shared__ FilterLds f;
     load_params(f.params, tree);
@@ -229,8 +277,8 @@ __global__ __launch_bounds__(kThreads) void filter_multi_kernel(
             load_climate(st, climates + w);
             world = w;
         }
-        const unsigned int m = filter_candidate<F>(st, f, tree.nodes, tree.len, tl, seeds[w],
-                                                   ccx[cand], ccz[cand], flags, hint);
+        const unsigned int m = filter_candidate<F, L>(st, f, tree.nodes, tree.len, tl, seeds[w],
+                                                   ccx[cand], ccz[cand], flags, lv, hint);
         if (threadIdx.x == 0) mask_out[cand] = (int32_t)m;
     }
 }
@@ -517,18 +565,17 @@ struct Context {
         return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream) == hipSuccess;
     }
 
-    int filter(const int32_t* cx, const int32_t* cz, int32_t count, int32_t flags, int32_t* out) {
+    int filter(const int32_t* cx, const int32_t* cz, int32_t count, int32_t flags,
+               const Levels& lv, int32_t* out) {
         const size_t ni = sizeof(int32_t) * (size_t)count;
         const int grid = (int)std::min<int64_t>(count, (int64_t)cus * 4);
         if (!upload(d_x, cx, ni) || !upload(d_z, cz, ni) ||
             hipEventRecord(ev0, stream) != hipSuccess)
             return MCGPU_ERR_CUDA;
-        if (family == 0)
-            hipLaunchKernelGGL(filter_kernel<0>, dim3(grid), dim3(kThreads), 0, stream, d_climate,
-                               targs, targets, world_seed, d_x, d_z, count, flags, d_out);
-        else
-            hipLaunchKernelGGL(filter_kernel<1>, dim3(grid), dim3(kThreads), 0, stream, d_climate,
-                               targs, targets, world_seed, d_x, d_z, count, flags, d_out);
+        auto* kernel = lv.n == 0 ? (family == 0 ? filter_kernel<0, false> : filter_kernel<1, false>)
+                                 : (family == 0 ? filter_kernel<0, true> : filter_kernel<1, true>);
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), 0, stream, d_climate, targs,
+                           targets, world_seed, d_x, d_z, count, flags, lv, d_out);
         if (hipGetLastError() != hipSuccess || hipEventRecord(ev1, stream) != hipSuccess ||
             finish(out, ni) != MCGPU_OK || hipEventElapsedTime(&last_ms, ev0, ev1) != hipSuccess)
             return MCGPU_ERR_CUDA;
@@ -606,8 +653,8 @@ struct Context {
 
 // Filter of candidates of many worlds in one launch (see filter_multi_kernel).
 int filter_multi(int32_t nworlds, int tree, const uint64_t* seeds, const int32_t* first,
-                 const int32_t* cx, const int32_t* cz, int32_t flags, bool large, int32_t* out,
-                 float* ms) {
+                 const int32_t* cx, const int32_t* cz, int32_t flags, const Levels& lv,
+                 bool large, int32_t* out, float* ms) {
     const int32_t count = first[nworlds];
     const int cus = device_cus();
     TreeArgs targs{};
@@ -648,14 +695,13 @@ int filter_multi(int32_t nworlds, int tree, const uint64_t* seeds, const int32_t
               hipMemcpyAsync(d_z, cz, ni, hipMemcpyHostToDevice, stream) == hipSuccess &&
               hipEventRecord(e0, stream) == hipSuccess;
     if (ok) {
-        if (kTreeDefs[tree].family == 0)
-            hipLaunchKernelGGL(filter_multi_kernel<0>, dim3(grid), dim3(kThreads), 0, stream,
-                               d_clim, d_seeds, d_first, nworlds, targs, targets, d_x, d_z, count,
-                               per_block, flags, d_out);
-        else
-            hipLaunchKernelGGL(filter_multi_kernel<1>, dim3(grid), dim3(kThreads), 0, stream,
-                               d_clim, d_seeds, d_first, nworlds, targs, targets, d_x, d_z, count,
-                               per_block, flags, d_out);
+        const bool f0 = kTreeDefs[tree].family == 0;
+        auto* kernel = lv.n == 0
+                           ? (f0 ? filter_multi_kernel<0, false> : filter_multi_kernel<1, false>)
+                           : (f0 ? filter_multi_kernel<0, true> : filter_multi_kernel<1, true>);
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), 0, stream, d_clim, d_seeds,
+                           d_first, nworlds, targs, targets, d_x, d_z, count, per_block, flags, lv,
+                           d_out);
         ok = hipGetLastError() == hipSuccess && hipEventRecord(e1, stream) == hipSuccess &&
              hipMemcpyAsync(out, d_out, ni, hipMemcpyDeviceToHost, stream) == hipSuccess &&
              hipStreamSynchronize(stream) == hipSuccess &&
@@ -715,6 +761,21 @@ bool count_ok(int32_t count) { return count > 0; }
 
 int tree_20() { return slime::biome::tree_for_version(1206); }
 
+// Flags of the level-range filter: a biome (bit 0 or 1) and optionally bit 2.
+bool level_flags_ok(int32_t flags) { return (flags & ~7) == 0 && (flags & 3) != 0; }
+
+// Block heights y_min, y_min + y_step, ... <= y_max: at most kMaxFilterLevels
+// heights within +-2^24 blocks (their quart y and sums stay in int32).
+constexpr int32_t kMaxFilterLevels = 4096;
+bool levels_of(int32_t y_min, int32_t y_max, int32_t y_step, slime::biome::Levels& lv) {
+    constexpr int32_t kLimit = 1 << 24;
+    if (y_step < 1 || y_min > y_max || y_min < -kLimit || y_max > kLimit) return false;
+    const int64_t n = ((int64_t)y_max - y_min) / y_step + 1;
+    if (n > kMaxFilterLevels) return false;
+    lv = slime::biome::Levels{y_min, y_step, (int32_t)n};
+    return true;
+}
+
 unsigned blocks_for(int32_t count) {
     return (unsigned)((count + slime::biome::kThreads - 1) / slime::biome::kThreads);
 }
@@ -747,7 +808,20 @@ int mcgpu_modern_ctx_filter_candidates(void* handle, const int32_t* center_chunk
     if (!ctx || ctx->tree < 0 || !center_chunk_x || !center_chunk_z || !mask_out || count <= 0 ||
         count > ctx->capacity || (flags & ~3) != 0 || flags == 0)
         return MCGPU_ERR_ARG;
-    return cleared(ctx->filter(center_chunk_x, center_chunk_z, count, flags, mask_out));
+    return cleared(ctx->filter(center_chunk_x, center_chunk_z, count, flags,
+                               slime::biome::Levels{0, 0, 0}, mask_out));
+}
+
+int mcgpu_modern_ctx_filter_candidates_levels(void* handle, const int32_t* center_chunk_x,
+                                              const int32_t* center_chunk_z, int32_t count,
+                                              int32_t flags, int32_t y_min, int32_t y_max,
+                                              int32_t y_step, int32_t* mask_out) {
+    Context* ctx = static_cast<Context*>(handle);
+    slime::biome::Levels lv{};
+    if (!ctx || ctx->tree < 0 || !center_chunk_x || !center_chunk_z || !mask_out || count <= 0 ||
+        count > ctx->capacity || !level_flags_ok(flags) || !levels_of(y_min, y_max, y_step, lv))
+        return MCGPU_ERR_ARG;
+    return cleared(ctx->filter(center_chunk_x, center_chunk_z, count, flags, lv, mask_out));
 }
 
 int mcgpu_modern_ctx_query(void* handle, const int32_t* xs, const int32_t* ys, const int32_t* zs,
@@ -770,20 +844,45 @@ int mcgpu_modern_ctx_columns(void* handle, const int32_t* xs, const int32_t* zs,
     return cleared(ctx->columns(xs, zs, count, out6));
 }
 
-int mcgpu_filter_multi(int32_t nworlds, int32_t version_code, const uint64_t* world_seeds,
-                       const int32_t* first, const int32_t* center_chunk_x,
-                       const int32_t* center_chunk_z, int32_t flags, int32_t large_biomes,
-                       int32_t* mask_out) {
+namespace {
+
+int filter_multi_checked(int32_t nworlds, int32_t version_code, const uint64_t* world_seeds,
+                         const int32_t* first, const int32_t* center_chunk_x,
+                         const int32_t* center_chunk_z, int32_t flags,
+                         const slime::biome::Levels& lv, int32_t large_biomes,
+                         int32_t* mask_out) {
     const int tree = slime::biome::tree_for_version(version_code);
     if (nworlds <= 0 || tree < 0 || !world_seeds || !first || !center_chunk_x ||
-        !center_chunk_z || !mask_out || (flags & ~3) != 0 || flags == 0 || first[0] != 0)
+        !center_chunk_z || !mask_out || first[0] != 0)
         return MCGPU_ERR_ARG;
     for (int32_t w = 0; w < nworlds; ++w)
         if (first[w + 1] < first[w]) return MCGPU_ERR_ARG;
     if (first[nworlds] == 0) return MCGPU_OK;
     return cleared(slime::biome::filter_multi(nworlds, tree, world_seeds, first, center_chunk_x,
-                                              center_chunk_z, flags, large_biomes != 0, mask_out,
-                                              &g_multi_ms));
+                                              center_chunk_z, flags, lv, large_biomes != 0,
+                                              mask_out, &g_multi_ms));
+}
+
+}  // namespace
+
+int mcgpu_filter_multi(int32_t nworlds, int32_t version_code, const uint64_t* world_seeds,
+                       const int32_t* first, const int32_t* center_chunk_x,
+                       const int32_t* center_chunk_z, int32_t flags, int32_t large_biomes,
+                       int32_t* mask_out) {
+    if ((flags & ~3) != 0 || flags == 0) return MCGPU_ERR_ARG;
+    return filter_multi_checked(nworlds, version_code, world_seeds, first, center_chunk_x,
+                                center_chunk_z, flags, slime::biome::Levels{0, 0, 0},
+                                large_biomes, mask_out);
+}
+
+int mcgpu_filter_multi_levels(int32_t nworlds, int32_t version_code, const uint64_t* world_seeds,
+                              const int32_t* first, const int32_t* center_chunk_x,
+                              const int32_t* center_chunk_z, int32_t flags, int32_t large_biomes,
+                              int32_t y_min, int32_t y_max, int32_t y_step, int32_t* mask_out) {
+    slime::biome::Levels lv{};
+    if (!level_flags_ok(flags) || !levels_of(y_min, y_max, y_step, lv)) return MCGPU_ERR_ARG;
+    return filter_multi_checked(nworlds, version_code, world_seeds, first, center_chunk_x,
+                                center_chunk_z, flags, lv, large_biomes, mask_out);
 }
 
 float mcgpu_filter_multi_last_ms(void) { return g_multi_ms; }

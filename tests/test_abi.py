@@ -36,7 +36,7 @@ def test_python_mirror_lists_every_export():
 
 def test_library_exports_every_biome_symbol(native):
     names = declared_functions(BIOME_HEADER)
-    assert len(names) == 28
+    assert len(names) == 30
     for name in names:
         assert hasattr(native, name), name
     assert native.mcgpu_api_version() == 1
@@ -61,6 +61,28 @@ def test_invalid_arguments_return_minus_one(native):
     assert native.search_slime_clusters_gpu_v1(1, -1, 40, 221, 0, buf, 4, 0) == -1
     assert native.search_slime_clusters(1, 10, 50, 40, 0, buf, 4, 1, 0) == -1
     assert native.slime_scan_rect(1, 0, 0, 0, 10, 40, 221, buf, 4, None) == -1
+
+
+def test_biome_filter_argument_checks_need_no_device(native):
+    """Invalid flags, heights and world lists are refused before any device
+    call (MCGPU_ERR_ARG = -1)."""
+    import slime_amd
+    slime_amd.lib()  # argtypes
+    seeds = (C.c_uint64 * 2)(1, 2)
+    first = (C.c_int32 * 3)(0, 1, 2)
+    a = (C.c_int32 * 2)()
+    f = native.mcgpu_filter_multi_levels
+    assert f(2, 2602, seeds, first, a, a, 3, 0, -64, 320, 0, a) == -1  # step
+    assert f(2, 2602, seeds, first, a, a, 3, 0, 9, 8, 1, a) == -1  # empty range
+    assert f(2, 2602, seeds, first, a, a, 3, 0, 0, 4096, 1, a) == -1  # 4097 heights
+    assert f(2, 2602, seeds, first, a, a, 4, 0, -64, 0, 4, a) == -1  # no biome
+    assert f(2, 2602, seeds, first, a, a, 16 | 1, 0, -64, 0, 4, a) == -1
+    assert f(2, 1179, seeds, first, a, a, 3, 0, -64, 0, 4, a) == -1  # no tree before 1.18
+    assert f(0, 2602, seeds, first, a, a, 3, 0, -64, 0, 4, a) == -1
+    bad_first = (C.c_int32 * 3)(0, 2, 1)
+    assert f(2, 2602, seeds, bad_first, a, a, 3, 0, -64, 0, 4, a) == -1
+    assert native.mcgpu_filter_multi(2, 2602, seeds, first, a, a, 4, 0, a) == -1
+    assert native.mcgpu_modern_ctx_filter_candidates_levels(None, a, a, 1, 3, -64, 0, 4, a) == -1
 
 
 def test_is_slime_chunk_matches_reference_bits(native):

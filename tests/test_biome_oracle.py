@@ -341,3 +341,86 @@ def test_first_descent_reaches_a_leaf(bl, tree):
     no leaf instead of node 0 changes nothing)."""
     leaf = bl.ob_tree_first_descent(tree)
     assert leaf >= 0 and (bl.ob_tree_node(tree, leaf) >> 56) == 0xFF
+
+
+def disk_mask_brute(bl, c, seed, cx, cz, ys, all_chunks, tree=0):
+    """The level-range filter's mask by point queries: every sampled chunk of
+    the disk, its 16 quart columns, quart y = floor(y / 4) per height."""
+    from oracle.pyoracle import oracle_lib
+    ol = oracle_lib()
+    mask = 0
+    for dx in range(-8, 9):
+        for dz in range(-8, 9):
+            if dx * dx + dz * dz > 68:
+                continue
+            x, z = cx + dx, cz + dz
+            if not all_chunks and not ol.oracle_is_slime(seed, x, z):
+                continue
+            for q in range(16):
+                qx, qz = 4 * x + q // 4, 4 * z + q % 4
+                for y in ys:
+                    b = bl.ob_biome_at(c, qx, y >> 2, qz, tree)
+                    mask |= (1 if b == 183 else 0) | (2 if b == 14 else 0)
+    return mask
+
+
+def deep_dark_site(bl, c, rng, yq=-10, tree=0):
+    """A candidate whose disk holds a Deep Dark quart at quart y yq (the chunk
+    3 east and 2 north of the center)."""
+    for _ in range(20000):
+        qx, qz = rng.randint(-2_000_000, 2_000_000), rng.randint(-2_000_000, 2_000_000)
+        if bl.ob_biome_at(c, qx, yq, qz, tree) == 183:
+            return (qx >> 2) - 3, (qz >> 2) + 2
+    return None
+
+
+def test_level_filter_reduces_to_the_reference_set(bl):
+    """Heights -64..0 (any step up to 4, any phase within the quart) with Deep
+    Dark alone, and 64 alone with Mushroom Fields, over slime chunks, give the
+    reference filter's masks."""
+    seed = 0x0DDBA11CAFE5EED5
+    rng = random.Random(71)
+    c = bl.ob_climate_new(seed, 0)
+    try:
+        cands = [(rng.randint(-1_800_000, 1_800_000), rng.randint(-1_800_000, 1_800_000))
+                 for _ in range(6)]
+        site = deep_dark_site(bl, c, rng)
+        assert site is not None
+        cands.append(site)
+        found = 0
+        for x, z in cands:
+            m1 = bl.ob_filter(c, seed, x, z, 1, 0)
+            found |= m1
+            for lo, hi, step in ((-64, 0, 4), (-64, 3, 1), (-61, 3, 4), (-63, 2, 2)):
+                assert bl.ob_filter_levels(c, seed, x, z, 1, lo, hi, step, 0) == m1, (lo, hi, step)
+            m2 = bl.ob_filter(c, seed, x, z, 2, 0)
+            for lo, hi, step in ((64, 64, 1), (64, 67, 1), (67, 70, 8)):
+                assert bl.ob_filter_levels(c, seed, x, z, 2, lo, hi, step, 0) == m2, (lo, hi, step)
+        # the planted chunk need not be a slime chunk; with every chunk it counts
+        x, z = site
+        assert bl.ob_filter_levels(c, seed, x, z, 1 | 4, -64, 0, 4, 0) == 1
+    finally:
+        bl.ob_climate_free(c)
+
+
+@pytest.mark.parametrize("all_chunks", [False, True])
+def test_level_filter_equals_point_queries(bl, all_chunks):
+    """The level-range mask equals the union over point queries of the disk's
+    sampled chunks, at heights that fall in the middle of quarts and below 0."""
+    seed = -3141592653589793238
+    rng = random.Random(72 + all_chunks)
+    c = bl.ob_climate_new(seed & ((1 << 64) - 1), 0)
+    try:
+        site = deep_dark_site(bl, c, rng, yq=-10)
+        cands = [site, (rng.randint(-100_000, 100_000), rng.randint(-100_000, 100_000))]
+        ys = list(range(-38, 40, 19))  # -38, -19, 0, 19, 38: quarts -10, -5, 0, 4, 9
+        flags = 3 | (4 if all_chunks else 0)
+        masks = []
+        for x, z in cands:
+            got = bl.ob_filter_levels(c, seed, x, z, flags, ys[0], ys[-1], 19, 0)
+            assert got == disk_mask_brute(bl, c, seed, x, z, ys, all_chunks)
+            masks.append(got)
+        if all_chunks:
+            assert masks[0] & 1  # the planted quart lies at quart y -10 = floor(-38 / 4)
+    finally:
+        bl.ob_climate_free(c)

@@ -279,3 +279,107 @@ def test_filter_many_candidates(gpu, bl, version, tree):
               sum(m >> 1 & 1 for m in got), flush=True)
     finally:
         bl.ob_climate_free(clim)
+
+
+def deep_dark_site(bl, clim, rng, yq=-10, tree=0):
+    """A candidate whose disk holds a Deep Dark quart at quart y yq (in the
+    chunk 3 east and 2 north of the center, slime or not)."""
+    for _ in range(20000):
+        qx, qz = rng.randint(-2_000_000, 2_000_000), rng.randint(-2_000_000, 2_000_000)
+        if bl.ob_biome_at(clim, qx, yq, qz, tree) == 183:
+            return (qx >> 2) - 3, (qz >> 2) + 2
+    return None
+
+
+@pytest.mark.parametrize("version,tree", [(2602, 0), (1180, 6)])
+def test_level_filter_matches_oracle(gpu, bl, version, tree):
+    """The level-range filter (the wider sweep y -64..320 step 4, both biomes
+    at every height) over slime chunks and over every chunk of the disk,
+    against the restatement."""
+    import slime_amd
+    seed = 0x51DE5EED0000002A + tree
+    rng = random.Random(40 + tree)
+    clim = bl.ob_climate_new(seed, 0)
+    try:
+        cands = [(rng.randint(-1_800_000, 1_800_000), rng.randint(-1_800_000, 1_800_000))
+                 for _ in range(6)]
+        for yq in (-10, -2):
+            site = deep_dark_site(bl, clim, rng, yq, tree)  # none before 1.19
+            if site:
+                cands.append(site)
+        n_dd = len(cands) - 6
+        mf = find_biome_chunk(bl, clim, seed, 14, [16, 40], rng, tries=60000, tree=tree)
+        if mf:
+            cands.append((mf[0] - 1, mf[1] + 4))
+        print("candidates", cands, flush=True)
+        cx, cz = [c[0] for c in cands], [c[1] for c in cands]
+        with slime_amd.BiomeContext(seed, capacity=len(cands), version=version) as ctx:
+            for flags, levels in ((3, (-64, 320, 4)), (3 | 4, (-64, 320, 16)), (1 | 4, (-64, 0, 4)),
+                                  (2, (-70, 330, 5))):
+                got = ctx.filter_levels(cx, cz, *levels, flags=flags)
+                want = [bl.ob_filter_levels(clim, seed, x, z, flags, *levels, tree)
+                        for x, z in cands]
+                print("flags", flags, levels, got, flush=True)
+                assert got == want
+            got = ctx.filter_levels(cx, cz, -64, 0, 4, flags=1 | 4)
+            assert all(m & 1 for m in got[6:6 + n_dd])
+    finally:
+        bl.ob_climate_free(clim)
+
+
+def test_level_filter_reduces_to_the_reference_filter(gpu):
+    """Over slime chunks, Deep Dark at heights -64..0 (steps up to 4) and
+    Mushroom Fields at 64 give the reference filter's masks, candidate by
+    candidate, on many candidates."""
+    import slime_amd
+    seed = 0x7E57AB1E
+    rng = random.Random(44)
+    n = 600
+    cx = [rng.randint(-1_800_000, 1_800_000) for _ in range(n)]
+    cz = [rng.randint(-1_800_000, 1_800_000) for _ in range(n)]
+    with slime_amd.BiomeContext(seed, capacity=n) as ctx:
+        m1 = ctx.filter(cx, cz, 1)
+        assert sum(m1) > 0
+        for levels in ((-64, 0, 4), (-64, 3, 1), (-61, 3, 4), (-63, 2, 2)):
+            assert ctx.filter_levels(cx, cz, *levels, flags=1) == m1, levels
+        m2 = ctx.filter(cx, cz, 2)
+        for levels in ((64, 64, 1), (64, 67, 1)):
+            assert ctx.filter_levels(cx, cz, *levels, flags=2) == m2, levels
+        # both biomes over the range, every chunk: a superset of both
+        m3 = ctx.filter_levels(cx, cz, -64, 320, 4, flags=3 | 4)
+        assert all((a | b) & ~c == 0 for a, b, c in zip(m1, m2, m3))
+
+
+def test_level_filter_multi_equals_per_world_contexts(gpu):
+    import slime_amd
+    rng = random.Random(45)
+    seeds = [rng.getrandbits(64) for _ in range(4)]
+    cands = [[(rng.randint(-1_800_000, 1_800_000), rng.randint(-1_800_000, 1_800_000))
+              for _ in range(rng.randint(0, 40))] for _ in seeds]
+    levels = (-64, 320, 4)
+    masks, ms = slime_amd.filter_multi(seeds, cands, 3 | slime_amd.ALL_CHUNKS, levels=levels)
+    assert ms > 0
+    for seed, cs, got in zip(seeds, cands, masks):
+        if not cs:
+            assert got == []
+            continue
+        with slime_amd.BiomeContext(seed, capacity=len(cs)) as ctx:
+            assert got == ctx.filter_levels([c[0] for c in cs], [c[1] for c in cs], *levels,
+                                            flags=3 | slime_amd.ALL_CHUNKS)
+
+
+def test_level_filter_rejects_bad_arguments(gpu):
+    import slime_amd
+    L = slime_amd.lib()
+    with slime_amd.BiomeContext(1, capacity=4) as ctx:
+        a = (C.c_int32 * 8)()
+        f = L.mcgpu_modern_ctx_filter_candidates_levels
+        assert f(ctx._h, a, a, 2, 3, -64, 320, 4, a) == 0
+        assert f(ctx._h, a, a, 2, 3, -64, 320, 0, a) == -1  # step
+        assert f(ctx._h, a, a, 2, 3, 10, 9, 1, a) == -1  # empty range
+        assert f(ctx._h, a, a, 2, 3, 0, 4096, 1, a) == -1  # 4097 heights
+        assert f(ctx._h, a, a, 2, 3, 0, 4095, 1, a) == 0
+        assert f(ctx._h, a, a, 2, 4, -64, 0, 4, a) == -1  # no biome
+        assert f(ctx._h, a, a, 2, 8 | 1, -64, 0, 4, a) == -1
+        assert f(ctx._h, a, a, 5, 3, -64, 0, 4, a) == -1  # > capacity
+        assert f(ctx._h, a, a, 1, 3, -(1 << 24) - 1, 0, 1 << 20, a) == -1
