@@ -112,6 +112,44 @@ def cpu_baseline(gpu_lib):
             "sample": sample, "seconds": dt, "gpu_matches_sample": gpu == result}
 
 
+C1 = (0x123456789ABCDEF0, 2048, 40)  # seed, radius, threshold of the CPU-runnable search
+
+
+def c1_leg():
+    """The reference's CPU-runnable search (radius 2,048 around (0,0), threshold
+    40, every hit returned): the GPU call against the reference SlimeCore on
+    the host's cores, rows compared."""
+    from oracle import pyoracle as P
+    import slime_amd
+    seed, rd, thr = C1
+    k = 65536
+    centers = (2 * rd + 1) ** 2
+    gpu = slime_amd.search_slime_clusters_centered(seed, rd, thr, 221, 0, 0, 0, k)  # warm-up
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        got = slime_amd.search_slime_clusters_centered(seed, rd, thr, 221, 0, 0, 0, k)
+        times.append(time.perf_counter() - t0)
+        if got != gpu:
+            raise SystemExit("C1 results differ between runs")
+    out = {"workload": f"seed {seed:#x}, radius {rd} around (0,0), threshold {thr}, all hits",
+           "centers": centers, "hits": gpu[0], "gpu_ms": statistics.median(times) * 1e3,
+           "gpu_t_centers_per_s": centers / statistics.median(times) / 1e12}
+    ref = P.ref_lib()
+    if ref is not None:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        want = P.ref_search(ref, seed, rd, thr, 221, 0, 0, 0, k, threads)  # warm-up
+        ct = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            want = P.ref_search(ref, seed, rd, thr, 221, 0, 0, 0, k, threads)
+            ct.append(time.perf_counter() - t0)
+        out.update({"reference_cpu_ms": statistics.median(ct) * 1e3, "reference_cores": threads,
+                    "identical": want == gpu})
+    print(f"C1: {gpu[0]} hits, gpu {out['gpu_ms']:.2f} ms", file=sys.stderr, flush=True)
+    return out
+
+
 def batched_leg(repeats):
     """The batched configuration: sweep vs separate calls (same results)."""
     import slime_amd
@@ -502,6 +540,7 @@ def main():
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(lib)
+        out["c1"] = c1_leg()
     if not args.no_batched:
         out["batched"] = batched_leg(2)
     if not args.no_served:

@@ -166,7 +166,8 @@ __device__ __forceinline__ uint32_t group_row(const uint32_t* ring, int first, u
                                               int r) {
     const int gbit = (int)(code >> 8), b = (int)((code >> 1) & 15u), ph = (int)(code & 1u);
     const uint32_t flip = ((b ^ ph) & 1) ? 0u : 0xFFFFFFFFu;  // phase mask is E or ~E
-    const uint32_t* row = ring + ((first + r) % kRing) * Ring<NB>::kRowDw;
+    // first >= 0 wherever a group is refined: an unsigned remainder is cheaper
+    const uint32_t* row = ring + ((uint32_t)(first + r) % (uint32_t)kRing) * Ring<NB>::kRowDw;
     return window20(row + (b + 1) * kPlaneDw, gbit) & (window20(row, gbit) ^ flip);
 }
 
@@ -454,7 +455,10 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
                 if (idx < p.hit_cap) p.hits[idx] = Hit{(int32_t)c, cx, cz, p.tag};
             }
         };
-        refine_wave<NB, UB>(pass, code_of, ring, 4 * m, p.min_size, emit);
+        // a wave-uniform test first: the refinement's per-lane row addresses are
+        // then computed only in the (rare) steps where some lane has a group
+        if (__ballot(pass != 0) != 0)
+            refine_wave<NB, UB>(pass, code_of, ring, 4 * m, p.min_size, emit);
     }
 }
 

@@ -333,13 +333,17 @@ void plan_plain_rows(const SearchSpec& spec, int32_t x0, int64_t w, std::deque<W
 
 // Banded launches over every center of the spec: two x-phases above P
 // columns, one below (x_parts); bands grouped evenly by at most g_max_bands
-// per launch. Tile height: the largest (up to kBandTileH) that still gives
-// every CU a few tiles of the plan; small regions trade halo rows for
-// occupancy.
+// per launch, and at most kUnionBoundMaxBands where the union bound applies
+// (its kernels exist up to that many bands; at thresholds near 50 it beats
+// the square bound on more bands). Tile height: the largest (up to
+// kBandTileH) that still gives every CU a few tiles of the plan; small regions
+// trade halo rows for occupancy.
 void plan_banded(const SearchSpec& spec, std::deque<Work>& out) {
     const std::vector<XPart> parts = x_parts(spec, spec.w > kPeriod ? 2 : 1);
     const int64_t nbz = bands_of(spec);
-    const int64_t ngroups = (nbz + g_max_bands - 1) / g_max_bands;
+    const bool union_bound = union_bound_for(spec.mode, spec.min_size);
+    const int max_bands = union_bound ? std::min(g_max_bands, kUnionBoundMaxBands) : g_max_bands;
+    const int64_t ngroups = (nbz + max_bands - 1) / max_bands;
     std::vector<int> groups;
     for (int64_t gi = 0, left = nbz; gi < ngroups; ++gi) {
         const int64_t nb = (left + (ngroups - gi) - 1) / (ngroups - gi);
@@ -362,7 +366,7 @@ void plan_banded(const SearchSpec& spec, std::deque<Work>& out) {
         wk.banded = true;
         wk.nb = j.nb;
         wk.ph = j.ph;
-        wk.union_bound = union_bound_for(spec.mode, spec.min_size);
+        wk.union_bound = union_bound;
         wk.band = j.p;
         wk.band.tag = 0;
         wk.centers = j.centers;

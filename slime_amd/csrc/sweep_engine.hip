@@ -304,6 +304,10 @@ bool Sweeper::plan(SweepRequest& r) const {
         const int64_t tiles = (int64_t)xp.nblocks * xp.tiles_per_block() * ((s_rows + th - 1) / th);
         return (double)tiles * tile_ns(c, th);
     };
+    // the union bound has kernels for the classes of up to
+    // kUnionBoundMaxBands bands: where it applies, only those cover the bands
+    const bool ub = union_bound_for(spec.mode, spec.min_size);
+    const int ncls = ub ? kSweepUnionClasses : kSweepClasses;
     // cheapest cover of bands [b0, b0 + n) over s_rows slot rows
     auto cover = [&](const XPart& xp, int64_t b0, int64_t n, int32_t s_rows,
                      std::vector<Group>& out) {
@@ -311,7 +315,7 @@ bool Sweeper::plan(SweepRequest& r) const {
         std::vector<int> pick(n + 1, 0);
         cost[0] = 0;
         for (int64_t k = 1; k <= n; ++k)
-            for (int c = 0; c < kSweepClasses; ++c) {
+            for (int c = 0; c < ncls; ++c) {
                 const int nb = kSweepClass[c].nb;
                 if (kSweepClass[c].ph != xp.ph || nb > k || cost[k - nb] < 0) continue;
                 const double v = cost[k - nb] + group_cost(xp, nb, s_rows);
@@ -379,7 +383,6 @@ bool Sweeper::plan(SweepRequest& r) const {
                            int64_t(1) << 62, r.jobs);
     r.job_cls.clear();
     r.centers = 0;
-    const bool ub = union_bound_for(spec.mode, spec.min_size);
     for (const BandJob& j : r.jobs) {
         const int c = sweep_class_of(j.nb, j.ph);
         if (c < 0) return false;

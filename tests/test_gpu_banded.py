@@ -174,7 +174,7 @@ def test_full_width_search_hits_recount(gpu, oracle):
 
 
 @pytest.mark.parametrize("radius,thr", [(3000, 38), (70000, 50), (140000, 52), (200000, 55),
-                                        (150000, 60)])
+                                        (150000, 60), (700000, 53)])
 def test_union_and_square_bounds_agree(gpu, radius, thr):
     """The rolling union-of-disks bound and the square bound send different
     groups to refinement but must give identical results (and equal the plain
