@@ -363,8 +363,9 @@ def percentile(xs, q):
     return xs[i]
 
 
-def served_leg(n, rate):
-    """The served configuration: one session vs one search at a time."""
+def served_leg(n, rate, with_plain=True):
+    """The served configuration: one session vs one search at a time (through
+    the banded per-search engine, and through the plain per-chunk kernel)."""
     import threading
     import slime_amd
     from slime_amd.workloads import served_requests, SERVED_STATE
@@ -406,37 +407,59 @@ def served_leg(n, rate):
     done = [i for i in results]
     centers = sum((2 * reqs[i]["search"][1] + 1) ** 2 for i in done)
     print(f"served: {len(done)} requests in {served_wall:.2f} s", file=sys.stderr, flush=True)
-    # sequential: one search at a time in arrival order (cancelled ones skipped,
-    # a paused one holds the device for the pause time)
-    seq_lat, seq_res = {}, {}
-    t0 = time.perf_counter()
-    for i, r in enumerate(reqs):
-        now = time.perf_counter() - t0
-        if now < r["arrival"]:
-            time.sleep(r["arrival"] - now)
-        if r["action"] == "cancel":
-            continue
-        if r["action"] == "pause":
-            time.sleep(0.05)
-        seq_res[i] = slime_amd.search_slime_clusters_gpu_v1_centered(*r["search"], max_results=k)
-        seq_lat[i] = time.perf_counter() - t0 - r["arrival"]
-    seq_wall = time.perf_counter() - t0
-    print(f"sequential: {len(seq_res)} requests in {seq_wall:.2f} s", file=sys.stderr, flush=True)
-    seq_centers = sum((2 * reqs[i]["search"][1] + 1) ** 2 for i in seq_res)
-    ident = all(results[i] == seq_res.get(i) for i in done) and set(done) == set(seq_res)
-    return {
+    def sequential(run, name):
+        """One search at a time in arrival order (cancelled ones skipped, a
+        paused one holds the device for the pause time)."""
+        seq_lat, seq_res = {}, {}
+        t0 = time.perf_counter()
+        for i, r in enumerate(reqs):
+            now = time.perf_counter() - t0
+            if now < r["arrival"]:
+                time.sleep(r["arrival"] - now)
+            if r["action"] == "cancel":
+                continue
+            if r["action"] == "pause":
+                time.sleep(0.05)
+            seq_res[i] = run(r["search"])
+            seq_lat[i] = time.perf_counter() - t0 - r["arrival"]
+        seq_wall = time.perf_counter() - t0
+        print(f"{name}: {len(seq_res)} requests in {seq_wall:.2f} s", file=sys.stderr, flush=True)
+        seq_centers = sum((2 * reqs[i]["search"][1] + 1) ** 2 for i in seq_res)
+        ident = all(results[i] == seq_res.get(i) for i in done) and set(done) == set(seq_res)
+        return {"wall_s": seq_wall, "t_centers_per_s": seq_centers / seq_wall / 1e12,
+                "p50_s": percentile(list(seq_lat.values()), 50),
+                "p99_s": percentile(list(seq_lat.values()), 99)}, ident
+
+    seq, ident = sequential(
+        lambda s: slime_amd.search_slime_clusters_gpu_v1_centered(*s, max_results=k), "sequential")
+
+    def plain_search(s):
+        seed, rd, thr, mx, _, cx, cz = s
+        tot, rows, _ = slime_amd.scan_bands(seed, cx - rd, cz - rd, 2 * rd + 1, 2 * rd + 1, thr,
+                                            mx, 0, slime_amd.MODE_PLAIN, k)
+        return tot, rows
+
+    seq_plain, ident_plain = (None, True) if not with_plain else sequential(
+        plain_search, "sequential plain kernel")
+    served = {"wall_s": served_wall, "t_centers_per_s": centers / served_wall / 1e12,
+              "p50_s": percentile(list(lat.values()), 50),
+              "p99_s": percentile(list(lat.values()), 99),
+              "kernel_s": stats["kernel_ns"] / 1e9, "rounds": stats["rounds"]}
+    out = {
         "workload": "Poisson arrivals: 70% radius 256..4096, 25% radius 50000, 5% full world; "
                     "thr 50..65; 10% pause+resume, 5% cancel",
         "state": hex(SERVED_STATE), "requests": n, "rate_per_s": rate, "completed": len(done),
-        "served": {"wall_s": served_wall, "t_centers_per_s": centers / served_wall / 1e12,
-                   "p50_s": percentile(list(lat.values()), 50),
-                   "p99_s": percentile(list(lat.values()), 99),
-                   "kernel_s": stats["kernel_ns"] / 1e9, "rounds": stats["rounds"]},
-        "sequential": {"wall_s": seq_wall, "t_centers_per_s": seq_centers / seq_wall / 1e12,
-                       "p50_s": percentile(list(seq_lat.values()), 50),
-                       "p99_s": percentile(list(seq_lat.values()), 99)},
-        "identical": ident,
+        "served": served,
+        "sequential": seq,
+        "speedup_vs_sequential": served["t_centers_per_s"] / seq["t_centers_per_s"],
+        "identical": ident and ident_plain,
     }
+    if seq_plain is not None:
+        # the same arrivals through the plain per-chunk kernel, one at a time
+        out["sequential_plain_kernel"] = seq_plain
+        out["speedup_vs_sequential_plain_kernel"] = (served["t_centers_per_s"] /
+                                                     seq_plain["t_centers_per_s"])
+    return out
 
 
 def main():
@@ -455,6 +478,8 @@ def main():
     ap.add_argument("--only-batched", action="store_true",
                     help="run only the batched leg (for profiling its kernels)")
     ap.add_argument("--no-served", action="store_true", help="skip the served leg")
+    ap.add_argument("--no-served-plain", action="store_true",
+                    help="skip the served leg's one-at-a-time plain-kernel baseline")
     ap.add_argument("--served-requests", type=int, default=80)
     ap.add_argument("--served-rate", type=float, default=6.0, help="arrivals per second")
     args = ap.parse_args()
@@ -545,7 +570,8 @@ def main():
     if not args.no_batched:
         out["batched"] = batched_leg(2)
     if not args.no_served:
-        out["served"] = served_leg(args.served_requests, args.served_rate)
+        out["served"] = served_leg(args.served_requests, args.served_rate,
+                                   not args.no_served_plain)
     print(json.dumps(out), flush=True)
     return 0
 
