@@ -193,6 +193,31 @@ def test_union_and_square_bounds_agree(gpu, radius, thr):
         assert u[0] == p[0] and u[1] == p[1] and u[2] == p[2]
 
 
+@pytest.mark.parametrize("radius,corner", [(70000, 0), (70000, 1), (200000, 2), (200000, 3)])
+def test_banded_at_int32_edges(gpu, radius, corner):
+    """Squares touching the int32 corners of the coordinate space (centers at
+    the largest valid offsets, whose disks reach +-(2^31 - 1)): the banded
+    kernels, whose phase-1 columns and halo rows run past the edge, against
+    the plain kernel; both against the search call."""
+    import slime_amd
+    seed = s64(0x5EEDC0DE * (radius + corner))
+    lo, hi = -(1 << 31) + 8 + radius, INT32_MAX - 8 - radius
+    cx = hi if corner & 1 else lo
+    cz = hi if corner & 2 else lo
+    w = 2 * radius + 1
+    print("edge", radius, cx, cz, flush=True)
+    b = slime_amd.scan_bands(seed, cx - radius, cz - radius, w, w, 44, 221, 0,
+                             slime_amd.MODE_BANDED, 500)
+    p = slime_amd.scan_bands(seed, cx - radius, cz - radius, w, w, 44, 221, 0,
+                             slime_amd.MODE_PLAIN, 500)
+    assert b[0] == p[0] and b[1] == p[1] and b[2] == p[2]
+    assert b[0] > 0
+    total, rows = slime_amd.search_slime_clusters_gpu_v1_centered(seed, radius, 44, 221, 0, cx,
+                                                                  cz, 500)
+    assert total == b[0]
+    assert [r[0] for r in rows] == [r[0] for r in b[1]]
+
+
 @pytest.mark.parametrize("nb", [1, 2, 4, 8, 9])
 def test_union_bound_band_sets_vs_oracle(gpu, oracle, nb):
     """Strided band sets through the union-bound kernel (9 bands: past the
