@@ -12,9 +12,11 @@
 //   4. a group whose square reaches min_size gets the exact 221-cell count of
 //      its 16 centers from the ring (SlimeCore.cpp:526-533), which feeds the
 //      score histogram and the hit buffer.
-// The ring holds first-draw bits (next(31) % 10 == 0, the slot arithmetic of
-// slime_math.h for one phase and one band); the engine recounts the few
-// centers near a chunk whose first draw Java rejects.
+// The ring holds first-draw bits (next(31) % 10 == 0), computed from Java's
+// LCG directly (first_y of the 48-bit seed sum), with none of the banded
+// kernels' phase/band arithmetic: an independent second path for the parity
+// tests. The engine recounts the few centers near a chunk whose first draw
+// Java rejects.
 #include "scan.h"
 #include "slime_math.h"
 
@@ -81,13 +83,10 @@ __global__ __launch_bounds__(kThreads) void scan_plain_kernel(ScanParams p) {
     const uint32_t chunk_x0 = (uint32_t)p.x0 + (uint32_t)off_x - 8u;
     const uint32_t chunk_z0 = (uint32_t)p.z0 + (uint32_t)off_z - 8u;
 
-    uint32_t xlo[kWords], xh[kWords];
+    uint64_t xs[kWords];  // seed + xt(x) of the lane's chunk in each row word
 #pragma unroll
-    for (int j = 0; j < kWords; ++j) {
-        const uint64_t X = (uint64_t)p.seed + x_term((int32_t)(chunk_x0 + 64u * j + (uint32_t)lane));
-        xlo[j] = (uint32_t)X << 16;
-        xh[j] = (uint32_t)(X >> 16);
-    }
+    for (int j = 0; j < kWords; ++j)
+        xs[j] = (uint64_t)p.seed + x_term((int32_t)(chunk_x0 + 64u * j + (uint32_t)lane));
 
     const int g = tid;
     const int gbit = 4 * g;
@@ -98,13 +97,10 @@ __global__ __launch_bounds__(kThreads) void scan_plain_kernel(ScanParams p) {
     for (int k = 0; k < nbatch; ++k) {
         const int row = 4 * k + wave;
         const uint64_t Z = z_term((int32_t)(chunk_z0 + (uint32_t)row));
-        const uint32_t zlo = (uint32_t)Z << 16, zhi = (uint32_t)(Z >> 16);
         uint32_t mine = 0;
 #pragma unroll
         for (int j = 0; j < kWords; ++j) {
-            // one phase, one band: a hit needs the even parity and the fifth test
-            const BandSlot sl = band_slot(xlo[j], xh[j], xh[j], zlo, zhi);
-            const bool hit = !sl.e && band_value(sl, 0, zhi) <= kFifthLimit;
+            const bool hit = first_draw_hit(first_y(xs[j] + Z));
             const uint64_t m = __ballot(hit);
             mine = lane == 2 * j ? (uint32_t)m : mine;
             mine = lane == 2 * j + 1 ? (uint32_t)(m >> 32) : mine;

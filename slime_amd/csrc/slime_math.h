@@ -98,10 +98,11 @@ constexpr uint32_t kMulInv5 = kJavaMulLo * kInv5;  // M * 5^-1 (mod 2^32)
 constexpr uint32_t kFifthLimit = 858993459u;       // (2^32 - 1) / 5
 
 struct BandSlot {
-    uint32_t xa;    // high half of the phase even in even bands (carry included)
-    uint32_t xb;    // the other phase
-    uint32_t bias;  // u + (w - (Y & 1)) * 5^-1
-    bool e;         // first-draw parity of phase 0, band 0
+    uint32_t xa;     // high half of the phase even in even bands (no carry)
+    uint32_t xb;     // the other phase
+    uint32_t carry;  // carry out of the low half, common to all chunks of the slot
+    uint32_t bias;   // u + (w - (Y & 1)) * 5^-1
+    bool e;          // first-draw parity of phase 0, band 0
 };
 
 // xlo = low 16 bits of X0 in the top half; xh0/xh1 = bits 16..47 of X0/X1;
@@ -111,14 +112,16 @@ SLIME_HD BandSlot band_slot(uint32_t xlo, uint32_t xh0, uint32_t xh1, uint32_t z
     const uint64_t lsum = (uint64_t)xlo + zlo;
     const uint32_t carry = (uint32_t)(lsum >> 32);
     const uint32_t u = (uint32_t)lsum ^ kLowScramble;  // low 16 state bits << 16
-    const uint32_t b0 = xh0 + carry, b1 = xh1 + carry;
     // w = ((low16 * 0xDEECE66D + 11) >> 16), the low half's share of Y
     const uint32_t w = (uint32_t)(((uint64_t)u * kJavaMulLo + (11u << 16)) >> 32);
-    const uint32_t t = b0 + zhi0 + w;  // Y (phase 0, band 0) mod 4 == t mod 4
+    const uint32_t t = xh0 + carry + zhi0 + w;  // Y (phase 0, band 0) mod 4 == t mod 4
     BandSlot s;
     s.e = (t & 2u) != 0;
-    s.xa = s.e ? b1 : b0;
-    s.xb = s.e ? b0 : b1;
+    // the phases are selected before the carry is added: the carry joins each
+    // band's sum (one three-operand add) instead of costing an add per phase
+    s.xa = s.e ? xh1 : xh0;
+    s.xb = s.e ? xh0 : xh1;
+    s.carry = carry;
     s.bias = u + (w - (t & 1u)) * kInv5;
     return s;
 }
@@ -126,7 +129,7 @@ SLIME_HD BandSlot band_slot(uint32_t xlo, uint32_t xh0, uint32_t xh1, uint32_t z
 // (Y - (Y & 1)) * 5^-1 of band b's tested chunk; next(31) % 10 == 0 iff the
 // result is <= kFifthLimit.
 SLIME_HD uint32_t band_value(const BandSlot& s, int b, uint32_t zhi_b) {
-    const uint32_t a = (((b & 1) ? s.xb : s.xa) + zhi_b) ^ kHighScramble;
+    const uint32_t a = (((b & 1) ? s.xb : s.xa) + zhi_b + s.carry) ^ kHighScramble;
     return a * kMulInv5 + s.bias;
 }
 
