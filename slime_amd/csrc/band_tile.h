@@ -313,6 +313,8 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
     const int ph_lo = C::kSplitPhase ? half : 0;
     const int gbit = 4 * g;
     const bool counts = g < kGroupCols;
+    // E (not ~E) selects phase ph_lo of band b_lo when b_lo ^ ph_lo is odd
+    const uint32_t flip0 = ((b_lo ^ ph_lo) & 1) ? 0u : 0xFFFFFu;
     // Rolling state per owned band-phase q = kPhases*(b - b_lo) + (ph - ph_lo):
     // sq = square | (strip leaving next) << 16; hist = last four strips, a byte each.
     uint32_t sq[C::kOwned], hist[C::kOwned];
@@ -387,18 +389,24 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const uint32_t* rowp = ring + ((4 * k + r) % kRing) * R::kRowDw;
-                const uint32_t ew = window20(rowp, gbit);
-                const uint32_t e0 = ~ew & 0xFFFFFu;  // slots with e == 0
+                // phase ph of band b lies where e ^ (b & 1) == ph: the slots of
+                // E for (b ^ ph) odd, of ~E otherwise. `base` is that mask for
+                // (b_lo, ph_lo); the other parities use its complement.
+                const uint32_t base = window20(rowp, gbit) ^ flip0;
+                const uint32_t nbase = base ^ 0xFFFFFu;
 #pragma unroll
                 for (int hb = 0; hb < C::kBands; ++hb) {
                     const int b = b_lo + hb;
                     if (b >= NB) continue;  // odd NB: the second half is one band short
                     const uint32_t hw = window20(rowp + (b + 1) * kPlaneDw, gbit);
+                    uint32_t w_first = 0;
 #pragma unroll
                     for (int pp = 0; pp < C::kPhases; ++pp) {
-                        // phase ph of band b: where e ^ (b & 1) == ph
-                        const bool use_e = ((b ^ (ph_lo + pp)) & 1) != 0;
-                        const uint32_t w = hw & (use_e ? ew : e0);
+                        // the two phases of a band split its hits: the second is
+                        // what the first leaves
+                        const uint32_t w =
+                            pp == 0 ? hw & (((hb ^ pp) & 1) ? nbase : base) : hw ^ w_first;
+                        w_first = w;
                         const int q = C::kPhases * hb + pp;
                         c4[q] += __popc(w);
                         if constexpr (UB) {
