@@ -28,8 +28,14 @@
 // (reference: x_phase.py:185-252 and benchmarks/algorithm-optimizations.en.md
 // describe the same 2^17 parity relation for 2 X phases and up to 15 Z bands.)
 //
-// Counting: per band-phase a rolling 20x20 square of each 4x4 group; the
-// groups that reach min_size are refined by the whole wave together (a bound
+// Counting: per band-phase a rolling 20x20 square of each 4x4 group. With the
+// square bound and at least four band-phases, threads count plane words: the
+// 20-row counts of 4-column blocks as bytes (nibble popcounts, 4-row strips,
+// five strips kept), and the squares of the 8 groups whose windows start in
+// the word as sums of 5 blocks in 16-bit lanes. Otherwise (few band-phases,
+// or the union bound with its corner counts) a thread counts the 20-bit
+// windows of its groups row by row. The groups that reach min_size are
+// refined by the whole wave together (a bound
 // on the union of the group's 16 disks, 20 lanes per group, then the exact
 // 221-cell counts, 4 lanes per center), so a rare passing group costs the
 // wave a few instructions instead of a long divergent run of one lane.
@@ -57,6 +63,19 @@ static_assert(kBandTileOut + 16 == kRowBits, "tile row = centers + halo");
 __device__ __forceinline__ uint32_t window20(const uint32_t* row, int bit) {
     const int d = bit >> 5;
     return __builtin_amdgcn_alignbit(row[d + 1], row[d], bit & 31) & 0xFFFFFu;
+}
+
+// Per-nibble popcounts: nibble j of the result counts the set bits of nibble
+// j of x.
+__device__ __forceinline__ uint32_t nibble_counts(uint32_t x) {
+    x = x - ((x >> 1) & 0x55555555u);
+    return (x & 0x33333333u) + ((x >> 2) & 0x33333333u);
+}
+
+// v of the next lane in the lane's row of 16 (DPP row_shl:1); the last lane of
+// a row reads 0.
+__device__ __forceinline__ uint32_t from_next_lane16(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x101, 0xF, 0xF, true);
 }
 
 // Compile-time loop: f(std::integral_constant<int, I>) for I in [I0, N).
@@ -325,6 +344,51 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
 #pragma unroll
     for (int q = 0; q < (UB ? C::kOwned : 1); ++q) pend[q] = 0;
 
+    // Square bound with at least four band-phases: counting by plane words
+    // (fewer instructions per band than by groups; with fewer band-phases the
+    // work would sit on too few threads). Thread t takes band
+    // wb = t / 16 over plane word wd = t % 16, i.e. the 8 nibble columns
+    // 8 wd .. 8 wd + 7 of the row and the windows (5 nibble columns each) of
+    // the groups 8 wd + i, for all its phases. Per phase it keeps the 20-row
+    // counts of its nibble columns as bytes (even columns in `we`, odd in
+    // `wo`, at most 80 each) and its last five 4-row strips in the same form;
+    // the next word's counts come from the next lane of the row of 16.
+    // With two phases and at most 8 bands a thread takes one phase (wph) of
+    // its band, so that 16 x NB x 2 threads share the work.
+    constexpr bool kWord = !UB && NB * PH >= 4;
+    constexpr bool kWSplit = PH == 2 && NB <= 8;
+    const int wd = tid & 15;
+    const int wb = kWSplit ? tid >> 5 : tid >> 4;
+    const int wph = kWSplit ? (tid >> 4) & 1 : 0;
+    const bool w_active = wb < NB;
+    const int wbc = w_active ? wb : 0;  // idle threads read a valid plane
+    // phase 0 of band b lies where e == (b & 1): E for odd b, ~E for even b;
+    // a thread of phase 1 counts the rest
+    const uint32_t wflip = ((wbc ^ wph) & 1) ? 0u : 0xFFFFFFFFu;
+    constexpr int kWPh = !kWord || kWSplit ? 1 : PH;
+    uint32_t we[kWPh], wo[kWPh], he[kWPh][5], ho[kWPh][5];
+    uint32_t wvalid[kWPh];  // bit i: group 8 wd + i has a center inside the job
+#pragma unroll
+    for (int ph = 0; ph < kWPh; ++ph) {
+        we[ph] = wo[ph] = 0;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) he[ph][j] = ho[ph][j] = 0;
+        wvalid[ph] = 0;
+        if constexpr (kWord) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int gi = 8 * wd + i;
+                const int64_t ox0 = block_off + (int64_t)(ph + wph) * kPeriod + col0 + 4 * gi;
+                if (gi < kGroupCols && col0 + 4 * gi < p.phase_cols && ox0 < p.w &&
+                    ox0 + 3 >= p.ox_min)
+                    wvalid[ph] |= 1u << i;
+            }
+        }
+    }
+    // 0x8000 - min_size in both 16-bit lanes: bit 15 / 31 of square + this is
+    // set when the lane's square reaches min_size
+    const uint32_t wbias = (uint32_t)(0x8000 - p.min_size) * 0x10001u;
+
     const int32_t tile_s0 = p.s_begin + ts * p.tile_h;
     const int nbatch = p.tile_h / 4 + 4;
     // kept in a register: the asm compare takes its limit as a VGPR operand
@@ -377,7 +441,61 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
         const int m = k - 4;
         const int32_t s_g = tile_s0 + 4 * m;  // first slot row of the groups
         uint32_t pass = 0;  // owned band-phases whose square reaches min_size
-        if (counts) {
+        if constexpr (kWord) {
+            // every thread takes part (the next-lane reads need whole rows)
+            uint32_t nsum[kWPh], nlast[kWPh];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t* rowp = ring + ((4 * k + r) % kRing) * R::kRowDw;
+                const uint32_t e = rowp[wd];
+                const uint32_t h = rowp[(wbc + 1) * kPlaneDw + wd];
+                uint32_t c[kWPh];
+                c[0] = nibble_counts(h & (e ^ wflip));
+                if constexpr (kWPh == 2) c[1] = nibble_counts(h) - c[0];  // the rest
+#pragma unroll
+                for (int ph = 0; ph < kWPh; ++ph) {
+                    if (r == 0) nsum[ph] = c[ph];
+                    else if (r < 3) nsum[ph] += c[ph];  // three rows: at most 12 a nibble
+                    else nlast[ph] = c[ph];
+                }
+            }
+#pragma unroll
+            for (int ph = 0; ph < kWPh; ++ph) {
+                // the strip's counts as bytes: even nibble columns, odd ones
+                const uint32_t se = (nsum[ph] & 0x0F0F0F0Fu) + (nlast[ph] & 0x0F0F0F0Fu);
+                const uint32_t so =
+                    ((nsum[ph] >> 4) & 0x0F0F0F0Fu) + ((nlast[ph] >> 4) & 0x0F0F0F0Fu);
+                we[ph] += se - he[ph][0];
+                wo[ph] += so - ho[ph][0];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    he[ph][j] = he[ph][j + 1];
+                    ho[ph][j] = ho[ph][j + 1];
+                }
+                he[ph][4] = se;
+                ho[ph][4] = so;
+                // nibble columns n = 8 wd + j as 16-bit lanes: W_s holds
+                // (n_s, n_{s+4}), N_s the next word's (n_{s+8}, n_{s+12})
+                const uint32_t ne = from_next_lane16(we[ph]), no = from_next_lane16(wo[ph]);
+                const uint32_t w0 = we[ph] & 0x00FF00FFu, w2 = (we[ph] >> 8) & 0x00FF00FFu;
+                const uint32_t w1 = wo[ph] & 0x00FF00FFu, w3 = (wo[ph] >> 8) & 0x00FF00FFu;
+                const uint32_t n0 = ne & 0x00FF00FFu, n2 = (ne >> 8) & 0x00FF00FFu;
+                const uint32_t n1 = no & 0x00FF00FFu, n3 = (no >> 8) & 0x00FF00FFu;
+                // square of group s (lane 0) and s + 4 (lane 1): 5 nibble columns
+                const uint32_t q0 = w0 + w1 + w2 + w3 + __builtin_amdgcn_alignbit(n0, w0, 16);
+                const uint32_t q1 = q0 - w0 + __builtin_amdgcn_alignbit(n1, w1, 16);
+                const uint32_t q2 = q1 - w1 + __builtin_amdgcn_alignbit(n2, w2, 16);
+                const uint32_t q3 = q2 - w2 + __builtin_amdgcn_alignbit(n3, w3, 16);
+                const uint32_t c0 = q0 + wbias, c1 = q1 + wbias, c2 = q2 + wbias,
+                               c3 = q3 + wbias;
+                const uint32_t pm = ((c0 >> 15) & 1u) | ((c1 >> 14) & 2u) | ((c2 >> 13) & 4u) |
+                                    ((c3 >> 12) & 8u) | ((c0 >> 27) & 16u) | ((c1 >> 26) & 32u) |
+                                    ((c2 >> 25) & 64u) | ((c3 >> 24) & 128u);
+                pass |= (pm & wvalid[ph]) << (8 * ph);
+            }
+            if (!w_active || m < 0 || s_g >= p.s_end || (int64_t)wb * kPeriod + s_g >= p.h_valid)
+                pass = 0;
+        } else if (counts) {
             uint32_t c4[C::kOwned];
             // UB: this strip's corner cells at offsets 16+r (now) and r (in
             // four steps)
@@ -441,7 +559,10 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
         }
         // ---- refinement by the whole wave ----------------------------------
         auto code_of = [&](int q) {
-            return group_code(gbit, b_lo + q / C::kPhases, ph_lo + q % C::kPhases);
+            if constexpr (kWord)  // bit 8 ph + i: group 8 wd + i, band wb, phase wph + ph
+                return group_code(4 * (8 * wd + (q & 7)), wb, wph + (q >> 3));
+            else
+                return group_code(gbit, b_lo + q / C::kPhases, ph_lo + q % C::kPhases);
         };
         auto emit = [&](uint32_t code, int i, int jr, uint32_t c) {
             const int gb = (int)(code >> 8), b = (int)((code >> 1) & 15u), ph = (int)(code & 1u);
