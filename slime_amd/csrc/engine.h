@@ -34,7 +34,7 @@ enum class ScanMode : int32_t {
 // The bound the banded scan uses for a threshold when the mode does not force
 // one: the union of disks at thresholds up to kUnionBoundMaxMin, where many
 // 4x4 groups pass the square.
-constexpr int32_t kUnionBoundMaxMin = 55;
+constexpr int32_t kUnionBoundMaxMin = 54;
 inline bool union_bound_for(ScanMode mode, int32_t min_size) {
     if (mode == ScanMode::kBandedSquare) return false;
     if (mode == ScanMode::kBandedUnion) return true;
