@@ -67,8 +67,9 @@ def test_batch_graph_replay_and_invalid_entries(gpu):
     searches[7] = (1, 10, 50, 40, 0, 0, 0)         # max_size < min_size
     plain = run_batch(searches, 200)
     assert plain[3][0] == -1 and plain[7][0] == -1
-    for _ in range(2):
+    for _ in range(3):  # the graph is captured once and replayed, with eager batches between
         assert run_batch(searches, 200, use_graph=True) == plain
+        assert run_batch(searches, 200) == plain
 
 
 def test_batch_rejected_chunks(gpu, oracle):
