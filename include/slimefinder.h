@@ -158,8 +158,8 @@ int64_t slime_scan_rect(int64_t seed, int32_t x0, int32_t z0, int64_t w, int64_t
  * z0 + b*2^17 + s (s < band_rows), with a forced scan mode: 0 automatic,
  * 1 plain per-chunk scan, 2 banded scan (one x-phase up to 2^17 columns,
  * two above), 3 banded with the square bound before refinement, 4 banded
- * with the rolling union-of-disks bound (modes 0 and 2 take the union bound
- * at min_size <= 54). Lets tests and callers pick the kernel explicitly. */
+ * with the union bound (the square minus cells outside the union of disks;
+ * modes 0 and 2 take it). Lets tests and callers pick the kernel explicitly. */
 int64_t slime_scan_bands(int64_t seed, int32_t x0, int32_t z0, int64_t w, int64_t h,
                          int32_t band_rows, int32_t mode, int32_t min_size, int32_t max_size,
                          ExtChunkResult* results_buffer, int32_t max_results, int64_t* hist222);

@@ -355,7 +355,12 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
     // the next word's counts come from the next lane of the row of 16.
     // With two phases and at most 8 bands a thread takes one phase (wph) of
     // its band, so that 16 x NB x 2 threads share the work.
-    constexpr bool kWord = !UB && NB * PH >= 4;
+    constexpr bool kWord = NB * PH >= 4;
+    // With the union bound, word counting takes a bound between the square and
+    // the union of disks: the square minus its first and last 4-column block
+    // in window rows 0, 1, 18 and 19 (32 cells, all outside the union; 368
+    // remain). Groups that pass it are refined as with the square bound.
+    constexpr bool kCorner = UB && kWord;
     constexpr bool kWSplit = PH == 2 && NB <= 8;
     const int wd = tid & 15;
     const int wb = kWSplit ? tid >> 5 : tid >> 4;
@@ -367,6 +372,13 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
     const uint32_t wflip = ((wbc ^ wph) & 1) ? 0u : 0xFFFFFFFFu;
     constexpr int kWPh = !kWord || kWSplit ? 1 : PH;
     uint32_t we[kWPh], wo[kWPh], he[kWPh][5], ho[kWPh][5];
+    // kCorner: per-nibble counts of rows 0 and 1 of the last four strips
+    // (at most 8 a nibble), [0] oldest
+    uint32_t hq[kCorner ? kWPh : 1][4];
+#pragma unroll
+    for (int ph = 0; ph < (kCorner ? kWPh : 1); ++ph)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) hq[ph][j] = 0;
     uint32_t wvalid[kWPh];  // bit i: group 8 wd + i has a center inside the job
 #pragma unroll
     for (int ph = 0; ph < kWPh; ++ph) {
@@ -443,7 +455,7 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
         uint32_t pass = 0;  // owned band-phases whose square reaches min_size
         if constexpr (kWord) {
             // every thread takes part (the next-lane reads need whole rows)
-            uint32_t nsum[kWPh], nlast[kWPh];
+            uint32_t nhead[kWPh], ntail[kWPh];  // rows 0-1 and 2-3 of the strip
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const uint32_t* rowp = ring + ((4 * k + r) % kRing) * R::kRowDw;
@@ -453,18 +465,19 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
                 c[0] = nibble_counts(h & (e ^ wflip));
                 if constexpr (kWPh == 2) c[1] = nibble_counts(h) - c[0];  // the rest
 #pragma unroll
-                for (int ph = 0; ph < kWPh; ++ph) {
-                    if (r == 0) nsum[ph] = c[ph];
-                    else if (r < 3) nsum[ph] += c[ph];  // three rows: at most 12 a nibble
-                    else nlast[ph] = c[ph];
+                for (int ph = 0; ph < kWPh; ++ph) {  // two rows: at most 8 a nibble
+                    if (r == 0) nhead[ph] = c[ph];
+                    else if (r == 1) nhead[ph] += c[ph];
+                    else if (r == 2) ntail[ph] = c[ph];
+                    else ntail[ph] += c[ph];
                 }
             }
 #pragma unroll
             for (int ph = 0; ph < kWPh; ++ph) {
                 // the strip's counts as bytes: even nibble columns, odd ones
-                const uint32_t se = (nsum[ph] & 0x0F0F0F0Fu) + (nlast[ph] & 0x0F0F0F0Fu);
+                const uint32_t se = (nhead[ph] & 0x0F0F0F0Fu) + (ntail[ph] & 0x0F0F0F0Fu);
                 const uint32_t so =
-                    ((nsum[ph] >> 4) & 0x0F0F0F0Fu) + ((nlast[ph] >> 4) & 0x0F0F0F0Fu);
+                    ((nhead[ph] >> 4) & 0x0F0F0F0Fu) + ((ntail[ph] >> 4) & 0x0F0F0F0Fu);
                 we[ph] += se - he[ph][0];
                 wo[ph] += so - ho[ph][0];
 #pragma unroll
@@ -482,10 +495,29 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
                 const uint32_t n0 = ne & 0x00FF00FFu, n2 = (ne >> 8) & 0x00FF00FFu;
                 const uint32_t n1 = no & 0x00FF00FFu, n3 = (no >> 8) & 0x00FF00FFu;
                 // square of group s (lane 0) and s + 4 (lane 1): 5 nibble columns
-                const uint32_t q0 = w0 + w1 + w2 + w3 + __builtin_amdgcn_alignbit(n0, w0, 16);
-                const uint32_t q1 = q0 - w0 + __builtin_amdgcn_alignbit(n1, w1, 16);
-                const uint32_t q2 = q1 - w1 + __builtin_amdgcn_alignbit(n2, w2, 16);
-                const uint32_t q3 = q2 - w2 + __builtin_amdgcn_alignbit(n3, w3, 16);
+                uint32_t q0 = w0 + w1 + w2 + w3 + __builtin_amdgcn_alignbit(n0, w0, 16);
+                uint32_t q1 = q0 - w0 + __builtin_amdgcn_alignbit(n1, w1, 16);
+                uint32_t q2 = q1 - w1 + __builtin_amdgcn_alignbit(n2, w2, 16);
+                uint32_t q3 = q2 - w2 + __builtin_amdgcn_alignbit(n3, w3, 16);
+                if constexpr (kCorner) {
+                    // window rows 18, 19 (this strip's rows 2, 3) and 0, 1 (rows
+                    // 0, 1 of the strip four steps back), per nibble column as
+                    // bytes; a group loses its first and last column's counts
+                    const uint32_t old = hq[ph][0];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) hq[ph][j] = hq[ph][j + 1];
+                    hq[ph][3] = nhead[ph];
+                    const uint32_t pe = (ntail[ph] & 0x0F0F0F0Fu) + (old & 0x0F0F0F0Fu);
+                    const uint32_t po =
+                        ((ntail[ph] >> 4) & 0x0F0F0F0Fu) + ((old >> 4) & 0x0F0F0F0Fu);
+                    const uint32_t npe = from_next_lane16(pe), npo = from_next_lane16(po);
+                    const uint32_t p0 = pe & 0x00FF00FFu, p2 = (pe >> 8) & 0x00FF00FFu;
+                    const uint32_t p1 = po & 0x00FF00FFu, p3 = (po >> 8) & 0x00FF00FFu;
+                    q0 -= p0 + __builtin_amdgcn_alignbit(npe & 0x00FF00FFu, p0, 16);
+                    q1 -= p1 + __builtin_amdgcn_alignbit(npo & 0x00FF00FFu, p1, 16);
+                    q2 -= p2 + __builtin_amdgcn_alignbit((npe >> 8) & 0x00FF00FFu, p2, 16);
+                    q3 -= p3 + __builtin_amdgcn_alignbit((npo >> 8) & 0x00FF00FFu, p3, 16);
+                }
                 const uint32_t c0 = q0 + wbias, c1 = q1 + wbias, c2 = q2 + wbias,
                                c3 = q3 + wbias;
                 const uint32_t pm = ((c0 >> 15) & 1u) | ((c1 >> 14) & 2u) | ((c2 >> 13) & 4u) |
@@ -587,7 +619,7 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
         // a wave-uniform test first: the refinement's per-lane row addresses are
         // then computed only in the (rare) steps where some lane has a group
         if (__ballot(pass != 0) != 0)
-            refine_wave<NB, UB>(pass, code_of, ring, 4 * m, p.min_size, emit);
+            refine_wave<NB, UB && !kWord>(pass, code_of, ring, 4 * m, p.min_size, emit);
     }
 }
 

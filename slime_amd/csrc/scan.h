@@ -84,7 +84,11 @@ struct BandParams {
 // kUnionBoundMaxBands; the square bound otherwise).
 hipError_t launch_scan_band(const BandParams& p, int nb, int ph, bool union_bound,
                             hipStream_t stream);
-constexpr int kUnionBoundMaxBands = 8;  // more bands spill with the union state
+// The union-bound kernels exist up to 8 bands: there a thread counts one
+// phase of one band, so all four waves share the words; with more bands a
+// thread takes both phases and the full world's 10-band launches ran slower
+// (1.73 s) than 8-band ones (1.56 s).
+constexpr int kUnionBoundMaxBands = 8;
 
 // ---- multi-search sweep -----------------------------------------------------
 //
@@ -107,7 +111,7 @@ inline int sweep_class_of(int nb, int ph) {
     return -1;
 }
 // Kernels: one per class with the square bound (index = class), and one per
-// class of at most kUnionBoundMaxBands bands with the rolling union bound
+// class of at most kUnionBoundMaxBands bands with the union bound
 // (index = kSweepClasses + class).
 constexpr int kSweepUnionClasses = 10;
 constexpr int kSweepKernels = kSweepClasses + kSweepUnionClasses;

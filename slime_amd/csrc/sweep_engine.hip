@@ -206,6 +206,9 @@ bool Sweeper::calibrate() {
         return false;
     for (int c = 0; c < kSweepClasses; ++c) {
         const int nb = kSweepClass[c].nb, ph = kSweepClass[c].ph;
+        // the kernel searches of the default mode run: the class's union-bound
+        // form where it has one
+        const int kern = sweep_kernel_of(c, true);
         double a[2] = {0, 0}, r[2] = {0, 0};
         const int32_t ths[2] = {64, 1024};
         for (int k = -1; k < 2; ++k) {  // k = -1: warm-up launch
@@ -216,7 +219,7 @@ bool Sweeper::calibrate() {
             spec.z0 = 12345;
             spec.w = ph == 2 ? 2 * kPeriod : kPeriod;
             const int64_t tiles_x = ph == 2 ? kTilesPerPhase : kTilesPerPhase;
-            const int64_t want = (k < 0 ? 1 : (th == 64 ? 24 : 6)) * (int64_t)grid_[c];
+            const int64_t want = (k < 0 ? 1 : (th == 64 ? 24 : 6)) * (int64_t)grid_[kern];
             const int64_t row_tiles = std::max<int64_t>(1, (want + tiles_x - 1) / tiles_x);
             const int64_t slot_rows = std::min<int64_t>(kPeriod / th * th, row_tiles * th);
             spec.h = (int64_t)(nb - 1) * kPeriod + slot_rows;
@@ -228,7 +231,7 @@ bool Sweeper::calibrate() {
             if (jobs.size() != 1) return false;
             SweepJob sj{};
             sj.p = jobs[0].p;
-            sj.cls = c;
+            sj.cls = kern;
             sj.req = slot;
             sj.p.tag = slot;
             sj.p.hits = d_hits_;
@@ -247,9 +250,9 @@ bool Sweeper::calibrate() {
                                stream_) != hipSuccess ||
                 hipEventRecord(ev0_, stream_) != hipSuccess)
                 return false;
-            SweepArgs args{d_jobs_, d_order_, 1, d_cursor_, d_done_, d_status_, d_claims_ + c,
-                           0xFFFFFFFFu};
-            if (launch_sweep(args, c, grid_[c], stream_) != hipSuccess ||
+            SweepArgs args{d_jobs_, d_order_, 1, d_cursor_, d_done_, d_status_,
+                           d_claims_ + kern, 0xFFFFFFFFu};
+            if (launch_sweep(args, kern, grid_[kern], stream_) != hipSuccess ||
                 hipEventRecord(ev1_, stream_) != hipSuccess ||
                 hipStreamSynchronize(stream_) != hipSuccess ||
                 hipEventElapsedTime(&ms, ev0_, ev1_) != hipSuccess)

@@ -218,15 +218,18 @@ def test_banded_at_int32_edges(gpu, radius, corner):
     assert [r[0] for r in rows] == [r[0] for r in b[1]]
 
 
-@pytest.mark.parametrize("nb", [1, 2, 4, 8, 9])
-def test_union_bound_band_sets_vs_oracle(gpu, oracle, nb):
-    """Strided band sets through the union-bound kernel (9 bands: past the
-    union's band limit, the square bound runs) against the oracle."""
+@pytest.mark.parametrize("nb,w", [(1, 2 * P + 333), (2, 2 * P + 333), (4, 2 * P + 333),
+                                  (8, 2 * P + 333), (9, 2 * P + 333), (12, 2 * P + 333),
+                                  (13, 2 * P + 333), (3, 30000), (5, 30000)])
+def test_union_bound_band_sets_vs_oracle(gpu, oracle, nb, w):
+    """Strided band sets through the union-bound kernels against the oracle:
+    by group windows (up to 3 band-phases) and by plane words with the corner
+    bound (one phase of one band per thread), two x-phases and one; 9, 12 and
+    13 bands run as launches of at most 8."""
     import slime_amd
-    rng = random.Random(500 + nb)
+    rng = random.Random(500 + nb + w)
     seed = s64(rng.getrandbits(64))
     band_rows = 16
-    w = 2 * P + 333
     x0 = rng.randint(-10 ** 8, 10 ** 8)
     z0 = rng.randint(-10 ** 8, 10 ** 8)
     h = (nb - 1) * P + band_rows
