@@ -54,9 +54,9 @@ OPS_PER_CENTER = 6      # algorithmic INT32 ops per candidate center
 CPU_SAMPLE_RADIUS = 131072
 # HBM bytes per scan_band_kernel launch (rocprofv3 FETCH_SIZE + WRITE_SIZE of
 # the full-world run, scaled by the same counters on a copy of known size;
-# profiles/scan_band_kernel_full_world_thr60.md): 90.7 KiB x 2.0 + 0.02 KiB x 1.0
-# per launch, averaged over the search's 31 launches.
-TRAFFIC_BYTES = 185700
+# profiles/scan_band_kernel_full_world_thr60.md): 80.0 KiB x 2.0 + 0.02 KiB x 1.0
+# per launch, averaged over the search's 33 launches.
+TRAFFIC_BYTES = 163900
 
 
 def hip_sync():
