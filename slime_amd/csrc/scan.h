@@ -140,6 +140,17 @@ struct SweepArgs {
     unsigned int* claims;   // claim attempts in this launch (cleared before it)
     uint32_t budget;        // claim attempts allowed in this launch
 };
+// A batch's launch without claims (every job from its first tile, nothing
+// paused): one workgroup per tile, job i of order holding the blocks
+// [first[i], first[i+1]).
+struct BatchArgs {
+    const SweepJob* jobs;
+    const int32_t* order;
+    const int64_t* first;  // n + 1 entries, first[0] == 0
+    int32_t n;
+    unsigned int* done;    // per job: tiles finished
+};
+hipError_t launch_batch(const BatchArgs& a, int kernel, int64_t blocks, hipStream_t stream);
 // Workgroups of class cls's sweep kernel that fit one CU at once.
 int sweep_blocks_per_cu(int kernel);
 // The persistent kernel of class cls over a.order (jobs of that class only).

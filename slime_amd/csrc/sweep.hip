@@ -96,6 +96,37 @@ __global__ __launch_bounds__(kThreads, kSweepWaves[K]) void sweep_kernel(SweepAr
     }
 }
 
+// A batch's launch of kernel K without claims: one workgroup per tile of the
+// jobs a.order[0..n), job i's tiles being the blocks [first[i], first[i+1]).
+// Each workgroup finds its job by binary search and runs the tile as the
+// per-search kernels do (same register budget: no claim loop around it).
+template <int K>
+__global__ __launch_bounds__(kThreads, 3) void batch_kernel(BatchArgs a) {
+    constexpr int C = sweep_class_of_kernel(K);
+    constexpr int NB = kSweepClass[C].nb, PH = kSweepClass[C].ph;
+    constexpr bool UB = K >= kSweepClasses;
+    __shared__ band::TileLds<NB> lds;
+    for (int i = threadIdx.x; i < kHistBins; i += kThreads) lds.lhist[i] = 0;
+    const int64_t blk = blockIdx.x;
+    int lo = 0, hi = a.n - 1;  // the last i with first[i] <= blk
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.first[mid] <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    const int job = a.order[lo];
+    const BandParams p = a.jobs[job].p;
+    band::band_tile<NB, PH, UB>(p, p.emit_min, (int)(blk - a.first[lo]), lds);
+    band::flush_hist(lds.lhist, p.hist);
+    if (threadIdx.x == 0) atomicAdd(&a.done[job], 1u);
+}
+
+template <int K>
+hipError_t launch_batch_kernel(const BatchArgs& a, int64_t blocks, hipStream_t stream) {
+    hipLaunchKernelGGL(batch_kernel<K>, dim3((unsigned)blocks), dim3(kThreads), 0, stream, a);
+    return hipGetLastError();
+}
+
 template <int C>
 hipError_t launch_class(const SweepArgs& a, int grid, hipStream_t stream) {
     hipLaunchKernelGGL(sweep_kernel<C>, dim3(grid), dim3(kThreads), 0, stream, a);
@@ -175,6 +206,35 @@ hipError_t launch_sweep(const SweepArgs& a, int kernel, int grid, hipStream_t st
     }
 }
 static_assert(kSweepKernels == 21, "one case per kernel");
+
+hipError_t launch_batch(const BatchArgs& a, int kernel, int64_t blocks, hipStream_t stream) {
+    if (a.n <= 0 || blocks <= 0) return hipSuccess;
+    if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
+    switch (kernel) {
+        case 0: return launch_batch_kernel<0>(a, blocks, stream);
+        case 1: return launch_batch_kernel<1>(a, blocks, stream);
+        case 2: return launch_batch_kernel<2>(a, blocks, stream);
+        case 3: return launch_batch_kernel<3>(a, blocks, stream);
+        case 4: return launch_batch_kernel<4>(a, blocks, stream);
+        case 5: return launch_batch_kernel<5>(a, blocks, stream);
+        case 6: return launch_batch_kernel<6>(a, blocks, stream);
+        case 7: return launch_batch_kernel<7>(a, blocks, stream);
+        case 8: return launch_batch_kernel<8>(a, blocks, stream);
+        case 9: return launch_batch_kernel<9>(a, blocks, stream);
+        case 10: return launch_batch_kernel<10>(a, blocks, stream);
+        case 11: return launch_batch_kernel<11>(a, blocks, stream);
+        case 12: return launch_batch_kernel<12>(a, blocks, stream);
+        case 13: return launch_batch_kernel<13>(a, blocks, stream);
+        case 14: return launch_batch_kernel<14>(a, blocks, stream);
+        case 15: return launch_batch_kernel<15>(a, blocks, stream);
+        case 16: return launch_batch_kernel<16>(a, blocks, stream);
+        case 17: return launch_batch_kernel<17>(a, blocks, stream);
+        case 18: return launch_batch_kernel<18>(a, blocks, stream);
+        case 19: return launch_batch_kernel<19>(a, blocks, stream);
+        case 20: return launch_batch_kernel<20>(a, blocks, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
 
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t stream) {
     if (n == 0) return hipSuccess;

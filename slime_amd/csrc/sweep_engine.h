@@ -58,6 +58,9 @@ struct RoundPolicy {
     int64_t round_ns = 0;         // device time per class launch (0: unbounded)
     bool shortest_first = false;  // claim order: least work left first (else largest tiles first)
     bool use_graph = false;       // replay the launch sequence from a captured hipGraph
+    // with round_ns 0, over jobs that have not started: one workgroup per
+    // tile and no claims (the requests' status words are not read: batches)
+    bool direct = false;
 };
 
 struct RoundStats {
@@ -108,6 +111,7 @@ class Sweeper {
     uint32_t* d_done_ = nullptr;
     uint32_t* h_done_ = nullptr;
     int32_t* d_order_ = nullptr;
+    int64_t* d_first_ = nullptr;  // batch launches: first block of each job (job_cap + kernels)
     int32_t* d_status_ = nullptr;
     unsigned long long* d_hist_ = nullptr;
     unsigned long long* d_emitted_ = nullptr;

@@ -104,6 +104,7 @@ void Sweeper::release() {
     f(d_cursor_);
     f(d_done_);
     f(d_order_);
+    f(d_first_);
     f(d_status_);
     f(d_hist_);
     f(d_emitted_);
@@ -121,6 +122,7 @@ void Sweeper::release() {
     d_jobs_ = nullptr;
     d_cursor_ = d_done_ = d_claims_ = nullptr;
     d_order_ = d_status_ = nullptr;
+    d_first_ = nullptr;
     d_hist_ = d_emitted_ = nullptr;
     d_hits_ = nullptr;
     d_table_ = nullptr;
@@ -159,6 +161,7 @@ bool Sweeper::init(int32_t req_cap, int64_t job_cap, int64_t hit_cap) {
         hipMalloc(&d_cursor_, sizeof(uint32_t) * job_cap) == hipSuccess &&
         hipMalloc(&d_done_, sizeof(uint32_t) * job_cap) == hipSuccess &&
         hipMalloc(&d_order_, sizeof(int32_t) * job_cap) == hipSuccess &&
+        hipMalloc(&d_first_, sizeof(int64_t) * (job_cap + kSweepKernels)) == hipSuccess &&
         hipHostMalloc(&h_done_, sizeof(uint32_t) * job_cap, hipHostMallocDefault) == hipSuccess &&
         hipMalloc(&d_status_, sizeof(int32_t) * req_cap) == hipSuccess &&
         hipMalloc(&d_hist_, sizeof(unsigned long long) * kHistBins * req_cap) == hipSuccess &&
@@ -565,6 +568,7 @@ bool Sweeper::round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& p
         int32_t job;
         int64_t left;
         int32_t tile_h;
+        bool fresh;  // no tile of the job has run yet
     };
     std::vector<Item> items[kSweepKernels];
     int64_t seq = 0;
@@ -577,16 +581,23 @@ bool Sweeper::round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& p
             const double key = pol.shortest_first ? r->remaining_ns : -tile_ns(c, r->jobs[j].p.tile_h);
             if (left > 0)
                 items[c].push_back(Item{key, seq, (int32_t)(r->job_off + (int64_t)j), left,
-                                        r->jobs[j].p.tile_h});
+                                        r->jobs[j].p.tile_h, r->done[j] == 0});
             ++seq;
         }
     }
     std::vector<int32_t> order;
+    // A round without a claim budget over jobs that all start at their first
+    // tile (a batch) runs without claims: one workgroup per tile, as the
+    // per-search kernels do (their register budget, no claim loop).
+    bool direct = pol.direct && pol.round_ns <= 0;
+    std::vector<int64_t> firsts;  // per launch: n + 1 block offsets
     struct Launch {
         int cls;
         int32_t off, n;
         uint32_t budget;
         double min_key;
+        int32_t first_off;
+        int64_t blocks;
     };
     std::vector<Launch> launches;
     for (int c = 0; c < kSweepKernels; ++c) {
@@ -602,11 +613,16 @@ bool Sweeper::round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& p
         l.min_key = v.front().key;
         int64_t tiles = 0;
         double tile_sum = 0;
+        l.first_off = (int32_t)firsts.size();
+        firsts.push_back(0);
         for (const Item& it : v) {
             order.push_back(it.job);
             tiles += it.left;
             tile_sum += tile_ns(c, it.tile_h) * (double)it.left;
+            firsts.push_back(tiles);
+            if (!it.fresh) direct = false;  // a job already under way needs the cursors
         }
+        l.blocks = tiles;
         if (pol.round_ns <= 0) {
             l.budget = 0xFFFFFFFFu;
         } else {
@@ -622,12 +638,19 @@ bool Sweeper::round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& p
     std::stable_sort(launches.begin(), launches.end(),
                      [](const Launch& a, const Launch& b) { return a.min_key < b.min_key; });
     if (hipMemcpyAsync(d_order_, order.data(), sizeof(int32_t) * order.size(),
-                       hipMemcpyHostToDevice, stream_) != hipSuccess)
+                       hipMemcpyHostToDevice, stream_) != hipSuccess ||
+        (direct && hipMemcpyAsync(d_first_, firsts.data(), sizeof(int64_t) * firsts.size(),
+                                  hipMemcpyHostToDevice, stream_) != hipSuccess))
         return false;
 
     auto enqueue = [&]() -> bool {
         if (launch_fill_u32(d_claims_, kSweepKernels, 0u, stream_) != hipSuccess) return false;
         for (const Launch& l : launches) {
+            if (direct) {
+                BatchArgs a{d_jobs_, d_order_ + l.off, d_first_ + l.first_off, l.n, d_done_};
+                if (launch_batch(a, l.cls, l.blocks, stream_) != hipSuccess) return false;
+                continue;
+            }
             SweepArgs a{d_jobs_, d_order_ + l.off, l.n, d_cursor_, d_done_, d_status_,
                         d_claims_ + l.cls, l.budget};
             if (launch_sweep(a, l.cls, grid_[l.cls], stream_) != hipSuccess) return false;
@@ -639,11 +662,14 @@ bool Sweeper::round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& p
         // the launch sequence, keyed by its arguments; replays reuse the device
         // arrays (jobs, orders) whose contents were written above
         std::vector<int64_t> key;
+        key.push_back(direct ? 1 : 0);
         for (const Launch& l : launches) {
             key.push_back(l.cls);
             key.push_back(l.off);
             key.push_back(l.n);
             key.push_back(l.budget);
+            key.push_back(l.first_off);
+            key.push_back(l.blocks);
         }
         auto it = graphs_.find(key);
         if (it == graphs_.end()) {
@@ -804,6 +830,7 @@ int run_batch(const SearchSpec* specs, int32_t n, int32_t max_results, ResultRow
     RoundPolicy pol;
     pol.round_ns = 0;  // one unbounded round: every tile in one launch per class
     pol.shortest_first = false;
+    pol.direct = true;
     pol.use_graph = opt.use_graph;
     RoundStats rs{};
     const auto t1 = Clock::now();

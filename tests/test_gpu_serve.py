@@ -29,6 +29,8 @@ def searches(seed, n, radii):
 def test_served_mix_equals_alone(gpu):
     import slime_amd
     reqs = searches(3, 18, [40, 300, 3000, 20000, 50000, 100000, 150000, 5])
+    # a last one large enough (2e12 centers) to be still running when paused
+    reqs.append((0x5EEDF00D, 700000, 56, 221, 0, 123456, -654321))
     k = 300
     with slime_amd.Session(hbm_budget=2 << 30, max_active=8, round_ms=5) as s:
         ids = [None] * len(reqs)
@@ -42,9 +44,11 @@ def test_served_mix_equals_alone(gpu):
         t.start()
         t.join()
         assert all(i and i > 0 for i in ids)
-        paused = [ids[5], ids[6]]
-        for rid in paused:
-            assert s.pause(rid)
+        paused = [ids[5], ids[6], ids[-1]]
+        # the smaller ones may have finished already
+        assert s.pause(ids[-1])
+        for rid in paused[:2]:
+            assert s.pause(rid) or s.poll(rid)[0] == slime_amd.DONE
         cancelled = ids[13]
         s.cancel(cancelled)
         time.sleep(0.05)
