@@ -23,6 +23,7 @@
 #include <cstring>
 #include <deque>
 #include <mutex>
+#include <new>
 #include <thread>
 
 #include "engine.h"
@@ -53,6 +54,63 @@ int g_max_bands = [] {
     return n >= 1 && n <= kMaxBands ? n : 12;
 }();
 
+// Host array of center fixes, in pinned memory where it can be had (the copies
+// to and from the device then need no staging), kept across searches so that a
+// full-world search's 2.3 M fixes (55 MB) find their pages already mapped.
+class FixList {
+  public:
+    ~FixList() { release(); }
+    void release() {
+        if (data_) {
+            if (pinned_) (void)hipHostFree(data_);
+            else delete[] data_;
+        }
+        data_ = nullptr;
+        n_ = cap_ = 0;
+    }
+    void clear() { n_ = 0; }
+    // false only if no memory at all could be had
+    bool reserve(size_t want) {
+        if (want <= cap_) return true;
+        CenterFix* p = nullptr;
+        bool pinned = hipHostMalloc((void**)&p, sizeof(CenterFix) * want, hipHostMallocDefault) ==
+                      hipSuccess;
+        if (!pinned) {
+            (void)hipGetLastError();
+            p = new (std::nothrow) CenterFix[want];
+            if (!p) return false;
+        }
+        if (n_) std::memcpy(p, data_, sizeof(CenterFix) * n_);
+        const size_t n = n_;
+        release();
+        data_ = p;
+        pinned_ = pinned;
+        n_ = n;
+        cap_ = want;
+        return true;
+    }
+    void push_back(const CenterFix& f) {
+        if (n_ == cap_ && !reserve(std::max<size_t>(1024, 2 * cap_))) {
+            failed_ = true;
+            return;
+        }
+        data_[n_++] = f;
+    }
+    bool failed() const { return failed_; }
+    void reset_failed() { failed_ = false; }
+    CenterFix* data() { return data_; }
+    size_t size() const { return n_; }
+    bool empty() const { return n_ == 0; }
+    CenterFix* begin() { return data_; }
+    CenterFix* end() { return data_ + n_; }
+
+  private:
+    CenterFix* data_ = nullptr;
+    size_t n_ = 0, cap_ = 0;
+    bool pinned_ = false;
+    bool failed_ = false;
+};
+
 struct Buffers {
     int device = -1;
     int cus = 256;  // compute units of the device
@@ -70,6 +128,7 @@ struct Buffers {
     CenterFix* d_fixes = nullptr;
     size_t fixes_cap = 0;
 };
+FixList g_fixes;  // the host fixes of the current search (guarded by g_mutex)
 
 Buffers g_buf;
 std::mutex g_mutex;
@@ -89,6 +148,7 @@ void free_buffers() {
     if (g_buf.ev_start) (void)hipEventDestroy(g_buf.ev_start);
     if (g_buf.ev_stop) (void)hipEventDestroy(g_buf.ev_stop);
     if (g_buf.stream) (void)hipStreamDestroy(g_buf.stream);
+    g_fixes.release();
     g_buf = Buffers{};
 }
 
@@ -376,8 +436,7 @@ void plan_banded(const SearchSpec& spec, std::deque<Work>& out) {
 
 // Finds the first-draw mismatches of the region and recounts every affected
 // center exactly. Returns false on a device error.
-bool prepare_fixes(const SearchSpec& spec, MismatchIndex& index, Ranker& rk,
-                   std::vector<CenterFix>& fixes) {
+bool prepare_fixes(const SearchSpec& spec, MismatchIndex& index, Ranker& rk, FixList& fixes) {
     if (hipMemsetAsync(g_buf.d_mismatch_count, 0, sizeof(unsigned int), g_buf.stream) != hipSuccess ||
         launch_find_mismatches(spec.seed, (int32_t)((int64_t)spec.x0 - 8), spec.w + 16,
                                (int32_t)((int64_t)spec.z0 - 8), spec.h + 16, g_buf.d_table,
@@ -395,7 +454,10 @@ bool prepare_fixes(const SearchSpec& spec, MismatchIndex& index, Ranker& rk,
                             hipMemcpyDeviceToHost) != hipSuccess))
         return false;
     index.sort();
+    fixes.clear();
+    fixes.reset_failed();
     collect_fixes(spec, index, fixes);
+    if (fixes.failed()) return false;
     rk.recount = &index;
     if (fixes.empty()) return true;
     if (!ensure_fixes(fixes.size()) ||
@@ -465,7 +527,7 @@ int64_t run_search(const SearchSpec& spec, ResultRow* out, int32_t max_results,
     // Both kernels count first draws; the centers near a Java-rejected chunk
     // are recounted exactly afterwards.
     std::deque<Work> works;
-    std::vector<CenterFix> fixes;
+    FixList& fixes = g_fixes;
     MismatchIndex mismatches;
     if (!prepare_fixes(spec, mismatches, rk, fixes)) return fail();
     if (banded)

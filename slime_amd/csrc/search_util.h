@@ -47,12 +47,13 @@ struct MismatchIndex {
 };
 
 // Every center of the spec whose disk holds a mismatch chunk (index sorted),
-// once each, as fixes to recount.
-inline void collect_fixes(const SearchSpec& spec, const MismatchIndex& index,
-                          std::vector<CenterFix>& fixes) {
+// once each, appended to `fixes` (a container with reserve and push_back) as
+// fixes to recount.
+template <typename Fixes>
+inline void collect_fixes(const SearchSpec& spec, const MismatchIndex& index, Fixes& fixes) {
     const std::vector<MismatchChunk>& mm = index.mm;
     const int64_t rmin_sq = spec.rd_min * spec.rd_min;
-    fixes.clear();
+    fixes.reserve(fixes.size() + mm.size() * (size_t)kDiskCells);
     for (size_t i = 0; i < mm.size(); ++i) {
         const MismatchChunk& m = mm[i];
         // earlier mismatches close enough to share a center already added it

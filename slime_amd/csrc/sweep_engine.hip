@@ -489,6 +489,7 @@ bool Sweeper::prepare(const std::vector<SweepRequest*>& reqs) {
     std::vector<size_t> first(reqs.size() + 1, 0);
     for (size_t i = 0; i < reqs.size(); ++i) {
         reqs[i]->mm.sort();
+        reqs[i]->fixes.clear();
         collect_fixes(reqs[i]->spec, reqs[i]->mm, reqs[i]->fixes);
         first[i] = all.size();
         all.insert(all.end(), reqs[i]->fixes.begin(), reqs[i]->fixes.end());
