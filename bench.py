@@ -162,17 +162,20 @@ def batched_leg(repeats):
     res = slime_amd.search_batch(*cols, max_results=k)  # warm-up, calibrates the autotuner
     out = {"workload": "256 searches, radius log-uniform [256, 200000], thr 50..62",
            "state": hex(BATCH_STATE), "centers": centers, "max_results": k}
-    for graph in (False, True):
+    # one workgroup per tile (default), the same from a graph, and the
+    # persistent work-queue kernels
+    for key, graph, persistent in (("sweep", False, False), ("sweep_graph", True, False),
+                                   ("sweep_persistent", False, True)):
         times = []
         for _ in range(repeats):
             t0 = time.perf_counter()
-            got = slime_amd.search_batch(*cols, max_results=k, use_graph=graph)
+            got = slime_amd.search_batch(*cols, max_results=k, use_graph=graph,
+                                         persistent=persistent)
             times.append(time.perf_counter() - t0)
             if got != res:
                 raise SystemExit("batched results differ between runs")
         dt = min(times)
         st = slime_amd.batch_stats()
-        key = "sweep_graph" if graph else "sweep"
         out[key] = {"seconds": dt, "t_centers_per_s": centers / dt / 1e12,
                     "kernel_s": st["kernel_ns"] / 1e9, "launches": st["launches"],
                     "jobs": st["jobs"], "fallbacks": st["fallbacks"]}

@@ -185,11 +185,13 @@ void slime_refine_stats(int64_t* kernel_ns, int64_t* evaluations);
 /* ---- batched searches (extension; the reference runs one search per call)
  * n independent searches, each with the meaning of
  * search_slime_clusters_gpu_v1_centered (SlimeCoreGPU.cu:3927-3935), scanned
- * together by one persistent launch per kernel class fed from a device work
- * queue. Search i writes up to max_results ranked rows at
+ * together: one launch per kernel class over all the searches' tiles (one
+ * workgroup per tile), or with flags bit 1 one persistent launch per kernel
+ * class fed from a device work queue (per-job claim cursors; the form served
+ * sessions use). Search i writes up to max_results ranked rows at
  * results_buffer + i * max_results and its return value (hit count, or -1 for
- * invalid arguments) to totals[i]. flags bit 0: replay the sweep's launches
- * from a captured hipGraph. Returns 0, or -1 on bad arrays or a device error.
+ * invalid arguments) to totals[i]. flags bit 0: replay the launches from a
+ * captured hipGraph. Returns 0, or -1 on bad arrays or a device error.
  * Results are identical to n separate calls. */
 int32_t slime_search_batch(int32_t n, const int64_t* seeds, const int64_t* rd_max,
                            const int32_t* min_size, const int32_t* max_size, const int64_t* rd_min,

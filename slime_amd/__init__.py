@@ -335,9 +335,10 @@ def scan_rect(seed, x0, z0, w, h, min_size, max_size=221, max_results=65536):
 
 
 def search_batch(seeds, rd_max, min_size, max_size=None, rd_min=None, center_x=None,
-                 center_z=None, max_results=1000, use_graph=False):
-    """Many searches in one persistent sweep (slime_search_batch): a list of
-    (total, rows) per search, each identical to search_slime_clusters_gpu_v1_centered."""
+                 center_z=None, max_results=1000, use_graph=False, persistent=False):
+    """Many searches scanned together (slime_search_batch; persistent: the
+    work-queue kernels): a list of (total, rows) per search, each identical to
+    search_slime_clusters_gpu_v1_centered."""
     n = len(seeds)
     def arr(t, v, default):
         v = [default] * n if v is None else list(v)
@@ -352,7 +353,8 @@ def search_batch(seeds, rd_max, min_size, max_size=None, rd_min=None, center_x=N
     buf = (ExtChunkResult * (max(1, n) * max_results))()
     tot = (C.c_int64 * max(1, n))()
     rc = lib().slime_search_batch(n, a_seed, a_rd, a_min, a_max, a_rmin, a_cx, a_cz, buf,
-                                  max_results, tot, 1 if use_graph else 0)
+                                  max_results, tot,
+                                  (1 if use_graph else 0) | (2 if persistent else 0))
     if rc != 0:
         raise RuntimeError("slime_search_batch failed")
     out = []

@@ -386,6 +386,7 @@ int32_t slime_search_batch(int32_t n, const int64_t* seeds, const int64_t* rd_ma
     std::vector<int64_t> tot(specs.size(), -1);
     slime::SweepOptions opt;
     opt.use_graph = (flags & 1) != 0;
+    opt.persistent = (flags & 2) != 0;
     if (slime::run_batch(specs.data(), (int32_t)specs.size(), max_results, rows.data(),
                          tot.data(), opt, nullptr) != 0)
         return -1;

@@ -140,13 +140,14 @@ struct SweepArgs {
     unsigned int* claims;   // claim attempts in this launch (cleared before it)
     uint32_t budget;        // claim attempts allowed in this launch
 };
-// A batch's launch without claims (every job from its first tile, nothing
-// paused): one workgroup per tile, job i of order holding the blocks
+// A launch without claims: one workgroup per tile, job order[i] running its
+// tiles start[i] + (0 .. first[i+1] - first[i] - 1) as the blocks
 // [first[i], first[i+1]).
 struct BatchArgs {
     const SweepJob* jobs;
     const int32_t* order;
     const int64_t* first;  // n + 1 entries, first[0] == 0
+    const int32_t* start;  // n entries
     int32_t n;
     unsigned int* done;    // per job: tiles finished
 };

@@ -96,8 +96,9 @@ __global__ __launch_bounds__(kThreads, kSweepWaves[K]) void sweep_kernel(SweepAr
     }
 }
 
-// A batch's launch of kernel K without claims: one workgroup per tile of the
-// jobs a.order[0..n), job i's tiles being the blocks [first[i], first[i+1]).
+// A launch of kernel K without claims: one workgroup per tile of the jobs
+// a.order[0..n), job i's tiles start[i] + ... being the blocks
+// [first[i], first[i+1]).
 // Each workgroup finds its job by binary search and runs the tile as the
 // per-search kernels do (same register budget: no claim loop around it).
 template <int K>
@@ -116,7 +117,7 @@ __global__ __launch_bounds__(kThreads, 3) void batch_kernel(BatchArgs a) {
     }
     const int job = a.order[lo];
     const BandParams p = a.jobs[job].p;
-    band::band_tile<NB, PH, UB>(p, p.emit_min, (int)(blk - a.first[lo]), lds);
+    band::band_tile<NB, PH, UB>(p, p.emit_min, a.start[lo] + (int)(blk - a.first[lo]), lds);
     band::flush_hist(lds.lhist, p.hist);
     if (threadIdx.x == 0) atomicAdd(&a.done[job], 1u);
 }

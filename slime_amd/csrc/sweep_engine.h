@@ -58,8 +58,8 @@ struct RoundPolicy {
     int64_t round_ns = 0;         // device time per class launch (0: unbounded)
     bool shortest_first = false;  // claim order: least work left first (else largest tiles first)
     bool use_graph = false;       // replay the launch sequence from a captured hipGraph
-    // with round_ns 0, over jobs that have not started: one workgroup per
-    // tile and no claims (the requests' status words are not read: batches)
+    // one workgroup per tile over a range of tiles per job, no claims (the
+    // requests' status words are not read: pause and cancel act between rounds)
     bool direct = false;
 };
 
@@ -111,7 +111,8 @@ class Sweeper {
     uint32_t* d_done_ = nullptr;
     uint32_t* h_done_ = nullptr;
     int32_t* d_order_ = nullptr;
-    int64_t* d_first_ = nullptr;  // batch launches: first block of each job (job_cap + kernels)
+    int64_t* d_first_ = nullptr;  // direct launches: first block of each job (job_cap + kernels)
+    int32_t* d_start_ = nullptr;  // direct launches: first tile of each job
     int32_t* d_status_ = nullptr;
     unsigned long long* d_hist_ = nullptr;
     unsigned long long* d_emitted_ = nullptr;
@@ -142,6 +143,9 @@ class Sweeper {
 
 struct SweepOptions {
     bool use_graph = false;
+    // the persistent kernels fed from the device work queue (per-job claim
+    // cursors) instead of one workgroup per tile
+    bool persistent = false;
 };
 
 struct BatchStats {

@@ -105,6 +105,7 @@ void Sweeper::release() {
     f(d_done_);
     f(d_order_);
     f(d_first_);
+    f(d_start_);
     f(d_status_);
     f(d_hist_);
     f(d_emitted_);
@@ -123,6 +124,7 @@ void Sweeper::release() {
     d_cursor_ = d_done_ = d_claims_ = nullptr;
     d_order_ = d_status_ = nullptr;
     d_first_ = nullptr;
+    d_start_ = nullptr;
     d_hist_ = d_emitted_ = nullptr;
     d_hits_ = nullptr;
     d_table_ = nullptr;
@@ -162,6 +164,7 @@ bool Sweeper::init(int32_t req_cap, int64_t job_cap, int64_t hit_cap) {
         hipMalloc(&d_done_, sizeof(uint32_t) * job_cap) == hipSuccess &&
         hipMalloc(&d_order_, sizeof(int32_t) * job_cap) == hipSuccess &&
         hipMalloc(&d_first_, sizeof(int64_t) * (job_cap + kSweepKernels)) == hipSuccess &&
+        hipMalloc(&d_start_, sizeof(int32_t) * job_cap) == hipSuccess &&
         hipHostMalloc(&h_done_, sizeof(uint32_t) * job_cap, hipHostMallocDefault) == hipSuccess &&
         hipMalloc(&d_status_, sizeof(int32_t) * req_cap) == hipSuccess &&
         hipMalloc(&d_hist_, sizeof(unsigned long long) * kHistBins * req_cap) == hipSuccess &&
@@ -569,7 +572,7 @@ bool Sweeper::round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& p
         int32_t job;
         int64_t left;
         int32_t tile_h;
-        bool fresh;  // no tile of the job has run yet
+        int64_t start;  // tiles of the job finished before this round
     };
     std::vector<Item> items[kSweepKernels];
     int64_t seq = 0;
@@ -582,16 +585,19 @@ bool Sweeper::round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& p
             const double key = pol.shortest_first ? r->remaining_ns : -tile_ns(c, r->jobs[j].p.tile_h);
             if (left > 0)
                 items[c].push_back(Item{key, seq, (int32_t)(r->job_off + (int64_t)j), left,
-                                        r->jobs[j].p.tile_h, r->done[j] == 0});
+                                        r->jobs[j].p.tile_h, (int64_t)r->done[j]});
             ++seq;
         }
     }
     std::vector<int32_t> order;
-    // A round without a claim budget over jobs that all start at their first
-    // tile (a batch) runs without claims: one workgroup per tile, as the
-    // per-search kernels do (their register budget, no claim loop).
-    bool direct = pol.direct && pol.round_ns <= 0;
+    // Direct rounds: one workgroup per tile of a range of tiles per job (all
+    // its tiles left, or as many as the round's device time allows, most
+    // urgent jobs first), as the per-search kernels do (their register
+    // budget, no claim loop); pause, resume and cancel act between rounds.
+    // Otherwise the persistent kernels claim tiles under a claim budget.
+    const bool direct = pol.direct;
     std::vector<int64_t> firsts;  // per launch: n + 1 block offsets
+    std::vector<int32_t> starts;  // per direct item: its first tile this round
     struct Launch {
         int cls;
         int32_t off, n;
@@ -610,28 +616,39 @@ bool Sweeper::round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& p
         Launch l;
         l.cls = c;
         l.off = (int32_t)order.size();
-        l.n = (int32_t)v.size();
         l.min_key = v.front().key;
         int64_t tiles = 0;
         double tile_sum = 0;
-        l.first_off = (int32_t)firsts.size();
-        firsts.push_back(0);
         for (const Item& it : v) {
-            order.push_back(it.job);
             tiles += it.left;
             tile_sum += tile_ns(c, it.tile_h) * (double)it.left;
-            firsts.push_back(tiles);
-            if (!it.fresh) direct = false;  // a job already under way needs the cursors
         }
-        l.blocks = tiles;
-        if (pol.round_ns <= 0) {
-            l.budget = 0xFFFFFFFFu;
-        } else {
-            // claims that take about round_ns of the whole device
+        // tiles (claims) that take about round_ns of the whole device, at
+        // least one workgroup per resident slot
+        int64_t want = INT64_MAX;
+        if (pol.round_ns > 0) {
             const double per_tile = tile_sum / (double)std::max<int64_t>(1, tiles);
-            const double want = (double)pol.round_ns / std::max(1e-3, per_tile);
-            l.budget = (uint32_t)std::min<double>(4e9, std::max<double>((double)grid_[c], want));
+            want = (int64_t)std::min<double>(
+                4e9, std::max<double>((double)grid_[c], (double)pol.round_ns /
+                                                            std::max(1e-3, per_tile)));
         }
+        l.budget = pol.round_ns > 0 ? (uint32_t)want : 0xFFFFFFFFu;
+        l.first_off = (int32_t)firsts.size();
+        l.blocks = 0;
+        if (direct) {
+            firsts.push_back(0);
+            for (const Item& it : v) {
+                const int64_t n = std::min<int64_t>(it.left, want - l.blocks);
+                if (n <= 0) break;
+                order.push_back(it.job);
+                starts.push_back((int32_t)it.start);
+                l.blocks += n;
+                firsts.push_back(l.blocks);
+            }
+        } else {
+            for (const Item& it : v) order.push_back(it.job);
+        }
+        l.n = (int32_t)order.size() - l.off;
         launches.push_back(l);
     }
     if (launches.empty()) return true;
@@ -640,15 +657,18 @@ bool Sweeper::round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& p
                      [](const Launch& a, const Launch& b) { return a.min_key < b.min_key; });
     if (hipMemcpyAsync(d_order_, order.data(), sizeof(int32_t) * order.size(),
                        hipMemcpyHostToDevice, stream_) != hipSuccess ||
-        (direct && hipMemcpyAsync(d_first_, firsts.data(), sizeof(int64_t) * firsts.size(),
-                                  hipMemcpyHostToDevice, stream_) != hipSuccess))
+        (direct && (hipMemcpyAsync(d_first_, firsts.data(), sizeof(int64_t) * firsts.size(),
+                                   hipMemcpyHostToDevice, stream_) != hipSuccess ||
+                    hipMemcpyAsync(d_start_, starts.data(), sizeof(int32_t) * starts.size(),
+                                   hipMemcpyHostToDevice, stream_) != hipSuccess)))
         return false;
 
     auto enqueue = [&]() -> bool {
         if (launch_fill_u32(d_claims_, kSweepKernels, 0u, stream_) != hipSuccess) return false;
         for (const Launch& l : launches) {
             if (direct) {
-                BatchArgs a{d_jobs_, d_order_ + l.off, d_first_ + l.first_off, l.n, d_done_};
+                BatchArgs a{d_jobs_, d_order_ + l.off, d_first_ + l.first_off, d_start_ + l.off,
+                            l.n, d_done_};
                 if (launch_batch(a, l.cls, l.blocks, stream_) != hipSuccess) return false;
                 continue;
             }
@@ -831,7 +851,7 @@ int run_batch(const SearchSpec* specs, int32_t n, int32_t max_results, ResultRow
     RoundPolicy pol;
     pol.round_ns = 0;  // one unbounded round: every tile in one launch per class
     pol.shortest_first = false;
-    pol.direct = true;
+    pol.direct = !opt.persistent;
     pol.use_graph = opt.use_graph;
     RoundStats rs{};
     const auto t1 = Clock::now();

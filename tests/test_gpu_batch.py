@@ -23,11 +23,11 @@ def mixed_batch(seed, n):
     return out
 
 
-def run_batch(searches, k, use_graph=False):
+def run_batch(searches, k, use_graph=False, persistent=False):
     import slime_amd
     cols = list(zip(*searches))
     return slime_amd.search_batch(cols[0], cols[1], cols[2], cols[3], cols[4], cols[5], cols[6],
-                                  max_results=k, use_graph=use_graph)
+                                  max_results=k, use_graph=use_graph, persistent=persistent)
 
 
 def test_batch_equals_separate_calls(gpu):
@@ -41,6 +41,16 @@ def test_batch_equals_separate_calls(gpu):
         assert (total, rows) == want
     st = slime_amd.batch_stats()
     assert st["fallbacks"] == 0 and st["jobs"] >= len(searches)
+
+
+def test_batch_persistent_equals_direct(gpu):
+    """The work-queue form (persistent kernels claiming tiles) and the
+    one-workgroup-per-tile form give the same results, also from a graph."""
+    searches = mixed_batch(12, 20)
+    direct = run_batch(searches, 200)
+    assert run_batch(searches, 200, persistent=True) == direct
+    for _ in range(2):
+        assert run_batch(searches, 200, use_graph=True, persistent=True) == direct
 
 
 def test_batch_small_searches_vs_oracle(gpu, oracle):
