@@ -636,9 +636,12 @@ bool Sweeper::round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& p
         l.first_off = (int32_t)firsts.size();
         l.blocks = 0;
         if (direct) {
+            // a launch's grid holds at most 2^31 - 1 workgroups; the rest of
+            // the tiles wait for the next round
+            const int64_t cap = std::min<int64_t>(want, 0x7FFFFFFF);
             firsts.push_back(0);
             for (const Item& it : v) {
-                const int64_t n = std::min<int64_t>(it.left, want - l.blocks);
+                const int64_t n = std::min<int64_t>(it.left, cap - l.blocks);
                 if (n <= 0) break;
                 order.push_back(it.job);
                 starts.push_back((int32_t)it.start);
