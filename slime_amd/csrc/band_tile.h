@@ -199,26 +199,31 @@ __device__ __forceinline__ uint32_t group_row(const uint32_t* ring, int first, u
 //  2. A group whose union count reaches min_size (with UB, every group the
 //     lanes found): exact counts (exact_group).
 // Exact 221-cell counts of the 16 centers of one coded group by the whole
-// wave: lane 4c + part sums the disk rows dz = -8 + part + 4t of center c, a
-// quad reduction gives the count, and lane 4c emits it.
+// wave: lane r < 20 reads window row r once; lane 4c + part takes the disk
+// rows dz = -8 + part + 4t of center c from those lanes (ds_bpermute), a quad
+// reduction gives the count, and lane 4c emits it.
 template <int NB, typename EmitCenter>
 __device__ __forceinline__ void exact_group(uint32_t code, const uint32_t* ring, int first,
                                             const EmitCenter& emit) {
     const int lane = threadIdx.x & 63;
+    const uint32_t row = group_row<NB>(ring, first, code, lane < 20 ? lane : 19);
     const int center = lane >> 2, part = lane & 3;
     const int i = center & 3, jr = center >> 2;
     uint32_t cnt = 0;
 #pragma unroll
     for (int t = 0; t < 5; ++t) {
+        // dz <= 8 except for t = 4, part > 0; every lane takes part in the
+        // shuffle (a lane outside it would read as zero)
         const int dz = -8 + part + 4 * t;
+        const uint32_t w = (uint32_t)__shfl((int)row, 8 + jr + (dz <= 8 ? dz : 8));
         if (dz <= 8) {
             const int hw = half_width_of(dz < 0 ? -dz : dz);
-            const uint32_t w = group_row<NB>(ring, first, code, 8 + jr + dz);
             cnt += __popc(w & (((1u << (2 * hw + 1)) - 1u) << (8 + i - hw)));
         }
     }
-    cnt += __shfl_xor(cnt, 1);
-    cnt += __shfl_xor(cnt, 2);
+    // quad sums by DPP quad permutes: lanes 1,0,3,2 then 2,3,0,1
+    cnt += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)cnt, 0xB1, 0xF, 0xF, false);
+    cnt += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)cnt, 0x4E, 0xF, 0xF, false);
     if (part == 0) emit(code, i, jr, cnt);
 }
 
