@@ -367,6 +367,7 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
     // remain). Groups that pass it are refined as with the square bound.
     constexpr bool kCorner = UB && kWord;
     constexpr bool kWSplit = PH == 2 && NB <= 8;
+    constexpr int kWThreads = 16 * NB * (kWSplit ? 2 : 1);  // threads that hold a band
     const int wd = tid & 15;
     const int wb = kWSplit ? tid >> 5 : tid >> 4;
     const int wph = kWSplit ? (tid >> 4) & 1 : 0;
@@ -459,79 +460,84 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
         const int32_t s_g = tile_s0 + 4 * m;  // first slot row of the groups
         uint32_t pass = 0;  // owned band-phases whose square reaches min_size
         if constexpr (kWord) {
-            // every thread takes part (the next-lane reads need whole rows)
-            uint32_t nhead[kWPh], ntail[kWPh];  // rows 0-1 and 2-3 of the strip
+            // every thread of a wave that holds a band takes part (the next-lane
+            // reads need whole rows); the waves past the last band skip it
+            if (wave * 64 < kWThreads) {
+                uint32_t nhead[kWPh], ntail[kWPh];  // rows 0-1 and 2-3 of the strip
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const uint32_t* rowp = ring + ((4 * k + r) % kRing) * R::kRowDw;
-                const uint32_t e = rowp[wd];
-                const uint32_t h = rowp[(wbc + 1) * kPlaneDw + wd];
-                uint32_t c[kWPh];
-                c[0] = nibble_counts(h & (e ^ wflip));
-                if constexpr (kWPh == 2) c[1] = nibble_counts(h) - c[0];  // the rest
+                for (int r = 0; r < 4; ++r) {
+                    const uint32_t* rowp = ring + ((4 * k + r) % kRing) * R::kRowDw;
+                    const uint32_t e = rowp[wd];
+                    const uint32_t h = rowp[(wbc + 1) * kPlaneDw + wd];
+                    uint32_t c[kWPh];
+                    c[0] = nibble_counts(h & (e ^ wflip));
+                    if constexpr (kWPh == 2) c[1] = nibble_counts(h) - c[0];  // the rest
 #pragma unroll
-                for (int ph = 0; ph < kWPh; ++ph) {  // two rows: at most 8 a nibble
-                    if (r == 0) nhead[ph] = c[ph];
-                    else if (r == 1) nhead[ph] += c[ph];
-                    else if (r == 2) ntail[ph] = c[ph];
-                    else ntail[ph] += c[ph];
+                    for (int ph = 0; ph < kWPh; ++ph) {  // two rows: at most 8 a nibble
+                        if (r == 0) nhead[ph] = c[ph];
+                        else if (r == 1) nhead[ph] += c[ph];
+                        else if (r == 2) ntail[ph] = c[ph];
+                        else ntail[ph] += c[ph];
+                    }
                 }
+#pragma unroll
+                for (int ph = 0; ph < kWPh; ++ph) {
+                    // the strip's counts as bytes: even nibble columns, odd ones
+                    const uint32_t se = (nhead[ph] & 0x0F0F0F0Fu) + (ntail[ph] & 0x0F0F0F0Fu);
+                    const uint32_t so =
+                        ((nhead[ph] >> 4) & 0x0F0F0F0Fu) + ((ntail[ph] >> 4) & 0x0F0F0F0Fu);
+                    we[ph] += se - he[ph][0];
+                    wo[ph] += so - ho[ph][0];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        he[ph][j] = he[ph][j + 1];
+                        ho[ph][j] = ho[ph][j + 1];
+                    }
+                    he[ph][4] = se;
+                    ho[ph][4] = so;
+                    // nibble columns n = 8 wd + j as 16-bit lanes: W_s holds
+                    // (n_s, n_{s+4}), N_s the next word's (n_{s+8}, n_{s+12})
+                    const uint32_t ne = from_next_lane16(we[ph]), no = from_next_lane16(wo[ph]);
+                    const uint32_t w0 = we[ph] & 0x00FF00FFu, w2 = (we[ph] >> 8) & 0x00FF00FFu;
+                    const uint32_t w1 = wo[ph] & 0x00FF00FFu, w3 = (wo[ph] >> 8) & 0x00FF00FFu;
+                    const uint32_t n0 = ne & 0x00FF00FFu, n2 = (ne >> 8) & 0x00FF00FFu;
+                    const uint32_t n1 = no & 0x00FF00FFu, n3 = (no >> 8) & 0x00FF00FFu;
+                    // square of group s (lane 0) and s + 4 (lane 1): 5 nibble columns
+                    uint32_t q0 = w0 + w1 + w2 + w3 + __builtin_amdgcn_alignbit(n0, w0, 16);
+                    uint32_t q1 = q0 - w0 + __builtin_amdgcn_alignbit(n1, w1, 16);
+                    uint32_t q2 = q1 - w1 + __builtin_amdgcn_alignbit(n2, w2, 16);
+                    uint32_t q3 = q2 - w2 + __builtin_amdgcn_alignbit(n3, w3, 16);
+                    if constexpr (kCorner) {
+                        // window rows 18, 19 (this strip's rows 2, 3) and 0, 1 (rows
+                        // 0, 1 of the strip four steps back), per nibble column as
+                        // bytes; a group loses its first and last column's counts
+                        const uint32_t old = hq[ph][0];
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) hq[ph][j] = hq[ph][j + 1];
+                        hq[ph][3] = nhead[ph];
+                        const uint32_t pe = (ntail[ph] & 0x0F0F0F0Fu) + (old & 0x0F0F0F0Fu);
+                        const uint32_t po =
+                            ((ntail[ph] >> 4) & 0x0F0F0F0Fu) + ((old >> 4) & 0x0F0F0F0Fu);
+                        const uint32_t npe = from_next_lane16(pe), npo = from_next_lane16(po);
+                        const uint32_t p0 = pe & 0x00FF00FFu, p2 = (pe >> 8) & 0x00FF00FFu;
+                        const uint32_t p1 = po & 0x00FF00FFu, p3 = (po >> 8) & 0x00FF00FFu;
+                        q0 -= p0 + __builtin_amdgcn_alignbit(npe & 0x00FF00FFu, p0, 16);
+                        q1 -= p1 + __builtin_amdgcn_alignbit(npo & 0x00FF00FFu, p1, 16);
+                        q2 -= p2 + __builtin_amdgcn_alignbit((npe >> 8) & 0x00FF00FFu, p2, 16);
+                        q3 -= p3 + __builtin_amdgcn_alignbit((npo >> 8) & 0x00FF00FFu, p3, 16);
+                    }
+                    const uint32_t c0 = q0 + wbias, c1 = q1 + wbias, c2 = q2 + wbias,
+                                   c3 = q3 + wbias;
+                    const uint32_t pm =
+                        ((c0 >> 15) & 1u) | ((c1 >> 14) & 2u) | ((c2 >> 13) & 4u) |
+                        ((c3 >> 12) & 8u) | ((c0 >> 27) & 16u) | ((c1 >> 26) & 32u) |
+                        ((c2 >> 25) & 64u) | ((c3 >> 24) & 128u);
+                    pass |= (pm & wvalid[ph]) << (8 * ph);
+                }
+                if (!w_active || m < 0 || s_g >= p.s_end ||
+                    (int64_t)wb * kPeriod + s_g >= p.h_valid)
+                    pass = 0;
             }
-#pragma unroll
-            for (int ph = 0; ph < kWPh; ++ph) {
-                // the strip's counts as bytes: even nibble columns, odd ones
-                const uint32_t se = (nhead[ph] & 0x0F0F0F0Fu) + (ntail[ph] & 0x0F0F0F0Fu);
-                const uint32_t so =
-                    ((nhead[ph] >> 4) & 0x0F0F0F0Fu) + ((ntail[ph] >> 4) & 0x0F0F0F0Fu);
-                we[ph] += se - he[ph][0];
-                wo[ph] += so - ho[ph][0];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    he[ph][j] = he[ph][j + 1];
-                    ho[ph][j] = ho[ph][j + 1];
-                }
-                he[ph][4] = se;
-                ho[ph][4] = so;
-                // nibble columns n = 8 wd + j as 16-bit lanes: W_s holds
-                // (n_s, n_{s+4}), N_s the next word's (n_{s+8}, n_{s+12})
-                const uint32_t ne = from_next_lane16(we[ph]), no = from_next_lane16(wo[ph]);
-                const uint32_t w0 = we[ph] & 0x00FF00FFu, w2 = (we[ph] >> 8) & 0x00FF00FFu;
-                const uint32_t w1 = wo[ph] & 0x00FF00FFu, w3 = (wo[ph] >> 8) & 0x00FF00FFu;
-                const uint32_t n0 = ne & 0x00FF00FFu, n2 = (ne >> 8) & 0x00FF00FFu;
-                const uint32_t n1 = no & 0x00FF00FFu, n3 = (no >> 8) & 0x00FF00FFu;
-                // square of group s (lane 0) and s + 4 (lane 1): 5 nibble columns
-                uint32_t q0 = w0 + w1 + w2 + w3 + __builtin_amdgcn_alignbit(n0, w0, 16);
-                uint32_t q1 = q0 - w0 + __builtin_amdgcn_alignbit(n1, w1, 16);
-                uint32_t q2 = q1 - w1 + __builtin_amdgcn_alignbit(n2, w2, 16);
-                uint32_t q3 = q2 - w2 + __builtin_amdgcn_alignbit(n3, w3, 16);
-                if constexpr (kCorner) {
-                    // window rows 18, 19 (this strip's rows 2, 3) and 0, 1 (rows
-                    // 0, 1 of the strip four steps back), per nibble column as
-                    // bytes; a group loses its first and last column's counts
-                    const uint32_t old = hq[ph][0];
-#pragma unroll
-                    for (int j = 0; j < 3; ++j) hq[ph][j] = hq[ph][j + 1];
-                    hq[ph][3] = nhead[ph];
-                    const uint32_t pe = (ntail[ph] & 0x0F0F0F0Fu) + (old & 0x0F0F0F0Fu);
-                    const uint32_t po =
-                        ((ntail[ph] >> 4) & 0x0F0F0F0Fu) + ((old >> 4) & 0x0F0F0F0Fu);
-                    const uint32_t npe = from_next_lane16(pe), npo = from_next_lane16(po);
-                    const uint32_t p0 = pe & 0x00FF00FFu, p2 = (pe >> 8) & 0x00FF00FFu;
-                    const uint32_t p1 = po & 0x00FF00FFu, p3 = (po >> 8) & 0x00FF00FFu;
-                    q0 -= p0 + __builtin_amdgcn_alignbit(npe & 0x00FF00FFu, p0, 16);
-                    q1 -= p1 + __builtin_amdgcn_alignbit(npo & 0x00FF00FFu, p1, 16);
-                    q2 -= p2 + __builtin_amdgcn_alignbit((npe >> 8) & 0x00FF00FFu, p2, 16);
-                    q3 -= p3 + __builtin_amdgcn_alignbit((npo >> 8) & 0x00FF00FFu, p3, 16);
-                }
-                const uint32_t c0 = q0 + wbias, c1 = q1 + wbias, c2 = q2 + wbias,
-                               c3 = q3 + wbias;
-                const uint32_t pm = ((c0 >> 15) & 1u) | ((c1 >> 14) & 2u) | ((c2 >> 13) & 4u) |
-                                    ((c3 >> 12) & 8u) | ((c0 >> 27) & 16u) | ((c1 >> 26) & 32u) |
-                                    ((c2 >> 25) & 64u) | ((c3 >> 24) & 128u);
-                pass |= (pm & wvalid[ph]) << (8 * ph);
-            }
-            if (!w_active || m < 0 || s_g >= p.s_end || (int64_t)wb * kPeriod + s_g >= p.h_valid)
-                pass = 0;
         } else if (counts) {
             uint32_t c4[C::kOwned];
             // UB: this strip's corner cells at offsets 16+r (now) and r (in
