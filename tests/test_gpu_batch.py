@@ -1,4 +1,5 @@
-"""Batched searches (one persistent sweep per kernel class) against the
+"""Batched searches (one launch per kernel class over every search's tiles,
+or the persistent work-queue kernels) against the
 single-search path and the CPU oracle: every search's total and ranked rows
 must be identical to its own separate call."""
 import random
