@@ -119,35 +119,50 @@ C1 = (0x123456789ABCDEF0, 2048, 40)  # seed, radius, threshold of the CPU-runnab
 def c1_leg():
     """The reference's CPU-runnable search (radius 2,048 around (0,0), threshold
     40, every hit returned): the GPU call against the reference SlimeCore on
-    the host's cores, rows compared."""
+    the host's cores, rows compared. Both are timed as C ABI calls into a
+    result buffer allocated once (the latency a frontend sees); the Python
+    wrapper's figure (a new 65,536-row buffer and tuple rows per call) is
+    reported beside it."""
     from oracle import pyoracle as P
     import slime_amd
     seed, rd, thr = C1
     k = 65536
     centers = (2 * rd + 1) ** 2
-    gpu = slime_amd.search_slime_clusters_centered(seed, rd, thr, 221, 0, 0, 0, k)  # warm-up
+    gpu = slime_amd.search_slime_clusters_centered(seed, rd, thr, 221, 0, 0, 0, k)
+    lib = slime_amd.lib()
+    buf = (slime_amd.ExtChunkResult * k)()
     times = []
+    for _ in range(21):
+        t0 = time.perf_counter()
+        n = lib.search_slime_clusters_centered(seed, rd, thr, 221, 0, 0, 0, buf, k, 0, 0)
+        times.append(time.perf_counter() - t0)
+        if n != gpu[0]:
+            raise SystemExit("C1 results differ between runs")
+    wrapped = []
     for _ in range(5):
         t0 = time.perf_counter()
         got = slime_amd.search_slime_clusters_centered(seed, rd, thr, 221, 0, 0, 0, k)
-        times.append(time.perf_counter() - t0)
+        wrapped.append(time.perf_counter() - t0)
         if got != gpu:
             raise SystemExit("C1 results differ between runs")
     out = {"workload": f"seed {seed:#x}, radius {rd} around (0,0), threshold {thr}, all hits",
            "centers": centers, "hits": gpu[0], "gpu_ms": statistics.median(times) * 1e3,
+           "gpu_ms_python_wrapper": statistics.median(wrapped) * 1e3,
            "gpu_t_centers_per_s": centers / statistics.median(times) / 1e12}
     ref = P.ref_lib()
     if ref is not None:
         threads = max(1, min(16, os.cpu_count() or 1))
-        want = P.ref_search(ref, seed, rd, thr, 221, 0, 0, 0, k, threads)  # warm-up
+        want = P.ref_search(ref, seed, rd, thr, 221, 0, 0, 0, k, threads)  # warm-up, rows
+        rbuf = (P.ChunkRow * k)()
         ct = []
-        for _ in range(3):
+        for _ in range(5):
+            ref.reset_cancel()
             t0 = time.perf_counter()
-            want = P.ref_search(ref, seed, rd, thr, 221, 0, 0, 0, k, threads)
+            ref.search_slime_clusters_centered(seed, rd, thr, 221, 0, 0, 0, rbuf, k, threads, 0)
             ct.append(time.perf_counter() - t0)
         out.update({"reference_cpu_ms": statistics.median(ct) * 1e3, "reference_cores": threads,
                     "identical": want == gpu})
-    print(f"C1: {gpu[0]} hits, gpu {out['gpu_ms']:.2f} ms", file=sys.stderr, flush=True)
+    print(f"C1: {gpu[0]} hits, gpu {out['gpu_ms']:.3f} ms", file=sys.stderr, flush=True)
     return out
 
 
