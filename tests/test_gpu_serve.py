@@ -91,7 +91,8 @@ def test_admission_by_footprint_queues_and_finishes(gpu):
 
 def test_pause_holds_progress_and_cancel_queued(gpu):
     import slime_amd
-    big = (0x1234567, 150000, 55, 221, 0, 1000, -1000)
+    # about 0.2 s of device work, so that the pause lands while it runs
+    big = (0x1234567, 400000, 55, 221, 0, 1000, -1000)
     with slime_amd.Session(hbm_budget=1 << 30, max_active=1, round_ms=2) as s:
         rid = s.submit(*big, max_results=100)
         queued = s.submit(*big[:1], 100, 50, 221, 0, 0, 0, max_results=10)
