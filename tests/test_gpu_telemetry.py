@@ -50,3 +50,52 @@ def test_refinement_progress(gpu):
     L.reset_cancel()
     slime_amd.refine_candidates_y(42, rows)
     assert L.get_progress() == 99
+
+
+def test_pause_and_cancel_a_running_search(gpu):
+    """request_pause holds a running search between launches (the processed
+    centers stop moving), request_cancel ends it early with the hits found so
+    far ranked, and reset_cancel lets the next search run in full
+    (SlimeCore.cpp:341-362, SlimeCoreGPU.cu:3085-3108, 3754-3757)."""
+    import threading
+    import time
+    import slime_amd
+    L = gpu
+    L.reset_cancel()
+    rd, thr = 1_500_000, 58        # about a second of scanning in launches of ~70 ms
+    full = (2 * rd + 1) ** 2
+    out = {}
+
+    def run():
+        out["r"] = slime_amd.search_slime_clusters_gpu_v1_centered(0x5EED, rd, thr, 221, 0, 0, 0,
+                                                                   max_results=1000)
+
+    t = threading.Thread(target=run)
+    t.start()
+    try:
+        deadline = time.time() + 60
+        while L.get_processed_centers() == 0 and t.is_alive() and time.time() < deadline:
+            time.sleep(0.002)
+        assert t.is_alive(), "the search ended before it could be paused"
+        L.request_pause()
+        time.sleep(0.3)            # the launch in flight completes
+        p1 = L.get_processed_centers()
+        time.sleep(0.3)
+        p2 = L.get_processed_centers()
+        assert p1 == p2 and t.is_alive()
+        L.request_cancel()
+        t.join(60)
+        assert not t.is_alive()
+    finally:
+        L.request_cancel()
+        t.join(120)
+        done = L.get_processed_centers()
+        L.reset_cancel()
+    total, rows = out["r"]
+    assert 0 < done < full
+    assert total >= len(rows)
+    assert rows == sorted(rows, key=lambda r: (-r[0], r[1], r[2]))
+    assert all(thr <= r[0] <= 221 for r in rows)
+    assert L.get_processed_centers() == 0
+    total2, _ = slime_amd.search_slime_clusters_gpu_v1_centered(7, 3000, 45, 221, 0, 0, 0, 100)
+    assert total2 >= 0 and L.get_processed_centers() == 6001 * 6001 and L.get_progress() == 90
