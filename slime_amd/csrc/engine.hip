@@ -39,7 +39,8 @@ Control& control() {
 
 namespace {
 
-constexpr uint32_t kHitCap = 1u << 24;                       // hits per launch (192 MB)
+constexpr uint32_t kHitCap = 1u << 24;                       // hits per launch (256 MiB)
+constexpr size_t kEagerHits = 4096;  // hits copied back with every launch's counters (64 KB)
 constexpr int64_t kPlainCentersPerLaunch = int64_t(1) << 37;
 constexpr int64_t kBandCentersPerLaunch = int64_t(1) << 39;
 constexpr int kBandTileH = 2048;
@@ -251,10 +252,16 @@ struct Work {
 enum class RunResult { kOk, kSplit, kError };
 
 RunResult run_work(Work& wk, Ranker& rk) {
+    // The first hits come back with the counters (no second round trip when a
+    // launch emits at most this many); the rest are copied after the count is
+    // known.
+    const size_t eager = std::min<size_t>(kEagerHits, (size_t)kHitCap);
+    if (!ensure_host_hits(eager)) return RunResult::kError;
     for (;;) {
         hipError_t launch_err;
-        if (hipMemsetAsync(g_buf.d_counters, 0, sizeof(unsigned long long) * (1 + kHistBins),
-                           g_buf.stream) != hipSuccess ||
+        // a fill kernel: cheaper to issue than hipMemsetAsync
+        if (launch_fill_u32(reinterpret_cast<uint32_t*>(g_buf.d_counters), 2 * (1 + kHistBins),
+                            0u, g_buf.stream) != hipSuccess ||
             hipEventRecord(g_buf.ev_start, g_buf.stream) != hipSuccess)
             return RunResult::kError;
         if (wk.banded) {
@@ -278,6 +285,8 @@ RunResult run_work(Work& wk, Ranker& rk) {
             hipMemcpyAsync(g_buf.h_counters, g_buf.d_counters,
                            sizeof(unsigned long long) * (1 + kHistBins), hipMemcpyDeviceToHost,
                            g_buf.stream) != hipSuccess ||
+            hipMemcpyAsync(g_buf.h_hits, g_buf.d_hits, sizeof(Hit) * eager,
+                           hipMemcpyDeviceToHost, g_buf.stream) != hipSuccess ||
             hipStreamSynchronize(g_buf.stream) != hipSuccess)
             return RunResult::kError;
         float ms = 0.f;
@@ -297,7 +306,7 @@ RunResult run_work(Work& wk, Ranker& rk) {
             }
             return RunResult::kSplit;
         }
-        if (emitted > 0) {
+        if (emitted > eager) {
             if (!ensure_host_hits(emitted) ||
                 hipMemcpyAsync(g_buf.h_hits, g_buf.d_hits, sizeof(Hit) * emitted,
                                hipMemcpyDeviceToHost, g_buf.stream) != hipSuccess ||
@@ -437,17 +446,21 @@ void plan_banded(const SearchSpec& spec, std::deque<Work>& out) {
 // Finds the first-draw mismatches of the region and recounts every affected
 // center exactly. Returns false on a device error.
 bool prepare_fixes(const SearchSpec& spec, MismatchIndex& index, Ranker& rk, FixList& fixes) {
-    if (hipMemsetAsync(g_buf.d_mismatch_count, 0, sizeof(unsigned int), g_buf.stream) != hipSuccess ||
+    if (launch_fill_u32(g_buf.d_mismatch_count, 1, 0u, g_buf.stream) != hipSuccess ||
         launch_find_mismatches(spec.seed, (int32_t)((int64_t)spec.x0 - 8), spec.w + 16,
                                (int32_t)((int64_t)spec.z0 - 8), spec.h + 16, g_buf.d_table,
                                g_buf.table_n, g_buf.d_mismatch, kMismatchCap,
                                g_buf.d_mismatch_count, g_buf.stream) != hipSuccess)
         return false;
+    // the count comes back through the pinned counter buffer (free until the
+    // first scan launch)
     unsigned int count = 0;
-    if (hipMemcpyAsync(&count, g_buf.d_mismatch_count, sizeof(count), hipMemcpyDeviceToHost,
-                       g_buf.stream) != hipSuccess ||
-        hipStreamSynchronize(g_buf.stream) != hipSuccess || count > kMismatchCap)
+    if (hipMemcpyAsync(g_buf.h_counters, g_buf.d_mismatch_count, sizeof(count),
+                       hipMemcpyDeviceToHost, g_buf.stream) != hipSuccess ||
+        hipStreamSynchronize(g_buf.stream) != hipSuccess)
         return false;
+    std::memcpy(&count, g_buf.h_counters, sizeof(count));
+    if (count > kMismatchCap) return false;
     std::vector<MismatchChunk>& mm = index.mm;
     mm.resize(count);
     if (count && (hipMemcpy(mm.data(), g_buf.d_mismatch, sizeof(MismatchChunk) * count,
