@@ -1,7 +1,9 @@
 """ctypes bindings for the CPU checker (test infrastructure only).
 
-Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
-this module. It loads:
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline legs (the
+full-world sample, the reference timing and row check of the radius-2,048
+search, the AFK-Y refinement timing) import this module: there it is the
+timed CPU baseline and the checker of the GPU's rows, never the GPU path. It loads:
   * liboracle.so - the scalar C restatement in this folder;
   * _ref/libslimecore_ref[_avx2].so - the reference's SlimeCore.cpp compiled
     unmodified by `make -C oracle ref` (present only where it was built).

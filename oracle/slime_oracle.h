@@ -3,7 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY. Nothing in the product (slime_amd/, include/)
  * links, loads or calls this code. Only tests/, __graft_entry__.smoke() and
- * bench.py's cpu_baseline leg use it, as the checker.
+ * bench.py's CPU-baseline legs use it, as the checker and the timed CPU
+ * baseline.
  *
  * Each function is a scalar, from-scratch restatement of the reference
  * algorithm and cites the reference file:line it follows. The tests under
