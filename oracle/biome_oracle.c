@@ -15,15 +15,22 @@
  * of slime_amd/csrc/biome_data.inc. Compiled with -ffp-contract=off so every
  * double and float operation rounds as written.
  *
- * Pinning (tests/test_biome_oracle.py): the noise seeds are MD5 digests of the
- * noise names; xoroshiro and Perlin against independent Python restatements
- * of the published algorithms; the spline against the reference's own spline
- * source compiled in oracle/_ref; the tree table against the reference's
- * table text; the pruned tree search against an exhaustive nearest-leaf
- * search, and the search started from a known leaf against the search from
- * the root. Biome ids
- * against the reference's CUDA build or cubiomes: parity unpinned (neither
- * runs here). */
+ * Pinning (tests/test_biome_oracle.py, tests/test_biome_reference_text.py):
+ * the noise seeds are MD5 digests of the noise names and equal the
+ * reference's text; xoroshiro and Perlin against independent Python
+ * restatements of the published algorithms; every constant of the climate
+ * composition (octave seeds, amplitudes, first octaves, octave counts,
+ * lacunarity, persistence, double-Perlin amplitude and factor, shift,
+ * quantization, depth terms) is read from MinecraftGPUNoise.cu's text, and
+ * the climate doubles, quantized parameters and spline offsets of this file
+ * equal a Python composition built from those constants bit for bit; the
+ * spline equals spline_generated.cuh's text transliterated into numpy
+ * float32/float64; the filter's sampling set equals the reference kernel's;
+ * the tree table equals the reference's table text; the pruned tree search
+ * equals an exhaustive nearest-leaf search, and the search started from a
+ * known leaf equals the search from the root. Biome ids against the
+ * reference's CUDA build (nvcc contracts multiply-adds into FMAs by default)
+ * or cubiomes: parity unpinned (neither runs here). */
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>

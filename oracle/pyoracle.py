@@ -222,69 +222,63 @@ def biome_lib():
         path = os.path.join(HERE, "libbiome_oracle.so")
         if not os.path.exists(path):
             build()
-        L = C.CDLL(path)
-        u64, i32, i64, P32 = C.c_uint64, C.c_int32, C.c_int64, C.POINTER(C.c_int32)
-        L.ob_xoro_stream.argtypes = [u64, C.c_int, C.POINTER(u64)]
-        L.ob_xoro_stream.restype = None
-        L.ob_perlin_sample_seeded.argtypes = [u64, C.c_double, C.c_double, C.c_double]
-        L.ob_perlin_sample_seeded.restype = C.c_double
-        L.ob_climate_new.argtypes = [u64, C.c_int]
-        L.ob_climate_new.restype = C.c_void_p
-        L.ob_climate_free.argtypes = [C.c_void_p]
-        L.ob_climate_free.restype = None
-        L.ob_column.argtypes = [C.c_void_p, i32, i32, P32]
-        L.ob_column.restype = None
-        ci = C.c_int
-        L.ob_biome_at.argtypes = [C.c_void_p, i32, i32, i32, ci]
-        L.ob_biome_at.restype = ci
-        L.ob_filter.argtypes = [C.c_void_p, i64, i32, i32, ci, ci]
-        L.ob_filter.restype = ci
-        L.ob_filter_levels.argtypes = [C.c_void_p, i64, i32, i32, ci, i32, i32, i32, ci]
-        L.ob_filter_levels.restype = ci
-        L.ob_spline.argtypes = [C.POINTER(C.c_float)]
-        L.ob_spline.restype = C.c_float
-        L.ob_tree_for_version.argtypes = [ci]
-        L.ob_tree_for_version.restype = ci
-        L.ob_tree_leaf.argtypes = [P32, ci]
-        L.ob_tree_leaf.restype = ci
-        L.ob_tree_leaf_from.argtypes = [P32, ci, ci]
-        L.ob_tree_leaf_from.restype = ci
-        L.ob_tree_first_descent.argtypes = [ci]
-        L.ob_tree_first_descent.restype = ci
-        L.ob_tree_leaf_exhaustive.argtypes = [P32, ci]
-        L.ob_tree_leaf_exhaustive.restype = ci
-        L.ob_node_distance.argtypes = [P32, ci, ci]
-        L.ob_node_distance.restype = u64
-        L.ob_leaf_biome.argtypes = [ci, ci]
-        L.ob_leaf_biome.restype = ci
-        L.ob_tree_len.argtypes = [ci]
-        L.ob_tree_len.restype = ci
-        L.ob_tree_family.argtypes = [ci]
-        L.ob_tree_family.restype = ci
-        L.ob_tree_node.argtypes = [ci, ci]
-        L.ob_tree_node.restype = u64
-        L.ob_climate5.argtypes = [C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_double)]
-        L.ob_climate5.restype = None
-        L.ob_climate6.argtypes = [C.c_void_p, i32, i32, i32, P32]
-        L.ob_climate6.restype = None
-        L.ob_continentalness.argtypes = [C.c_void_p, C.c_double, C.c_double]
-        L.ob_continentalness.restype = C.c_double
-        _biome = L
+        _biome = load_biome_lib(path)
     return _biome
+
+
+def load_biome_lib(path):
+    """A build of biome_oracle.c at path, with its argument types set."""
+    L = C.CDLL(path)
+    u64, i32, i64, P32 = C.c_uint64, C.c_int32, C.c_int64, C.POINTER(C.c_int32)
+    L.ob_xoro_stream.argtypes = [u64, C.c_int, C.POINTER(u64)]
+    L.ob_xoro_stream.restype = None
+    L.ob_perlin_sample_seeded.argtypes = [u64, C.c_double, C.c_double, C.c_double]
+    L.ob_perlin_sample_seeded.restype = C.c_double
+    L.ob_climate_new.argtypes = [u64, C.c_int]
+    L.ob_climate_new.restype = C.c_void_p
+    L.ob_climate_free.argtypes = [C.c_void_p]
+    L.ob_climate_free.restype = None
+    L.ob_column.argtypes = [C.c_void_p, i32, i32, P32]
+    L.ob_column.restype = None
+    ci = C.c_int
+    L.ob_biome_at.argtypes = [C.c_void_p, i32, i32, i32, ci]
+    L.ob_biome_at.restype = ci
+    L.ob_filter.argtypes = [C.c_void_p, i64, i32, i32, ci, ci]
+    L.ob_filter.restype = ci
+    L.ob_filter_levels.argtypes = [C.c_void_p, i64, i32, i32, ci, i32, i32, i32, ci]
+    L.ob_filter_levels.restype = ci
+    L.ob_spline.argtypes = [C.POINTER(C.c_float)]
+    L.ob_spline.restype = C.c_float
+    L.ob_tree_for_version.argtypes = [ci]
+    L.ob_tree_for_version.restype = ci
+    L.ob_tree_leaf.argtypes = [P32, ci]
+    L.ob_tree_leaf.restype = ci
+    L.ob_tree_leaf_from.argtypes = [P32, ci, ci]
+    L.ob_tree_leaf_from.restype = ci
+    L.ob_tree_first_descent.argtypes = [ci]
+    L.ob_tree_first_descent.restype = ci
+    L.ob_tree_leaf_exhaustive.argtypes = [P32, ci]
+    L.ob_tree_leaf_exhaustive.restype = ci
+    L.ob_node_distance.argtypes = [P32, ci, ci]
+    L.ob_node_distance.restype = u64
+    L.ob_leaf_biome.argtypes = [ci, ci]
+    L.ob_leaf_biome.restype = ci
+    L.ob_tree_len.argtypes = [ci]
+    L.ob_tree_len.restype = ci
+    L.ob_tree_family.argtypes = [ci]
+    L.ob_tree_family.restype = ci
+    L.ob_tree_node.argtypes = [ci, ci]
+    L.ob_tree_node.restype = u64
+    L.ob_climate5.argtypes = [C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_double)]
+    L.ob_climate5.restype = None
+    L.ob_climate6.argtypes = [C.c_void_p, i32, i32, i32, P32]
+    L.ob_climate6.restype = None
+    L.ob_continentalness.argtypes = [C.c_void_p, C.c_double, C.c_double]
+    L.ob_continentalness.restype = C.c_double
+    return L
 
 
 # The oracle's trees in order (index = tree argument): file suffix of the
 # reference's gpu_noise/tables/btree<suffix>.h and first version code.
 BIOME_TREES = [("262", 2602), ("215", 1215), ("21wd", 1214), ("20", 1206), ("19", 1194),
                ("192", 1192), ("18", 1180)]
-
-
-def ref_biome_lib():
-    """_ref/libbiome_ref.so (the reference's spline source), or None."""
-    path = os.path.join(HERE, "_ref", "libbiome_ref.so")
-    if not os.path.exists(path):
-        return None
-    L = C.CDLL(path)
-    L.ref_depth_spline.argtypes = [C.POINTER(C.c_float)]
-    L.ref_depth_spline.restype = C.c_float
-    return L

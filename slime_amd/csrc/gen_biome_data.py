@@ -125,8 +125,9 @@ def main():
     w("// Generated by gen_biome_data.py; do not edit. Data of Minecraft's overworld")
     w("// multi-noise biome source (versions 1.18 to 26.2): noise seeds (MD5 of the")
     w("// noise names), the biome parameter search trees and the terrain-offset")
-    w("// spline, the latter two as encoded by cubiomes (MIT). BIOME_TABLE marks the")
-    w("// tables device code reads (the includer defines it).")
+    w("// spline, the latter two as encoded by cubiomes (Copyright (c) 2020 Cubitect,")
+    w("// MIT License; the notice is in LICENSE.cubiomes beside this file).")
+    w("// BIOME_TABLE marks the tables device code reads (the includer defines it).")
     w("")
     w("// Noise definitions: MD5 seed halves of the name (and of the large-biome")
     w("// name), first octave, number of octaves, amplitudes.")

@@ -4,8 +4,8 @@ reference's CUDA build:
   * noise seeds: MD5 digests of Minecraft's noise names;
   * xoroshiro128++ with Minecraft's seeding, and improved Perlin noise:
     against independent Python restatements of the published algorithms;
-  * the terrain spline: bit-exact against the reference's own spline source
-    compiled for the CPU (oracle/_ref, where built);
+  * the climate composition, its constants and the terrain spline: against
+    the reference's noise sources read as text (test_biome_reference_text.py);
   * the parameter trees of every version: the generated tables equal the
     reference's table text; the version selection; the pruned nearest-leaf
     search of both node layouts equals an exhaustive search.
@@ -13,7 +13,6 @@ Biome ids against the reference's CUDA build or cubiomes: parity unpinned.
 """
 import ctypes as C
 import hashlib
-import math
 import os
 import random
 import re
@@ -21,6 +20,7 @@ import struct
 
 import pytest
 
+from biome_restatement import ImprovedNoise, Xoroshiro128PlusPlus
 from conftest import ROOT
 
 INC = os.path.join(ROOT, "slime_amd", "csrc", "biome_data.inc")
@@ -62,93 +62,6 @@ def test_noise_seeds_are_md5_of_noise_names():
         assert (int(lo, 16), int(hi, 16)) == md5_halves(f"octave_{i}")
 
 
-# -- independent restatements of the published algorithms ---------------------------
-
-def rotl(x, r):
-    return ((x << r) | (x >> (64 - r))) & M64
-
-
-def mix13(z):
-    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
-    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
-    return z ^ (z >> 31)
-
-
-class Xoroshiro128PlusPlus:
-    """Blackman & Vigna's xoroshiro128++, seeded as Minecraft's
-    RandomSupport.upgradeSeedTo128bit (silver/golden ratio, Stafford mix 13)."""
-
-    def __init__(self, seed):
-        lo = (seed ^ 0x6A09E667F3BCC909) & M64
-        hi = (lo + 0x9E3779B97F4A7C15) & M64
-        self.s0, self.s1 = mix13(lo), mix13(hi)
-
-    def next_long(self):
-        s0, s1 = self.s0, self.s1
-        r = (rotl((s0 + s1) & M64, 17) + s0) & M64
-        s1 ^= s0
-        self.s0 = rotl(s0, 49) ^ s1 ^ ((s1 << 21) & M64)
-        self.s1 = rotl(s1, 28)
-        return r
-
-    def next_int(self, n):
-        while True:
-            r = (self.next_long() & 0xFFFFFFFF) * n
-            if (r & 0xFFFFFFFF) >= (-n % (1 << 32)) % n:
-                return r >> 32
-
-    def next_double(self):
-        return (self.next_long() >> 11) * 2.0 ** -53
-
-
-GRAD = [(1, 1, 0), (-1, 1, 0), (1, -1, 0), (-1, -1, 0), (1, 0, 1), (-1, 0, 1), (1, 0, -1),
-        (-1, 0, -1), (0, 1, 1), (0, -1, 1), (0, 1, -1), (0, -1, -1), (1, 1, 0), (0, -1, 1),
-        (-1, 1, 0), (0, -1, -1)]
-
-
-class ImprovedNoise:
-    """Ken Perlin's improved noise as Minecraft seeds it (origin from three
-    doubles, Fisher-Yates permutation)."""
-
-    def __init__(self, rng):
-        self.o = [rng.next_double() * 256.0 for _ in range(3)]
-        p = list(range(256))
-        for i in range(256):
-            j = i + rng.next_int(256 - i)
-            p[i], p[j] = p[j], p[i]
-        self.p = p + [p[0]]
-
-    @staticmethod
-    def fade(t):
-        return t * t * t * (t * (t * 6.0 - 15.0) + 10.0)
-
-    @staticmethod
-    def lerp(t, a, b):
-        return a + t * (b - a)
-
-    def grad(self, h, x, y, z):
-        g = GRAD[h & 15]
-        terms = [v if s > 0 else -v for s, v in zip(g, (x, y, z)) if s]
-        return terms[0] + terms[1]
-
-    def sample(self, x, y, z):
-        x, y, z = x + self.o[0], y + self.o[1], z + self.o[2]
-        fx, fy, fz = math.floor(x), math.floor(y), math.floor(z)
-        x, y, z = x - fx, y - fy, z - fz
-        X, Y, Z = int(fx) & 255, int(fy) & 255, int(fz) & 255
-        u, v, w = self.fade(x), self.fade(y), self.fade(z)
-        p = self.p
-        a, b = (p[X] + Y) & 255, (p[X + 1] + Y) & 255
-        aa, ba, ab, bb = (p[a] + Z) & 255, (p[b] + Z) & 255, (p[a + 1] + Z) & 255, (p[b + 1] + Z) & 255
-        L = self.lerp
-        g = self.grad
-        x0 = L(u, g(p[aa], x, y, z), g(p[ba], x - 1, y, z))
-        x1 = L(u, g(p[ab], x, y - 1, z), g(p[bb], x - 1, y - 1, z))
-        x2 = L(u, g(p[aa + 1], x, y, z - 1), g(p[ba + 1], x - 1, y, z - 1))
-        x3 = L(u, g(p[ab + 1], x, y - 1, z - 1), g(p[bb + 1], x - 1, y - 1, z - 1))
-        return L(w, L(v, x0, x1), L(v, x2, x3))
-
-
 def test_xoroshiro_matches_published_algorithm(bl):
     for seed in (0, 1, 12345, M64, 0x5EED5EED5EED):
         ref = Xoroshiro128PlusPlus(seed)
@@ -170,28 +83,6 @@ def test_perlin_matches_published_algorithm(bl):
 
 def f32(v):
     return struct.unpack("<f", struct.pack("<f", v))[0]
-
-
-def test_spline_equals_reference_source(bl):
-    from oracle import pyoracle
-    ref = pyoracle.ref_biome_lib()
-    if ref is None:
-        pytest.skip("oracle/_ref/libbiome_ref.so not built (needs the reference tree)")
-    rng = random.Random(3)
-    text = inc_text()
-    knots = [struct.unpack("<f", struct.pack("<I", int(h, 16)))[0]
-             for h in re.findall(r"\{0x([0-9a-f]{8})u, 0x[0-9a-f]{8}u, 0x[0-9a-f]{8}u, -?\d+\}", text)]
-    cases = 0
-    for i in range(20000):
-        v = [rng.uniform(-1.3, 1.3) for _ in range(4)]
-        if i % 4 == 0:  # land exactly on, just below and just above a knot
-            k = rng.choice(knots)
-            v[rng.randrange(3)] = k + rng.choice([0.0, -1e-7, 1e-7])
-        v[2] = f32(-(abs(abs(v[3]) - 0.6666667) - 0.33333334) * 3.0) if i % 2 else v[2]
-        a = (C.c_float * 4)(*[f32(x) for x in v])
-        assert struct.pack("<f", bl.ob_spline(a)) == struct.pack("<f", ref.ref_depth_spline(a)), v
-        cases += 1
-    assert cases == 20000
 
 
 def inc_tree(suffix):
