@@ -42,7 +42,8 @@ bool valid_args(int64_t rd_max, int64_t rd_max_limit, int32_t min_size, int32_t 
     if ((int64_t)cx - rd_max - 8 < INT32_MIN || (int64_t)cx + rd_max + 8 > INT32_MAX ||
         (int64_t)cz - rd_max - 8 < INT32_MIN || (int64_t)cz + rd_max + 8 > INT32_MAX)
         return false;
-    return true;
+    // this build's size limit (radius 8,388,607; the reference's is larger)
+    return slime::within_search_limit(2 * rd_max + 1, 2 * rd_max + 1);
 }
 
 // The square of centers within rd_max (Chebyshev) of (cx, cz), with the
@@ -181,7 +182,8 @@ int64_t slime_scan_rect(int64_t seed, int32_t x0, int32_t z0, int64_t w, int64_t
                         int32_t min_size, int32_t max_size, ExtChunkResult* results_buffer,
                         int32_t max_results, int64_t* hist222) {
     if (!results_buffer || max_results <= 0 || w <= 0 || h <= 0 || w > INT32_MAX ||
-        h > INT32_MAX || min_size < 0 || min_size > 221 || max_size < min_size)
+        h > INT32_MAX || !slime::within_search_limit(w, h) || min_size < 0 || min_size > 221 ||
+        max_size < min_size)
         return -1;
     slime::SearchSpec spec;
     spec.seed = seed;
@@ -199,8 +201,8 @@ int64_t slime_scan_bands(int64_t seed, int32_t x0, int32_t z0, int64_t w, int64_
                          int32_t band_rows, int32_t mode, int32_t min_size, int32_t max_size,
                          ExtChunkResult* results_buffer, int32_t max_results, int64_t* hist222) {
     if (!results_buffer || max_results <= 0 || w <= 0 || h <= 0 || w > INT32_MAX ||
-        h > INT32_MAX || band_rows < 0 || mode < 0 || mode > 4 || min_size < 0 ||
-        min_size > 221 || max_size < min_size)
+        h > INT32_MAX || !slime::within_search_limit(w, h) || band_rows < 0 || mode < 0 ||
+        mode > 4 || min_size < 0 || min_size > 221 || max_size < min_size)
         return -1;
     slime::SearchSpec spec;
     spec.seed = seed;

@@ -528,7 +528,7 @@ int64_t run_search(const SearchSpec& spec, ResultRow* out, int32_t max_results,
         ctl.progress.store(0, std::memory_order_relaxed);
         return -1;
     };
-    if (!ensure_buffers()) return fail();
+    if (!within_search_limit(spec.w, spec.h) || !ensure_buffers()) return fail();
 
     Ranker rk;
     rk.min_size = spec.min_size;
@@ -558,6 +558,7 @@ int64_t run_search(const SearchSpec& spec, ResultRow* out, int32_t max_results,
         g_last_kernel.store(16 + (int)(std::max_element(by_kernel, by_kernel + 3) - by_kernel),
                             std::memory_order_relaxed);
     bool cancelled = false;
+    std::vector<Work> finished;  // launches whose every center is decided
     while (!works.empty()) {
         wait_while_paused();
         if (ctl.cancel.load(std::memory_order_relaxed)) {
@@ -577,6 +578,7 @@ int64_t run_search(const SearchSpec& spec, ResultRow* out, int32_t max_results,
             continue;
         }
         const auto t1 = std::chrono::steady_clock::now();
+        finished.push_back(wk);
         done_centers += wk.centers;
         ctl.processed_centers.fetch_add(wk.centers, std::memory_order_relaxed);
         ctl.scan_work_ns.fetch_add(
@@ -585,14 +587,23 @@ int64_t run_search(const SearchSpec& spec, ResultRow* out, int32_t max_results,
         ctl.progress.store(1 + (int32_t)((done_centers * 79) / std::max<int64_t>(1, total_centers)),
                            std::memory_order_relaxed);
     }
-    // Replace the provisional first-draw counts of recounted centers.
-    if (!cancelled) {
-        for (const CenterFix& f : fixes) {
-            if (f.first_draw >= spec.min_size && f.first_draw <= spec.max_size) rk.hist[f.first_draw]--;
-            if (f.exact >= spec.min_size && f.exact <= spec.max_size) {
-                rk.hist[f.exact]++;
-                rk.cand.push_back(Hit{f.exact, f.cx, f.cz, 0});
-            }
+    // Replace the provisional first-draw counts of recounted centers: all of
+    // them, or after a cancel those the finished launches decided (the others
+    // were never counted).
+    auto decided = [&](const CenterFix& f) {
+        if (!cancelled) return true;
+        for (const Work& w : finished)
+            if (w.banded ? band_tile_of(w.band, w.nb, w.ph, f.cx, f.cz) >= 0
+                         : plain_covers(w.plain, f.cx, f.cz))
+                return true;
+        return false;
+    };
+    for (const CenterFix& f : fixes) {
+        if (!decided(f)) continue;
+        if (f.first_draw >= spec.min_size && f.first_draw <= spec.max_size) rk.hist[f.first_draw]--;
+        if (f.exact >= spec.min_size && f.exact <= spec.max_size) {
+            rk.hist[f.exact]++;
+            rk.cand.push_back(Hit{f.exact, f.cx, f.cz, 0});
         }
     }
 

@@ -219,6 +219,48 @@ inline void band_jobs(const SearchSpec& spec, const std::vector<XPart>& parts,
     }
 }
 
+// The tile of a band job that decides center (cx, cz), as an index into the
+// job's tiles (what band_tile's `tile` argument and the sweep's done counts
+// refer to), or -1 when the job does not cover that center. Tile t holds
+// x-tile tx = t % tiles_x (x-block tx / tiles_per_block, 496 slot columns
+// from slot_lo + (tx % tiles_per_block) * 496, both phases) and slot rows
+// s_begin + (t / tiles_x) * tile_h onwards, in every band of the job.
+inline int64_t band_tile_of(const BandParams& p, int nb, int ph, int64_t cx, int64_t cz) {
+    const int64_t ox = cx - (int64_t)p.x0;
+    if (ox < p.ox_min || ox >= p.w) return -1;
+    int64_t xblock = 0, c = ox;
+    if (ph == 2) {
+        xblock = ox / (2 * (int64_t)kPeriod);
+        c = ox - xblock * 2 * kPeriod;
+        if (c >= kPeriod) c -= kPeriod;
+    }
+    if (c < p.slot_lo || c >= p.phase_cols) return -1;
+    const int64_t oz = cz - (int64_t)p.z0;
+    if (oz < 0 || oz >= p.h_valid) return -1;
+    const int64_t b = oz / kPeriod, s = oz - b * kPeriod;
+    if (b >= nb || s < p.s_begin || s >= p.s_end) return -1;
+    const int64_t xtile = (c - p.slot_lo) / kBandTileOut;
+    const int64_t ts = (s - p.s_begin) / p.tile_h;
+    return ts * p.tiles_x + xblock * p.tiles_per_block + xtile;
+}
+
+// Whether a plain scan launch decides center (cx, cz).
+inline bool plain_covers(const ScanParams& p, int64_t cx, int64_t cz) {
+    const int64_t ox = cx - (int64_t)p.x0, oz = cz - (int64_t)p.z0;
+    return ox >= 0 && ox < p.w && oz >= 0 && oz < p.h;
+}
+
+// The largest region one search may cover, in centers. Its first-draw
+// mismatch list (about 209,712 chunks per 2^48) must fit kMismatchCap and its
+// recount list (up to 221 centers per mismatch, 24 bytes each: about 1.1 GB
+// at the limit) host memory. 2^48 centers is the square of radius 8,388,607
+// chunks, 4.5 times the world border's 1,875,000; the reference accepts radii
+// up to (2^31 - 20) / 2, which at its published rate would scan for weeks.
+constexpr int64_t kMaxSearchCenters = int64_t(1) << 48;
+inline bool within_search_limit(int64_t w, int64_t h) {
+    return w > 0 && h > 0 && w <= kMaxSearchCenters / h;
+}
+
 // Two phases over the whole 2P blocks from x0 and one phase over the
 // remaining columns (for regions wider than 2P).
 inline std::vector<XPart> x_parts_one_phase_tail(const SearchSpec& spec) {

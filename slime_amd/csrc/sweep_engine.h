@@ -51,6 +51,7 @@ struct SweepRequest {
     // result
     int64_t total = -1;
     bool fallback = false;
+    bool truncated = false;  // cancelled with more hits than its region held
     std::vector<ResultRow> rows;
 };
 
@@ -98,6 +99,7 @@ class Sweeper {
 
    private:
     bool ensure_fix_cap(size_t n);
+    bool prepare_fixes(const std::vector<SweepRequest*>& reqs);
 
     int device_ = -1;
     int cus_ = 256;

@@ -446,6 +446,18 @@ bool Sweeper::ensure_fix_cap(size_t n) {
 }
 
 bool Sweeper::prepare(const std::vector<SweepRequest*>& reqs) {
+    try {
+        return prepare_fixes(reqs);
+    } catch (const std::bad_alloc&) {
+        for (SweepRequest* r : reqs) {
+            r->mm.mm.clear();
+            r->fixes.clear();
+        }
+        return false;
+    }
+}
+
+bool Sweeper::prepare_fixes(const std::vector<SweepRequest*>& reqs) {
     if (reqs.empty()) return true;
     // 1. first-draw mismatches of every region in one launch
     std::vector<MismatchQuery> q;
@@ -763,7 +775,7 @@ bool Sweeper::finalize(SweepRequest& r, bool partial) {
                        stream_) != hipSuccess ||
         hipStreamSynchronize(stream_) != hipSuccess)
         return false;
-    if ((int64_t)emitted > r.hit_cap) {
+    if ((int64_t)emitted > r.hit_cap && !partial) {
         // more hits than the reserved region: rerun alone with a rising gate
         r.rows.assign((size_t)r.keep, ResultRow{});
         r.total = run_search(r.spec, r.rows.data(), r.keep);
@@ -771,10 +783,15 @@ bool Sweeper::finalize(SweepRequest& r, bool partial) {
         r.fallback = true;
         return r.total >= 0;
     }
-    std::vector<Hit> hits((size_t)emitted);
-    if (emitted && (hipMemcpyAsync(hits.data(), d_hits_ + r.hit_off, sizeof(Hit) * emitted,
-                                   hipMemcpyDeviceToHost, stream_) != hipSuccess ||
-                    hipStreamSynchronize(stream_) != hipSuccess))
+    // A cancelled request keeps what its finished tiles found: the total
+    // counts every decided center; the rows rank the stored hits (the first
+    // hit_cap emitted if the region overflowed: then r.truncated).
+    const size_t stored = (size_t)std::min<int64_t>((int64_t)emitted, r.hit_cap);
+    r.truncated = (int64_t)emitted > r.hit_cap;
+    std::vector<Hit> hits(stored);
+    if (stored && (hipMemcpyAsync(hits.data(), d_hits_ + r.hit_off, sizeof(Hit) * stored,
+                                  hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+                   hipStreamSynchronize(stream_) != hipSuccess))
         return false;
     int64_t h[kHistBins];
     for (int s = 0; s < kHistBins; ++s) h[s] = (int64_t)hist[s];
@@ -782,14 +799,23 @@ bool Sweeper::finalize(SweepRequest& r, bool partial) {
     cand.reserve(hits.size());
     for (const Hit& x : hits)
         if (!r.mm.affects(x.cx, x.cz)) cand.push_back(Hit{x.size, x.cx, x.cz, 0});
-    if (!partial) {
-        // replace the provisional first-draw counts of recounted centers
-        for (const CenterFix& f : r.fixes) {
-            if (f.first_draw >= r.spec.min_size && f.first_draw <= r.spec.max_size) h[f.first_draw]--;
-            if (f.exact >= r.spec.min_size && f.exact <= r.spec.max_size) {
-                h[f.exact]++;
-                cand.push_back(Hit{f.exact, f.cx, f.cz, 0});
-            }
+    // Replace the provisional first-draw counts of recounted centers: all of
+    // them, or for a cancelled request those in finished tiles (every job's
+    // tiles finish in index order, so its first done[j] tiles).
+    auto decided = [&](const CenterFix& f) {
+        if (!partial) return true;
+        for (size_t j = 0; j < r.jobs.size(); ++j) {
+            const int64_t t = band_tile_of(r.jobs[j].p, r.jobs[j].nb, r.jobs[j].ph, f.cx, f.cz);
+            if (t >= 0) return t < (int64_t)r.done[j];
+        }
+        return false;
+    };
+    for (const CenterFix& f : r.fixes) {
+        if (!decided(f)) continue;
+        if (f.first_draw >= r.spec.min_size && f.first_draw <= r.spec.max_size) h[f.first_draw]--;
+        if (f.exact >= r.spec.min_size && f.exact <= r.spec.max_size) {
+            h[f.exact]++;
+            cand.push_back(Hit{f.exact, f.cx, f.cz, 0});
         }
     }
     int64_t total = 0;
@@ -916,6 +942,7 @@ bool sweep_cost_table(double* ns_row, double* extra_rows) {
 namespace slime {
 
 struct Session::Entry {
+    int64_t id = 0;
     SweepRequest req;
     ReqState state = ReqState::kQueued;
     bool want_pause = false, want_resume = false, want_cancel = false;
@@ -946,16 +973,26 @@ Session::~Session() {
 }
 
 int64_t Session::submit(const SearchSpec& spec, int32_t keep) {
-    if (!ok_ || keep <= 0) return -1;
-    auto e = std::make_unique<Entry>();
-    e->req.spec = spec;
-    e->req.keep = keep;
-    e->t_submit = Clock::now();
-    if (!sw_.plan(e->req)) return -1;  // reads only the cost table
+    if (!ok_ || keep <= 0 || !within_search_limit(spec.w, spec.h)) return -1;
+    std::unique_ptr<Entry> e;
+    try {
+        e = std::make_unique<Entry>();
+        e->req.spec = spec;
+        e->req.keep = keep;
+        e->t_submit = Clock::now();
+        if (!sw_.plan(e->req)) return -1;  // reads only the cost table
+    } catch (const std::bad_alloc&) {
+        return -1;
+    }
     e->footprint = Sweeper::footprint(e->req);
-    if (e->footprint > budget_) return -1;  // could never be admitted
+    // requests that could never be admitted: over the HBM budget, or more
+    // jobs or hits than the session's device arrays hold
+    if (e->footprint > budget_ || (int64_t)e->req.jobs.size() > sw_.job_cap() ||
+        e->req.hit_cap > sw_.hit_cap())
+        return -1;
     std::lock_guard<std::mutex> lk(mu_);
     const int64_t id = next_id_++;
+    e->id = id;
     entries_[id] = std::move(e);
     queue_.push_back(id);
     cv_.notify_all();
@@ -1133,11 +1170,28 @@ void Session::worker() {
         std::vector<SweepRequest*> running;
         lk.unlock();
 
-        // 2. device work without the lock
+        // 2. device work without the lock. Fresh requests are prepared together;
+        // if that fails, one at a time, so that only those that fail alone (a
+        // recount list that does not fit, a device error) fail.
         bool ok = true;
         std::vector<SweepRequest*> fresh;
         for (Entry* e : admit) fresh.push_back(&e->req);
-        if (!fresh.empty()) ok = sw_.prepare(fresh) && sw_.upload(fresh);
+        std::vector<Entry*> rejected;
+        if (!fresh.empty() && !(sw_.prepare(fresh) && sw_.upload(fresh))) {
+            (void)hipGetLastError();
+            std::vector<Entry*> kept;
+            for (Entry* e : admit) {
+                const std::vector<SweepRequest*> one{&e->req};
+                if (sw_.prepare(one) && sw_.upload(one)) {
+                    kept.push_back(e);
+                } else {
+                    (void)hipGetLastError();
+                    (void)sw_.set_status(e->req, 0);
+                    rejected.push_back(e);
+                }
+            }
+            admit.swap(kept);
+        }
         for (Entry* e : admit) {
             // a pause that arrived while queued holds it right away
             if (ok && e->state == ReqState::kPaused) ok = sw_.set_status(e->req, 2);
@@ -1146,6 +1200,14 @@ void Session::worker() {
         for (Entry* e : cancelled) ok = ok && sw_.set_status(e->req, 0);
         RoundStats rs{};
         lk.lock();
+        for (Entry* e : rejected) {
+            e->state = ReqState::kFailed;
+            e->t_done = Clock::now();
+            sw_.unreserve(e->req);
+            in_use_ -= e->footprint;
+            admit_blocked_ = false;
+            admitted_.erase(std::find(admitted_.begin(), admitted_.end(), e->id));
+        }
         for (Entry* e : admit)
             if (e->state == ReqState::kQueued) e->state = ReqState::kRunning;
         for (auto& ed : edits)

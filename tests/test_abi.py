@@ -63,6 +63,25 @@ def test_invalid_arguments_return_minus_one(native):
     assert native.slime_scan_rect(1, 0, 0, 0, 10, 40, 221, buf, 4, None) == -1
 
 
+def test_size_limit_returns_minus_one(native):
+    """Regions over 2^48 centers (a square of radius above 8,388,607) are
+    refused before any device call, by every search entry point."""
+    import slime_amd
+    buf = (slime_amd.ExtChunkResult * 4)()
+    rd = 8_388_608
+    assert (2 * rd + 1) ** 2 > 1 << 48 > (2 * rd - 1) ** 2
+    assert native.search_slime_clusters_centered(1, rd, 60, 221, 0, 0, 0, buf, 4, 1, 0) == -1
+    assert native.search_slime_clusters(1, rd, 60, 221, 0, buf, 4, 1, 0) == -1
+    assert native.search_slime_clusters_gpu_v1_centered(1, rd, 60, 221, 0, 0, 0, buf, 4, 0) == -1
+    assert native.search_slime_clusters_gpu_v34(1, rd, 60, 221, 0, buf, 4, 0) == -1
+    assert native.slime_scan_rect(1, 0, 0, 1 << 24, (1 << 24) + 1, 60, 221, buf, 4, None) == -1
+    assert native.slime_scan_bands(1, 0, 0, (1 << 24) + 1, 1 << 24, 8, 0, 60, 221, buf, 4,
+                                   None) == -1
+    # a batch of only such searches: each total -1, no device work
+    assert slime_amd.search_batch([1, 2], [rd, rd + 5], [60, 60], max_results=4) == \
+        [(-1, []), (-1, [])]
+
+
 def test_biome_filter_argument_checks_need_no_device(native):
     """Invalid flags, heights and world lists are refused before any device
     call (MCGPU_ERR_ARG = -1)."""

@@ -52,11 +52,12 @@ def test_refinement_progress(gpu):
     assert L.get_progress() == 99
 
 
-def test_pause_and_cancel_a_running_search(gpu):
+def test_pause_and_cancel_a_running_search(gpu, oracle):
     """request_pause holds a running search between launches (the processed
     centers stop moving), request_cancel ends it early with the hits found so
-    far ranked, and reset_cancel lets the next search run in full
-    (SlimeCore.cpp:341-362, SlimeCoreGPU.cu:3085-3108, 3754-3757)."""
+    far ranked, each with its exact (Java) count, and reset_cancel lets the
+    next search run in full (SlimeCore.cpp:341-362, SlimeCoreGPU.cu:3085-3108,
+    3754-3757)."""
     import threading
     import time
     import slime_amd
@@ -96,6 +97,9 @@ def test_pause_and_cancel_a_running_search(gpu):
     assert total >= len(rows)
     assert rows == sorted(rows, key=lambda r: (-r[0], r[1], r[2]))
     assert all(thr <= r[0] <= 221 for r in rows)
+    lib = oracle.oracle_lib()
+    for r in rows:
+        assert lib.oracle_disk_count(0x5EED, (r[1] - 8) // 16, (r[2] - 8) // 16) == r[0]
     assert L.get_processed_centers() == 0
     total2, _ = slime_amd.search_slime_clusters_gpu_v1_centered(7, 3000, 45, 221, 0, 0, 0, 100)
     assert total2 >= 0 and L.get_processed_centers() == 6001 * 6001 and L.get_progress() == 90
