@@ -51,12 +51,70 @@ THRESHOLD = 60
 TOP_K = 65536
 REFERENCE_TPS = 1.409  # published: 14.06 T centers in 9.979 s (README.md:15,19)
 OPS_PER_CENTER = 6      # algorithmic INT32 ops per candidate center
+# MI355X's published dense FP32 vector rate, 157.3 TFLOP/s, is 78.6 T lane
+# operations per second (an FMA counts two flops): 256 CUs x 128 lanes x
+# 2.4 GHz. The integer scan is read against it; single-cycle 32-bit VOP2
+# operations (add, xor, and, shifts) issue at up to that rate, VOP3 forms
+# (multiplies, bcnt, alignbit, three-operand adds) at half of it.
+VALU_PEAK_T = 157.3 / 2
+# VALU lane instructions per center of the full-world search's banded kernels,
+# counted by rocprofv3 (SQ_INSTS_VALU x 64 / centers over all launches;
+# profiles/scan_band_kernel_full_world_thr60.md).
+VALU_INSTS_PER_CENTER = 5.27
 CPU_SAMPLE_RADIUS = 131072
 # HBM bytes per scan_band_kernel launch (rocprofv3 FETCH_SIZE + WRITE_SIZE of
 # the full-world run, scaled by the same counters on a copy of known size;
 # profiles/scan_band_kernel_full_world_thr60.md): 80.0 KiB x 2.0 + 0.02 KiB x 1.0
 # per launch, averaged over the search's 33 launches.
 TRAFFIC_BYTES = 163900
+
+
+def host_cpu():
+    """The host's CPU as the CPU baseline uses it: model, logical and physical
+    cores, SMT, and the threads this process may run on (its CPU affinity,
+    capped by a cgroup CPU quota where one is set: the box's CPU share)."""
+    info = {"model": None, "logical_cores": os.cpu_count(), "physical_cores": None, "smt": None}
+    try:
+        cores, model = set(), None
+        phys = core = None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+        info["model"] = model
+        info["physical_cores"] = len(cores) or None
+    except OSError:
+        pass
+    try:
+        with open("/sys/devices/system/cpu/smt/active") as f:
+            info["smt"] = f.read().strip() == "1"
+    except OSError:
+        if info["physical_cores"]:
+            info["smt"] = info["logical_cores"] > info["physical_cores"]
+    usable = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()
+            if q != "max":
+                quota = max(1, int(int(q) / int(period)))
+    except (OSError, ValueError):
+        pass
+    info["affinity_cpus"] = usable
+    info["cgroup_cpu_quota"] = quota
+    info["threads"] = min(usable, quota) if quota else usable
+    return info
 
 
 def hip_sync():
@@ -85,7 +143,8 @@ def cpu_baseline(gpu_lib):
     """Times the reference SlimeCore (or the scalar oracle port) on a sample."""
     from oracle import pyoracle as P
     import slime_amd
-    threads = max(1, min(16, os.cpu_count() or 1))
+    cpu = host_cpu()
+    threads = cpu["threads"]
     ref = P.ref_lib()
     rd = CPU_SAMPLE_RADIUS
     centers = (2 * rd + 1) ** 2
@@ -110,7 +169,7 @@ def cpu_baseline(gpu_lib):
         sample = f"radius {rd} around (0,0) (= {centers} centers), scalar C oracle, one run"
     gpu = slime_amd.search_slime_clusters_centered(FULL_WORLD_SEED, rd, THRESHOLD, 221, 0, 0, 0, TOP_K)
     return {"value": centers / dt / 1e12, "unit": "T centers/s", "cores": threads, "kind": kind,
-            "sample": sample, "seconds": dt, "gpu_matches_sample": gpu == result}
+            "sample": sample, "seconds": dt, "gpu_matches_sample": gpu == result, "host": cpu}
 
 
 C1 = (0x123456789ABCDEF0, 2048, 40)  # seed, radius, threshold of the CPU-runnable search
@@ -151,7 +210,7 @@ def c1_leg():
            "gpu_t_centers_per_s": centers / statistics.median(times) / 1e12}
     ref = P.ref_lib()
     if ref is not None:
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads = host_cpu()["threads"]
         want = P.ref_search(ref, seed, rd, thr, 221, 0, 0, 0, k, threads)  # warm-up, rows
         rbuf = (P.ChunkRow * k)()
         ct = []
@@ -324,9 +383,7 @@ def afk_y_leg(searches, res):
         cpu = afk_y_cpu(seeds, rows, got)
     finally:
         slime_amd.set_y_scan_config(0, -64, -64, 64, 4)
-    info = slime_amd.device_info()
-    cus, clk_khz = (info[0], info[1]) if info else (256, 2400000)
-    peak = cus * 64 * clk_khz * 1e3 / 1e12
+    peak = VALU_PEAK_T
     columns = n * 81 * refine_columns_per_point(*AFK_Y)
     achieved = columns * REFINE_VALU_PER_COLUMN / (ns / 1e9) / 1e12
     print(f"afk-y refine: {n} rows in {ns / 1e6:.2f} ms", file=sys.stderr, flush=True)
@@ -353,7 +410,7 @@ def afk_y_cpu(seeds, rows, got):
     ref = P.ref_lib()
     if ref is None:
         return None
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_cpu()["threads"]
     sample, same, t = 0, True, 0.0
     for w, (seed, rs) in enumerate(zip(seeds, rows)):
         if sample >= 2000:
@@ -381,9 +438,20 @@ def percentile(xs, q):
     return xs[i]
 
 
-def served_leg(n, rate, with_plain=True):
-    """The served configuration: one session vs one search at a time (through
-    the banded per-search engine, and through the plain per-chunk kernel)."""
+def latency_stats(lat, reqs, idx):
+    """p50/p99 of the latencies of idx, overall and for the small requests
+    (radius <= 4096, 70% of the mix)."""
+    small = [lat[i] for i in idx if reqs[i]["search"][1] <= 4096]
+    every = [lat[i] for i in idx]
+    return {"p50_s": percentile(every, 50), "p99_s": percentile(every, 99),
+            "small_p50_s": percentile(small, 50), "small_p99_s": percentile(small, 99)}
+
+
+def served_leg(n, rate, with_plain=False):
+    """The served configuration: one session vs one search at a time through
+    the same library's per-search engine (optionally also through the plain
+    per-chunk kernel). At the default rate requests arrive faster than the
+    device finishes them, so both forms are device-bound."""
     import threading
     import slime_amd
     from slime_amd.workloads import served_requests, SERVED_STATE
@@ -429,6 +497,7 @@ def served_leg(n, rate, with_plain=True):
         """One search at a time in arrival order (cancelled ones skipped, a
         paused one holds the device for the pause time)."""
         seq_lat, seq_res = {}, {}
+        slime_amd.kernel_stats(reset=True)
         t0 = time.perf_counter()
         for i, r in enumerate(reqs):
             now = time.perf_counter() - t0
@@ -441,12 +510,14 @@ def served_leg(n, rate, with_plain=True):
             seq_res[i] = run(r["search"])
             seq_lat[i] = time.perf_counter() - t0 - r["arrival"]
         seq_wall = time.perf_counter() - t0
+        _, kns, _ = slime_amd.kernel_stats()
         print(f"{name}: {len(seq_res)} requests in {seq_wall:.2f} s", file=sys.stderr, flush=True)
         seq_centers = sum((2 * reqs[i]["search"][1] + 1) ** 2 for i in seq_res)
         ident = all(results[i] == seq_res.get(i) for i in done) and set(done) == set(seq_res)
-        return {"wall_s": seq_wall, "t_centers_per_s": seq_centers / seq_wall / 1e12,
-                "p50_s": percentile(list(seq_lat.values()), 50),
-                "p99_s": percentile(list(seq_lat.values()), 99)}, ident
+        out = {"wall_s": seq_wall, "t_centers_per_s": seq_centers / seq_wall / 1e12,
+               "kernel_s": kns / 1e9}
+        out.update(latency_stats(seq_lat, reqs, list(seq_res)))
+        return out, ident
 
     seq, ident = sequential(
         lambda s: slime_amd.search_slime_clusters_gpu_v1_centered(*s, max_results=k), "sequential")
@@ -460,13 +531,14 @@ def served_leg(n, rate, with_plain=True):
     seq_plain, ident_plain = (None, True) if not with_plain else sequential(
         plain_search, "sequential plain kernel")
     served = {"wall_s": served_wall, "t_centers_per_s": centers / served_wall / 1e12,
-              "p50_s": percentile(list(lat.values()), 50),
-              "p99_s": percentile(list(lat.values()), 99),
               "kernel_s": stats["kernel_ns"] / 1e9, "rounds": stats["rounds"]}
+    served.update(latency_stats(lat, reqs, list(lat)))
     out = {
         "workload": "Poisson arrivals: 70% radius 256..4096, 25% radius 50000, 5% full world; "
                     "thr 50..65; 10% pause+resume, 5% cancel",
         "state": hex(SERVED_STATE), "requests": n, "rate_per_s": rate, "completed": len(done),
+        "centers": centers, "last_arrival_s": reqs[-1]["arrival"],
+        "device_bound": reqs[-1]["arrival"] < 0.5 * served_wall,
         "served": served,
         "sequential": seq,
         "speedup_vs_sequential": served["t_centers_per_s"] / seq["t_centers_per_s"],
@@ -495,11 +567,15 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
     ap.add_argument("--only-batched", action="store_true",
                     help="run only the batched leg (for profiling its kernels)")
+    ap.add_argument("--only-valu-rates", action="store_true",
+                    help="run only the VALU issue-rate microbenchmark (for profiling it)")
     ap.add_argument("--no-served", action="store_true", help="skip the served leg")
-    ap.add_argument("--no-served-plain", action="store_true",
-                    help="skip the served leg's one-at-a-time plain-kernel baseline")
-    ap.add_argument("--served-requests", type=int, default=80)
-    ap.add_argument("--served-rate", type=float, default=6.0, help="arrivals per second")
+    ap.add_argument("--served-plain", action="store_true",
+                    help="also serve the arrivals one at a time through the plain per-chunk "
+                         "kernel (a slow secondary baseline)")
+    ap.add_argument("--served-requests", type=int, default=120)
+    ap.add_argument("--served-rate", type=float, default=60.0,
+                    help="arrivals per second (the default outpaces the device)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -513,6 +589,11 @@ def main():
         return 1
     if args.only_batched:
         print(json.dumps({"batched": batched_leg(1)}), flush=True)
+        return 0
+    if args.only_valu_rates:
+        rates = {name: slime_amd.valu_rate(kind, 16384) / 1e12
+                 for kind, name in enumerate(slime_amd.VALU_KINDS)}
+        print(json.dumps({"valu_rates_t_per_s": rates, "peak": VALU_PEAK_T}), flush=True)
         return 0
     rd = args.radius
     centers = (2 * rd + 1) ** 2
@@ -548,7 +629,7 @@ def main():
 
     info = slime_amd.device_info()
     cus, clk_khz = (info[0], info[1]) if info else (256, 2400000)
-    peak = cus * 64 * clk_khz * 1e3 / 1e12  # INT32 VALU lane-ops/s in T
+    peak = VALU_PEAK_T
     kernels = {}
     for kind, name in ((slime_amd.KERNEL_BANDED, "scan_band_kernel"),
                        (slime_amd.KERNEL_PLAIN, "scan_plain_kernel")):
@@ -560,6 +641,12 @@ def main():
     dominant = max(kernels, key=lambda k: kernels[k]["kernel_s"])
     d = kernels[dominant]
     achieved = d["centers"] * OPS_PER_CENTER / d["kernel_s"] / 1e12
+    # the device's measured issue rate per instruction kind (T lane-instr/s)
+    issue = {}
+    for kind, name in enumerate(slime_amd.VALU_KINDS):
+        r = slime_amd.valu_rate(kind, 16384)
+        issue[name] = r / 1e12 if r else None
+    valu_rate = VALU_INSTS_PER_CENTER * d["centers"] / d["kernel_s"] / 1e12
     roofline = {
         "bound": "valu", "unit": "TOP/s", "achieved": achieved, "peak": peak,
         "frac": achieved / peak,
@@ -567,7 +654,12 @@ def main():
         "kernel": dominant,
         "ops_per_center": OPS_PER_CENTER, "avg_launch_ms": d["avg_launch_ms"],
         "centers_per_launch": d["centers_per_launch"],
-        "peak_basis": f"{cus} CUs x 64 lanes x {clk_khz / 1e6:.2f} GHz (INT32 VALU)",
+        "peak_basis": "published dense FP32 vector rate 157.3 TFLOP/s = 78.6 T lane-ops/s "
+                      f"({cus} CUs x 128 lanes x 2.4 GHz)",
+        "measured_issue_t_per_s": issue,
+        "valu_insts_per_center": VALU_INSTS_PER_CENTER,
+        "valu_insts_t_per_s": valu_rate,
+        "valu_frac": valu_rate / peak,
         "kernels": kernels,
     }
     out = {
@@ -588,8 +680,7 @@ def main():
     if not args.no_batched:
         out["batched"] = batched_leg(2)
     if not args.no_served:
-        out["served"] = served_leg(args.served_requests, args.served_rate,
-                                   not args.no_served_plain)
+        out["served"] = served_leg(args.served_requests, args.served_rate, args.served_plain)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -177,6 +177,18 @@ void slime_kernel_stats(int32_t kind, int64_t* launches, int64_t* kernel_ns, int
  * device is usable. */
 int32_t slime_device_info(int32_t* compute_units, int32_t* clock_khz, int64_t* hbm_bytes);
 
+/* Extension with no counterpart in the reference (measurement only): the
+ * device's issue rate of one kind of 32-bit integer VALU instruction, the
+ * measured ceiling the scan's roofline is read against. Runs iters x 64
+ * instructions of the kind in 8 independent chains per lane, 8 workgroups of
+ * 256 threads per compute unit; kind 0 v_add_u32, 1 v_xor_b32 (VOP2), 2
+ * v_mul_lo_u32, 3 v_bcnt_u32_b32, 4 v_mad_u32_u24, 5 v_alignbit_b32 (VOP3).
+ * Writes the lane instructions executed and the kernel's device time (HIP
+ * events); 0 on success, -1 on bad arguments (iters in 1..2^20) or a device
+ * error. */
+int32_t slime_valu_rate(int32_t kind, int32_t iters, int64_t* lane_instructions,
+                        int64_t* kernel_ns);
+
 /* Extension with no counterpart in the reference: device time (ns, HIP
  * events) of the last precise-AFK refinement kernel and its number of
  * (row, AFK point, height within 128 blocks of the platform) evaluations. */
@@ -216,16 +228,27 @@ int32_t slime_sweep_costs(double* ns_row, double* extra_rows, int32_t* nb, int32
 
 /* ---- served searches (extension; the reference serves one search at a time
  * behind process-global flags, SlimeCore.cpp:16-27) -------------------------
- * A session keeps every admitted search's state on the device (jobs, claim
- * cursors, histogram, hit region, status word) and scans all of them together
- * in rounds of persistent launches (continuous batching). Requests are
- * admitted in arrival order while their device footprint fits hbm_budget
- * bytes and fewer than max_active are admitted; round_ms bounds the device
- * time of one round (the latency of admission, pause and cancel). Each result
- * equals search_slime_clusters_gpu_v1_centered with the same arguments.
- * Request states: 0 queued, 1 running, 2 paused, 3 done, 4 cancelled
- * (partial result), 5 failed. */
+ * A session keeps every admitted search's state on the device (jobs, per-job
+ * done counts and claim cursors, histogram, hit region, status word) and scans
+ * all of them together in rounds (continuous batching): per kernel class one
+ * launch over the tiles of every active request, least work left first.
+ * Requests are admitted in arrival order while their device footprint fits
+ * hbm_budget bytes and fewer than max_active are admitted; round_ms bounds the
+ * device time of one round (the latency of admission, pause and cancel). Each
+ * result equals search_slime_clusters_gpu_v1_centered with the same
+ * arguments; a cancelled request keeps the exact rows of the tiles already
+ * scanned. Request states: 0 queued, 1 running, 2 paused, 3 done, 4 cancelled
+ * (partial result), 5 failed. A request over this build's size limit (2^48
+ * centers: radius 8,388,607), over the budget alone, or with more jobs or
+ * hits than the session holds is refused at submit (-1).
+ * slime_session_create runs direct rounds (one workgroup per tile, the
+ * per-search kernels' register budget; pause and cancel act between rounds);
+ * slime_session_create_ex with flags bit 0 runs the persistent work-queue
+ * kernels instead, which read each request's status word before every tile
+ * claim. NULL on bad arguments or no device. */
 void* slime_session_create(int64_t hbm_budget, int32_t max_active, int32_t round_ms);
+void* slime_session_create_ex(int64_t hbm_budget, int32_t max_active, int32_t round_ms,
+                              int32_t flags);
 void slime_session_destroy(void* session);
 /* Returns a request id (> 0), or -1 for invalid arguments. */
 int64_t slime_session_submit(void* session, int64_t seed, int64_t rd_max, int32_t min_size,

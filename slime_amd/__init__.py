@@ -35,6 +35,7 @@ EXPORTS = (
     "slime_session_cancel", "slime_session_poll", "slime_session_wait", "slime_session_result",
     "slime_session_release", "slime_session_stats", "score_candidates_precise",
     "refine_candidates_y", "slime_refine_stats", "refine_candidates_y_multi",
+    "slime_valu_rate", "slime_session_create_ex",
 )
 
 # Every symbol include/mcgpu.h declares (the biome filter).
@@ -122,6 +123,8 @@ def lib():
     L.slime_kernel_stats.restype = None
     L.slime_device_info.argtypes = [C.POINTER(i32), C.POINTER(i32), C.POINTER(i64)]
     L.slime_device_info.restype = i32
+    L.slime_valu_rate.argtypes = [i32, i32, C.POINTER(i64), C.POINTER(i64)]
+    L.slime_valu_rate.restype = i32
     L.slime_scan_bands.argtypes = [i64, i32, i32, i64, i64, i32, i32, i32, i32, P, i32,
                                    C.POINTER(C.c_int64)]
     L.slime_scan_bands.restype = i64
@@ -136,6 +139,8 @@ def lib():
     VP = C.c_void_p
     L.slime_session_create.argtypes = [i64, i32, i32]
     L.slime_session_create.restype = VP
+    L.slime_session_create_ex.argtypes = [i64, i32, i32, i32]
+    L.slime_session_create_ex.restype = VP
     L.slime_session_destroy.argtypes = [VP]
     L.slime_session_destroy.restype = None
     L.slime_session_submit.argtypes = [VP, i64, i64, i32, i32, i64, i32, i32, i32]
@@ -305,6 +310,19 @@ def device_info():
 KERNEL_PLAIN, KERNEL_BANDED, KERNEL_ALL = 0, 1, 2
 
 
+VALU_KINDS = ("v_add_u32", "v_xor_b32", "v_mul_lo_u32", "v_bcnt_u32_b32", "v_mad_u32_u24",
+              "v_alignbit_b32")
+
+
+def valu_rate(kind, iters=8192):
+    """Issue rate of one kind of integer VALU instruction on this device
+    (slime_valu_rate): lane instructions per second, or None on failure."""
+    n, ns = C.c_int64(), C.c_int64()
+    if lib().slime_valu_rate(kind, iters, C.byref(n), C.byref(ns)) != 0 or ns.value <= 0:
+        return None
+    return n.value / (ns.value * 1e-9)
+
+
 def kernel_stats(kind=KERNEL_ALL, reset=False):
     """(launches, kernel_ns, centers) of the scan kernels since the last reset."""
     n, ns, c = C.c_int64(), C.c_int64(), C.c_int64()
@@ -399,8 +417,9 @@ class Session:
     are admitted by device footprint, scanned together in rounds, and can be
     paused, resumed and cancelled while others run."""
 
-    def __init__(self, hbm_budget=4 << 30, max_active=64, round_ms=20):
-        self._h = lib().slime_session_create(hbm_budget, max_active, round_ms)
+    def __init__(self, hbm_budget=4 << 30, max_active=64, round_ms=20, persistent=False):
+        self._h = lib().slime_session_create_ex(hbm_budget, max_active, round_ms,
+                                                1 if persistent else 0)
         if not self._h:
             raise RuntimeError("slime_session_create failed")
 

@@ -434,9 +434,10 @@ int32_t slime_sweep_costs(double* ns_row, double* extra_rows, int32_t* nb, int32
     return n;
 }
 
-void* slime_session_create(int64_t hbm_budget, int32_t max_active, int32_t round_ms) {
-    if (hbm_budget <= 0 || max_active <= 0 || round_ms <= 0) return nullptr;
-    auto* s = new (std::nothrow) slime::Session(hbm_budget, max_active, round_ms);
+void* slime_session_create_ex(int64_t hbm_budget, int32_t max_active, int32_t round_ms,
+                              int32_t flags) {
+    if (hbm_budget <= 0 || max_active <= 0 || round_ms <= 0 || (flags & ~1)) return nullptr;
+    auto* s = new (std::nothrow) slime::Session(hbm_budget, max_active, round_ms, (flags & 1) != 0);
     if (!s) return nullptr;
     if (!s->ok()) {
         delete s;
@@ -444,6 +445,10 @@ void* slime_session_create(int64_t hbm_budget, int32_t max_active, int32_t round
         return nullptr;
     }
     return s;
+}
+
+void* slime_session_create(int64_t hbm_budget, int32_t max_active, int32_t round_ms) {
+    return slime_session_create_ex(hbm_budget, max_active, round_ms, 0);
 }
 
 void slime_session_destroy(void* session) { delete static_cast<slime::Session*>(session); }

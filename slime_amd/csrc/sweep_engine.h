@@ -180,8 +180,9 @@ namespace slime {
 // arrival order while their device footprint fits the HBM budget and a slot
 // is free; every round the worker applies pause/resume/cancel as writes of
 // the requests' device status words, admits what fits, runs one bounded
-// launch per kernel class over all active requests (least work left first)
-// and finishes the requests whose tiles are all done.
+// launch per kernel class over the tiles of all active requests (least work
+// left first; each job's done count stays on the device) and finishes the
+// requests whose tiles are all done.
 enum class ReqState : int32_t {
     kQueued = 0,
     kRunning = 1,
@@ -202,7 +203,12 @@ struct SessionStats {
 
 class Session {
    public:
-    Session(int64_t hbm_budget, int32_t max_active, int32_t round_ms);
+    // persistent: rounds of the persistent work-queue kernels, which read each
+    // request's status word before every claim (pause and cancel act within a
+    // launch); otherwise direct rounds, one workgroup per tile over a range of
+    // tiles per job, with the per-search kernels' register budget (pause and
+    // cancel act between rounds).
+    Session(int64_t hbm_budget, int32_t max_active, int32_t round_ms, bool persistent = false);
     ~Session();
     bool ok() const { return ok_; }
     int64_t submit(const SearchSpec& spec, int32_t keep);
@@ -223,6 +229,7 @@ class Session {
     int64_t budget_ = 0;
     int32_t max_active_ = 0;
     int64_t round_ns_ = 0;
+    bool persistent_ = false;
     std::mutex mu_;
     std::condition_variable cv_, done_cv_;
     std::thread thread_;

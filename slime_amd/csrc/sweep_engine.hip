@@ -951,7 +951,8 @@ struct Session::Entry {
     double progress = 0;
 };
 
-Session::Session(int64_t hbm_budget, int32_t max_active, int32_t round_ms) {
+Session::Session(int64_t hbm_budget, int32_t max_active, int32_t round_ms, bool persistent) {
+    persistent_ = persistent;
     budget_ = std::max<int64_t>(hbm_budget, int64_t(64) << 20);
     max_active_ = std::max(1, max_active);
     round_ns_ = (int64_t)std::max(1, round_ms) * 1000000;
@@ -1223,6 +1224,7 @@ void Session::worker() {
             RoundPolicy pol;
             pol.round_ns = round_ns_;
             pol.shortest_first = true;
+            pol.direct = !persistent_;
             ok = sw_.round(running, pol, &rs);
         }
         // finish completed and cancelled requests

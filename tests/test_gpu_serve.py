@@ -26,13 +26,16 @@ def searches(seed, n, radii):
     return out
 
 
-def test_served_mix_equals_alone(gpu):
+@pytest.mark.parametrize("persistent", [False, True])
+def test_served_mix_equals_alone(gpu, persistent):
+    """Direct rounds (the default) and the persistent work-queue kernels."""
     import slime_amd
     reqs = searches(3, 18, [40, 300, 3000, 20000, 50000, 100000, 150000, 5])
     # a last one large enough (2e12 centers) to be still running when paused
     reqs.append((0x5EEDF00D, 700000, 56, 221, 0, 123456, -654321))
     k = 300
-    with slime_amd.Session(hbm_budget=2 << 30, max_active=8, round_ms=5) as s:
+    with slime_amd.Session(hbm_budget=2 << 30, max_active=8, round_ms=5,
+                           persistent=persistent) as s:
         ids = [None] * len(reqs)
 
         def arrive():
@@ -89,11 +92,13 @@ def test_admission_by_footprint_queues_and_finishes(gpu):
         assert s.stats()["hbm_in_use"] == 0
 
 
-def test_pause_holds_progress_and_cancel_queued(gpu):
+@pytest.mark.parametrize("persistent", [False, True])
+def test_pause_holds_progress_and_cancel_queued(gpu, persistent):
     import slime_amd
     # about 0.2 s of device work, so that the pause lands while it runs
     big = (0x1234567, 400000, 55, 221, 0, 1000, -1000)
-    with slime_amd.Session(hbm_budget=1 << 30, max_active=1, round_ms=2) as s:
+    with slime_amd.Session(hbm_budget=1 << 30, max_active=1, round_ms=2,
+                           persistent=persistent) as s:
         rid = s.submit(*big, max_results=100)
         queued = s.submit(*big[:1], 100, 50, 221, 0, 0, 0, max_results=10)
         assert s.pause(rid)
@@ -176,7 +181,8 @@ def test_served_small_requests_equal_the_oracle(gpu, oracle):
         assert s.result(big_id, k) == alone(big, k)
 
 
-def test_served_cancel_returns_exact_partial_rows(gpu, oracle):
+@pytest.mark.parametrize("persistent", [False, True])
+def test_served_cancel_returns_exact_partial_rows(gpu, oracle, persistent):
     """A request cancelled while it runs returns what its finished tiles
     found: every row's count is the exact (Java) count of its disk, the rows
     are ranked, and the total is at most the full search's."""
@@ -184,7 +190,8 @@ def test_served_cancel_returns_exact_partial_rows(gpu, oracle):
     lib = oracle.oracle_lib()
     req = (0x5EEDCA7, 700_000, 52, 221, 0, 4321, -8765)
     k = 400
-    with slime_amd.Session(hbm_budget=2 << 30, max_active=4, round_ms=2) as s:
+    with slime_amd.Session(hbm_budget=2 << 30, max_active=4, round_ms=2,
+                           persistent=persistent) as s:
         rid = s.submit(*req, max_results=k)
         deadline = time.time() + 30
         while s.poll(rid)[1] == 0 and time.time() < deadline:
