@@ -112,6 +112,18 @@ void score_candidates_precise(int64_t seed, ExtChunkResult* results_buffer, int3
 void refine_candidates_y(int64_t seed, ExtChunkResult* results_buffer, int32_t count,
                          int32_t threads);
 
+/* Extension with no counterpart in the reference, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): refine_candidates_y without allocating
+ * anything. The rows are staged in a static device array of at most 65,536
+ * rows (1.5 MiB, held by each device the library runs on until the process
+ * ends; its address, a stream and two events are set up at the first call on
+ * each device). Rows and result as refine_candidates_y. Returns 0, -1 on a
+ * null buffer, negative count or device error (rows unchanged), -2 if count
+ * exceeds 65,536. Calls in one process are serialized by a lock; the array
+ * keeps the last call's rows. */
+int32_t refine_candidates_y_noalloc(int64_t seed, ExtChunkResult* results_buffer, int32_t count);
+
 /* Extension with no counterpart in the reference: refine_candidates_y over
  * the rows of many worlds in one launch. World w has seed world_seeds[w] and
  * rows [first[w], first[w+1]) of results_buffer (first[0] == 0); each row

@@ -35,7 +35,7 @@ EXPORTS = (
     "slime_session_cancel", "slime_session_poll", "slime_session_wait", "slime_session_result",
     "slime_session_release", "slime_session_stats", "score_candidates_precise",
     "refine_candidates_y", "slime_refine_stats", "refine_candidates_y_multi",
-    "slime_valu_rate", "slime_session_create_ex",
+    "slime_valu_rate", "slime_session_create_ex", "refine_candidates_y_noalloc",
 )
 
 # Every symbol include/mcgpu.h declares (the biome filter).
@@ -109,6 +109,8 @@ def lib():
     L.score_candidates_precise.restype = None
     L.refine_candidates_y.argtypes = [i64, P, i32, i32]
     L.refine_candidates_y.restype = None
+    L.refine_candidates_y_noalloc.argtypes = [i64, C.POINTER(ExtChunkResult), i32]
+    L.refine_candidates_y_noalloc.restype = i32
     L.refine_candidates_y_multi.argtypes = [i32, C.POINTER(i64), C.POINTER(i32), P]
     L.refine_candidates_y_multi.restype = i32
     L.slime_refine_stats.argtypes = [C.POINTER(i64), C.POINTER(i64)]

@@ -292,6 +292,19 @@ void refine_candidates_y(int64_t seed, ExtChunkResult* results_buffer, int32_t c
     control().progress.store(99, std::memory_order_relaxed);
 }
 
+int32_t refine_candidates_y_noalloc(int64_t seed, ExtChunkResult* results_buffer,
+                                    int32_t count) {
+    if (count < 0 || (count > 0 && !results_buffer)) return -1;
+    if (count > slime::kNoAllocMaxRows) return -2;
+    if (count == 0) return 0;
+    control().progress.store(90, std::memory_order_relaxed);
+    const int rc = slime::refine_rows_noalloc(refine_spec(seed, 1, slime::kRuleFirstOrder, 1),
+                                              reinterpret_cast<slime::ResultRow*>(results_buffer),
+                                              count);
+    control().progress.store(99, std::memory_order_relaxed);
+    return rc;
+}
+
 int32_t refine_candidates_y_multi(int32_t nworlds, const int64_t* world_seeds, const int32_t* first,
                                   ExtChunkResult* results_buffer) {
     if (nworlds < 0 || (nworlds > 0 && (!world_seeds || !first || !results_buffer)) ||

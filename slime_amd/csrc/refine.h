@@ -58,6 +58,13 @@ int refine_rows(const RefineSpec& spec, ResultRow* rows, int32_t count);
 int refine_rows_multi(const RefineSpec& spec, const int64_t* row_seeds, ResultRow* rows,
                       int32_t count);
 
+// The same as refine_rows without allocating anything: the rows are staged in
+// a static device array of kNoAllocMaxRows rows per device (set up at the
+// first call on each device). Returns 0, -1 on a device error (rows
+// unchanged) and -2 if count exceeds kNoAllocMaxRows.
+constexpr int32_t kNoAllocMaxRows = 65536;
+int refine_rows_noalloc(const RefineSpec& spec, ResultRow* rows, int32_t count);
+
 // Device time (ns, HIP events) of the last refinement kernel, and the number
 // of (row, point, level) evaluations it made.
 void refine_last_stats(int64_t* kernel_ns, int64_t* evaluations);

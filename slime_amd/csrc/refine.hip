@@ -228,10 +228,10 @@ int refine_rows(const RefineSpec& spec, ResultRow* rows, int32_t count) {
     return refine_rows_multi(spec, nullptr, rows, count);
 }
 
-int refine_rows_multi(const RefineSpec& spec, const int64_t* row_seeds, ResultRow* rows,
-                      int32_t count) {
-    if (count <= 0) return 0;
-    if (!rows) return -1;
+namespace {
+
+// The kernel's arguments for one refinement (rows and row_seeds left unset).
+RefineArgs refine_args(const RefineSpec& spec) {
     const YLevels& L = spec.levels;
     // Heights within 128 blocks of the platform; the others count 0.
     const int64_t lo = std::max<int64_t>(
@@ -251,6 +251,16 @@ int refine_rows_multi(const RefineSpec& spec, const int64_t* row_seeds, ResultRo
     a.group = a.noff == kOffsets ? kRefineThreads / kOffsets : kMaxGroup;
     a.rule = spec.rule;
     a.pack = spec.pack;
+    return a;
+}
+
+}  // namespace
+
+int refine_rows_multi(const RefineSpec& spec, const int64_t* row_seeds, ResultRow* rows,
+                      int32_t count) {
+    if (count <= 0) return 0;
+    if (!rows) return -1;
+    RefineArgs a = refine_args(spec);
 
     RefineState& s = state();
     std::lock_guard<std::mutex> lock(s.mu);
@@ -299,7 +309,7 @@ int refine_rows_multi(const RefineSpec& spec, const int64_t* row_seeds, ResultRo
     }
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, s.ev0, s.ev1) == hipSuccess) s.last_ns = (int64_t)(ms * 1e6);
-    s.last_evals = (int64_t)count * a.noff * lv_n;
+    s.last_evals = (int64_t)count * a.noff * a.lv_n;
     std::copy(out.begin(), out.end(), rows);
     return 0;
 }
@@ -309,6 +319,92 @@ void refine_last_stats(int64_t* kernel_ns, int64_t* evaluations) {
     std::lock_guard<std::mutex> lock(s.mu);
     if (kernel_ns) *kernel_ns = s.last_ns;
     if (evaluations) *evaluations = s.last_evals;
+}
+
+// ---- refinement without allocation ----------------------------------------
+//
+// The rows are staged in `scratch`, a file-scope device array of
+// kNoAllocMaxRows rows (1.5 MiB; every device the library runs on holds its
+// own copy until the process ends), and copied back through a static host
+// array of the same size. Host code reaches the device array only through
+// the address hipGetSymbolAddress gives for the current device, taken once per
+// device at its first call (with the device current) together with a stream
+// and two events. Nothing is allocated afterwards. Calls in one process are
+// serialized by a lock; the array keeps the last call's rows.
+
+namespace {
+
+static __device__ __align__(256) ResultRow scratch[kNoAllocMaxRows];
+
+constexpr int kNoAllocDevices = 64;
+
+struct NoAllocDevice {
+    bool ready = false;
+    ResultRow* rows = nullptr;  // this device's `scratch`
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+std::mutex g_noalloc_mu;
+NoAllocDevice g_noalloc[kNoAllocDevices];
+ResultRow g_noalloc_host[kNoAllocMaxRows];  // results before they replace the caller's rows
+
+// Set-up of the current device (once): the array's address, a stream, events.
+NoAllocDevice* noalloc_device() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocDevices) return nullptr;
+    NoAllocDevice& d = g_noalloc[dev];
+    if (d.ready) return &d;
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return nullptr;
+    if (!d.stream && hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking) != hipSuccess)
+        return nullptr;
+    if (!d.ev0 && hipEventCreate(&d.ev0) != hipSuccess) return nullptr;
+    if (!d.ev1 && hipEventCreate(&d.ev1) != hipSuccess) return nullptr;
+    d.rows = static_cast<ResultRow*>(ptr);
+    d.ready = true;
+    return &d;
+}
+
+}  // namespace
+
+int refine_rows_noalloc(const RefineSpec& spec, ResultRow* rows, int32_t count) {
+    if (count < 0 || count > kNoAllocMaxRows) return -2;
+    if (count == 0) return 0;
+    if (!rows) return -1;
+    RefineArgs a = refine_args(spec);
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    NoAllocDevice* d = noalloc_device();
+    if (!d) {
+        (void)hipGetLastError();
+        return -1;
+    }
+    a.rows = d->rows;
+    a.row_seeds = nullptr;
+    const size_t bytes = (size_t)count * sizeof(ResultRow);
+    if (hipMemcpyAsync(d->rows, rows, bytes, hipMemcpyHostToDevice, d->stream) != hipSuccess ||
+        hipEventRecord(d->ev0, d->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        return -1;
+    }
+    hipLaunchKernelGGL(refine_kernel, dim3((unsigned)count), dim3(kRefineThreads), 0, d->stream,
+                       a);
+    if (hipGetLastError() != hipSuccess || hipEventRecord(d->ev1, d->stream) != hipSuccess ||
+        hipMemcpyAsync(g_noalloc_host, d->rows, bytes, hipMemcpyDeviceToHost, d->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(d->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        return -1;
+    }
+    float ms = 0.f;
+    RefineState& s = state();
+    {
+        std::lock_guard<std::mutex> stats_lock(s.mu);
+        if (hipEventElapsedTime(&ms, d->ev0, d->ev1) == hipSuccess) s.last_ns = (int64_t)(ms * 1e6);
+        s.last_evals = (int64_t)count * a.noff * a.lv_n;
+    }
+    std::copy(g_noalloc_host, g_noalloc_host + count, rows);
+    return 0;
 }
 
 void release_refine() {
