@@ -193,8 +193,11 @@ int32_t slime_device_info(int32_t* compute_units, int32_t* clock_khz, int64_t* h
  * device's issue rate of one kind of 32-bit integer VALU instruction, the
  * measured ceiling the scan's roofline is read against. Runs iters x 64
  * instructions of the kind in 8 independent chains per lane, 8 workgroups of
- * 256 threads per compute unit; kind 0 v_add_u32, 1 v_xor_b32 (VOP2), 2
- * v_mul_lo_u32, 3 v_bcnt_u32_b32, 4 v_mad_u32_u24, 5 v_alignbit_b32 (VOP3).
+ * 256 threads per compute unit; kind 0 v_add_u32, 1 v_xor_b32 (VOP2, 4
+ * bytes), 2 v_mul_lo_u32, 3 v_bcnt_u32_b32, 4 v_mad_u32_u24, 5 v_alignbit_b32
+ * (VOP3, 8 bytes), 6 v_xor_b32 with a 32-bit literal (8 bytes), 7 v_add_u32 in
+ * its VOP3 encoding (8 bytes), 8 v_mul_u32_u24 (VOP2), 9 v_xor_b32 with an
+ * SGPR operand (VOP2).
  * Writes the lane instructions executed and the kernel's device time (HIP
  * events); 0 on success, -1 on bad arguments (iters in 1..2^20) or a device
  * error. */

@@ -313,7 +313,8 @@ KERNEL_PLAIN, KERNEL_BANDED, KERNEL_ALL = 0, 1, 2
 
 
 VALU_KINDS = ("v_add_u32", "v_xor_b32", "v_mul_lo_u32", "v_bcnt_u32_b32", "v_mad_u32_u24",
-              "v_alignbit_b32")
+              "v_alignbit_b32", "v_xor_b32 literal", "v_add_u32_e64", "v_mul_u32_u24",
+              "v_xor_b32 sgpr")
 
 
 def valu_rate(kind, iters=8192):

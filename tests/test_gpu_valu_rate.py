@@ -22,5 +22,5 @@ def test_valu_issue_rates(gpu):
         assert 1.0 < t <= PEAK_T * 1.02, (name, t)
     assert min(rates["v_add_u32"], rates["v_xor_b32"]) > max(rates["v_mul_lo_u32"],
                                                                rates["v_mad_u32_u24"])
-    assert slime_amd.lib().slime_valu_rate(6, 16, None, None) == -1
+    assert slime_amd.lib().slime_valu_rate(10, 16, None, None) == -1
     assert slime_amd.lib().slime_valu_rate(0, 0, None, None) == -1
