@@ -280,6 +280,11 @@ def batched_leg(repeats):
 
 FP64_PEAK_TF = 78.6          # MI355X dense FP64 vector peak
 FLOPS_PER_COLUMN = 52 * 60   # 52 double Perlin octave samples of ~60 FP64 ops
+# FP64 add and multiply wave instructions of one filter launch over these
+# sites, counted by rocprofv3 (SQ_INSTS_VALU_FLOPS_FP64, no FMA: the noise is
+# compiled without contraction; profiles/biome_filter_batched_top_sites.md),
+# times 64 lanes
+FP64_FLOPS_COUNTED = 4.285e8 * 64
 EXTENDED_LEVELS = (-64, 320, 4)  # block heights of the wider biome sweep
 
 
@@ -308,7 +313,7 @@ def biome_leg(searches, res):
             per_ms += ctx.last_filter_ms()
     per_wall = time.perf_counter() - t0
     columns = 16 * sum(sizes)  # quart columns under the sites' slime chunks (before early exits)
-    achieved = columns * FLOPS_PER_COLUMN / (ms / 1e3) / 1e12
+    achieved = FP64_FLOPS_COUNTED / (ms / 1e3) / 1e12
     print(f"biome filter: {n} sites in {ms:.1f} ms", file=sys.stderr, flush=True)
     # the wider sweep: both biomes at every height -64..320 step 4, over the
     # disk's slime chunks and over every chunk of the disk
@@ -332,8 +337,11 @@ def biome_leg(searches, res):
         "extended": extended,
         "roofline": {"bound": "fp64", "unit": "TFLOP/s", "peak": FP64_PEAK_TF,
                      "achieved": achieved, "frac": achieved / FP64_PEAK_TF,
-                     "flops_per_column": FLOPS_PER_COLUMN, "columns": columns,
-                     "note": "the parameter-tree search (integer, divergent) is most of the time"},
+                     "flops_counted": FP64_FLOPS_COUNTED,
+                     "flops_modelled": columns * FLOPS_PER_COLUMN, "columns": columns,
+                     "note": "FP64 is 17% of the VALU instructions; the parameter-tree "
+                             "searches (integer, latency-bound LDS reads) are most of the "
+                             "time"},
     }
 
 
@@ -567,6 +575,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
     ap.add_argument("--only-batched", action="store_true",
                     help="run only the batched leg (for profiling its kernels)")
+    ap.add_argument("--only-biome", action="store_true",
+                    help="run only the batched searches once and the biome filter of their top "
+                         "sites (for profiling the filter)")
     ap.add_argument("--only-valu-rates", action="store_true",
                     help="run only the VALU issue-rate microbenchmark (for profiling it)")
     ap.add_argument("--no-served", action="store_true", help="skip the served leg")
@@ -589,6 +600,12 @@ def main():
         return 1
     if args.only_batched:
         print(json.dumps({"batched": batched_leg(1)}), flush=True)
+        return 0
+    if args.only_biome:
+        from slime_amd.workloads import batched_searches
+        searches = batched_searches(256)
+        res = slime_amd.search_batch(*zip(*searches), max_results=1000)
+        print(json.dumps({"biome": biome_leg(searches, res)}), flush=True)
         return 0
     if args.only_valu_rates:
         rates = {name: slime_amd.valu_rate(kind, 16384) / 1e12

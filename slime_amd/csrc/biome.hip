@@ -255,30 +255,43 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(const Climate* climate
 }
 
 // Candidates of many worlds: world w's candidates are [first[w], first[w+1]),
-// its climate climates[w]. A workgroup takes a contiguous run of candidates,
-// so it reloads its LDS climate only where the run crosses into a new world.
+// its climate climates[w]. The candidates form a device work queue: a
+// workgroup takes the next one with one atomic add on *next (zero at launch)
+// until the list is exhausted, so that sites whose checks end early (a Deep
+// Dark quart found in the first layers) do not leave their CUs idle while
+// others still run; it reloads its LDS climate only when the world changes.
+// Every workgroup leaves the loop once the counter passes count.
 template <int F, bool L>
 __global__ __launch_bounds__(kThreads) void filter_multi_kernel(
     const Climate* climates, const int64_t* seeds, const int32_t* first, int32_t nworlds,
     TreeArgs tree, TargetLeaves tl, const int32_t* ccx, const int32_t* ccz, int32_t count,
-    int32_t per_block, int32_t flags, Levels lv, int32_t* mask_out) {
+    unsigned int* next, int32_t flags, Levels lv, int32_t* mask_out) {
     __shared__ Climate st;
     __shared__ FilterLds f;
+    __shared__ int s_cand;
     load_params(f.params, tree);
-    const int begin = blockIdx.x * per_block;
-    const int end = min(count, begin + per_block);
     int world = -1;
     int hint = -1;
-    for (int cand = begin; cand < end; ++cand) {
-        int w = world < 0 ? 0 : world;
-        while (w + 1 < nworlds && first[w + 1] <= cand) ++w;
+    for (;;) {
         __syncthreads();  // the previous candidate's LDS is no longer read
-        if (w != world) {
-            load_climate(st, climates + w);
-            world = w;
+        if (threadIdx.x == 0) s_cand = (int)atomicAdd(next, 1u);
+        __syncthreads();
+        const int cand = s_cand;
+        if (cand >= count) break;
+        // the candidate's world: the last w with first[w] <= cand
+        int lo = 0, hi = nworlds - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (first[mid] <= cand) lo = mid;
+            else hi = mid - 1;
         }
-        const unsigned int m = filter_candidate<F, L>(st, f, tree.nodes, tree.len, tl, seeds[w],
-                                                   ccx[cand], ccz[cand], flags, lv, hint);
+        if (lo != world) {
+            load_climate(st, climates + lo);
+            world = lo;
+        }
+        const unsigned int m = filter_candidate<F, L>(st, f, tree.nodes, tree.len, tl,
+                                                   seeds[lo], ccx[cand], ccz[cand], flags, lv,
+                                                   hint);
         if (threadIdx.x == 0) mask_out[cand] = (int32_t)m;
     }
 }
@@ -674,17 +687,17 @@ int filter_multi(int32_t nworlds, int tree, const uint64_t* seeds, const int32_t
     hipStream_t stream = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     const size_t ni = sizeof(int32_t) * (size_t)count;
-    // about 4 workgroups per CU's worth of runs, each a contiguous range
-    const int32_t blocks_wanted = std::max(1, cus * 4);
-    const int32_t per_block = std::max<int32_t>(1, (count + blocks_wanted - 1) / blocks_wanted);
-    const int32_t grid = (count + per_block - 1) / per_block;
+    // the workgroups resident at once (4 per CU at the kernels' register use),
+    // at most one per candidate; d_out's last word is the queue counter
+    const int32_t grid = std::max(1, std::min(count, cus * 4));
     bool ok = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) == hipSuccess &&
               hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
               hipMalloc(&d_clim, sizeof(Climate) * (size_t)nworlds) == hipSuccess &&
               hipMalloc(&d_seeds, sizeof(int64_t) * (size_t)nworlds) == hipSuccess &&
               hipMalloc(&d_first, sizeof(int32_t) * (size_t)(nworlds + 1)) == hipSuccess &&
               hipMalloc(&d_x, ni) == hipSuccess && hipMalloc(&d_z, ni) == hipSuccess &&
-              hipMalloc(&d_out, ni) == hipSuccess &&
+              hipMalloc(&d_out, ni + sizeof(int32_t)) == hipSuccess &&
+              hipMemsetAsync(d_out + count, 0, sizeof(int32_t), stream) == hipSuccess &&
               hipMemcpyAsync(d_clim, host.data(), sizeof(Climate) * (size_t)nworlds,
                              hipMemcpyHostToDevice, stream) == hipSuccess &&
               hipMemcpyAsync(d_seeds, sseed.data(), sizeof(int64_t) * (size_t)nworlds,
@@ -700,8 +713,8 @@ int filter_multi(int32_t nworlds, int tree, const uint64_t* seeds, const int32_t
                            ? (f0 ? filter_multi_kernel<0, false> : filter_multi_kernel<1, false>)
                            : (f0 ? filter_multi_kernel<0, true> : filter_multi_kernel<1, true>);
         hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), 0, stream, d_clim, d_seeds,
-                           d_first, nworlds, targs, targets, d_x, d_z, count, per_block, flags, lv,
-                           d_out);
+                           d_first, nworlds, targs, targets, d_x, d_z, count,
+                           reinterpret_cast<unsigned int*>(d_out + count), flags, lv, d_out);
         ok = hipGetLastError() == hipSuccess && hipEventRecord(e1, stream) == hipSuccess &&
              hipMemcpyAsync(out, d_out, ni, hipMemcpyDeviceToHost, stream) == hipSuccess &&
              hipStreamSynchronize(stream) == hipSuccess &&
