@@ -560,6 +560,41 @@ def served_leg(n, rate, with_plain=False):
     return out
 
 
+def window_ab_leg(blocks=3072, steps=4096, min_size=60, seed=0x5EED):
+    """Counting stage alone (slime_window_ab): VALU word counting (method 0)
+    against int8 MFMA window sums (method 1) over the same hashed bitplane
+    ring, for 8 bands (the full-world launches) and 4 (the largest searches
+    of the batched set). Rate: window centers counted per second, 4 rows x
+    496 centers x 2 phases x bands per strip and workgroup; best of two
+    launches; both methods' checksums must agree."""
+    import slime_amd
+    out = {}
+    for bands in (8, 4):
+        row = {}
+        for method, name in ((slime_amd.WINDOW_VALU, "valu"), (slime_amd.WINDOW_MFMA, "mfma")):
+            best, res = None, None
+            for _ in range(2):
+                r = slime_amd.window_ab(method, bands, blocks, steps, min_size, seed)
+                if r is None:
+                    raise SystemExit("window_ab failed")
+                if res is not None and r[:2] != res[:2]:
+                    raise SystemExit("window_ab: launches disagree")
+                res = r
+                best = r[2] if best is None else min(best, r[2])
+            centers = blocks * steps * 4 * 496 * 2 * bands
+            row[name] = {"kernel_ms": best / 1e6, "t_centers_per_s": centers / (best * 1e-9) / 1e12,
+                         "checksum": res[0], "passes": res[1]}
+            print(f"window_ab bands {bands} {name}: {best / 1e6:.3f} ms", file=sys.stderr,
+                  flush=True)
+        if row["valu"]["checksum"] != row["mfma"]["checksum"] or \
+                row["valu"]["passes"] != row["mfma"]["passes"]:
+            raise SystemExit("window_ab: methods disagree")
+        row["mfma_over_valu_time"] = row["mfma"]["kernel_ms"] / row["valu"]["kernel_ms"]
+        out[f"bands_{bands}"] = row
+    out["workload"] = {"blocks": blocks, "steps": steps, "min_size": min_size, "seed": seed}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -580,6 +615,9 @@ def main():
                          "sites (for profiling the filter)")
     ap.add_argument("--only-valu-rates", action="store_true",
                     help="run only the VALU issue-rate microbenchmark (for profiling it)")
+    ap.add_argument("--only-window-ab", action="store_true",
+                    help="run only the counting-stage A/B (VALU word counting vs int8 MFMA "
+                         "window sums) at 8 and 4 bands")
     ap.add_argument("--no-served", action="store_true", help="skip the served leg")
     ap.add_argument("--served-plain", action="store_true",
                     help="also serve the arrivals one at a time through the plain per-chunk "
@@ -611,6 +649,9 @@ def main():
         rates = {name: slime_amd.valu_rate(kind, 16384) / 1e12
                  for kind, name in enumerate(slime_amd.VALU_KINDS)}
         print(json.dumps({"valu_rates_t_per_s": rates, "peak": VALU_PEAK_T}), flush=True)
+        return 0
+    if args.only_window_ab:
+        print(json.dumps({"window_ab": window_ab_leg()}), flush=True)
         return 0
     rd = args.radius
     centers = (2 * rd + 1) ** 2

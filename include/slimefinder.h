@@ -204,6 +204,22 @@ int32_t slime_device_info(int32_t* compute_units, int32_t* clock_khz, int64_t* h
 int32_t slime_valu_rate(int32_t kind, int32_t iters, int64_t* lane_instructions,
                         int64_t* kernel_ns);
 
+/* Extension with no counterpart in the reference (whose binary tensor-core
+ * popcounts are cuda_x2_z15/tensor_cold.py:4-39); measurement only. A/B of
+ * the banded scan's counting stage over a hashed bitplane ring in LDS laid
+ * out as the scan's: the rolling 20x20 window of every 4x4 group less its 32
+ * corner cells, per band-phase, for `steps` strips of 4 rows. method 0 counts
+ * on the VALU by plane words as the scan kernels do; method 1 forms the
+ * 5-column window sums with int8 matrix instructions (v_mfma_i32_16x16x64_i8).
+ * bands 2, 4 or 8 (two x-phases each), blocks 1..65536 workgroups, steps
+ * 5..2^20, min_size 1..400. Writes the wrapping sum of a weight of (strip,
+ * band-phase, group) over the windows reaching min_size, their number, and
+ * the timed launch's device time (HIP events); 0 on success, -1 on bad
+ * arguments or a device error. */
+int32_t slime_window_ab(int32_t method, int32_t bands, int32_t blocks, int32_t steps,
+                        int32_t min_size, uint32_t seed, uint32_t* checksum, uint64_t* passes,
+                        int64_t* kernel_ns);
+
 /* Extension with no counterpart in the reference: device time (ns, HIP
  * events) of the last precise-AFK refinement kernel and its number of
  * (row, AFK point, height within 128 blocks of the platform) evaluations. */

@@ -36,6 +36,7 @@ EXPORTS = (
     "slime_session_release", "slime_session_stats", "score_candidates_precise",
     "refine_candidates_y", "slime_refine_stats", "refine_candidates_y_multi",
     "slime_valu_rate", "slime_session_create_ex", "refine_candidates_y_noalloc",
+    "slime_window_ab",
 )
 
 # Every symbol include/mcgpu.h declares (the biome filter).
@@ -127,6 +128,9 @@ def lib():
     L.slime_device_info.restype = i32
     L.slime_valu_rate.argtypes = [i32, i32, C.POINTER(i64), C.POINTER(i64)]
     L.slime_valu_rate.restype = i32
+    L.slime_window_ab.argtypes = [i32, i32, i32, i32, i32, C.c_uint32, C.POINTER(C.c_uint32),
+                                  C.POINTER(C.c_uint64), C.POINTER(i64)]
+    L.slime_window_ab.restype = i32
     L.slime_scan_bands.argtypes = [i64, i32, i32, i64, i64, i32, i32, i32, i32, P, i32,
                                    C.POINTER(C.c_int64)]
     L.slime_scan_bands.restype = i64
@@ -324,6 +328,19 @@ def valu_rate(kind, iters=8192):
     if lib().slime_valu_rate(kind, iters, C.byref(n), C.byref(ns)) != 0 or ns.value <= 0:
         return None
     return n.value / (ns.value * 1e-9)
+
+
+WINDOW_VALU, WINDOW_MFMA = 0, 1
+
+
+def window_ab(method, bands, blocks, steps, min_size, seed=0):
+    """Counting-stage A/B (slime_window_ab): (checksum, passes, kernel_ns),
+    or None on failure."""
+    cs, n, ns = C.c_uint32(), C.c_uint64(), C.c_int64()
+    if lib().slime_window_ab(method, bands, blocks, steps, min_size, seed & 0xFFFFFFFF,
+                             C.byref(cs), C.byref(n), C.byref(ns)) != 0:
+        return None
+    return cs.value, n.value, ns.value
 
 
 def kernel_stats(kind=KERNEL_ALL, reset=False):

@@ -41,12 +41,25 @@
 // wave a few instructions instead of a long divergent run of one lane.
 // Java's rejected first draws are not handled here; the engine recounts the
 // few affected centers.
+//
+// With the union bound, two x-phases and 2-8 bands (the full-world and the
+// largest batched launches), the 5-column window sums less the corner cells
+// are int8 contractions on the matrix cores instead: the 20-row counts and
+// corner counts of 16 band-phase rows x 32 nibble columns per wave are the A
+// operand of v_mfma_i32_16x16x64_i8, a constant 0/+1/-1 window matrix B,
+// the threshold bias C; two instructions per wave and step give 16 rows x 32
+// groups, the second one with the counts of the next two plane words through
+// LDS. Build with SLIME_BAND_MFMA=0 to count those windows on the VALU.
 #pragma once
 
 #include <type_traits>
 
 #include "scan.h"
 #include "slime_math.h"
+
+#ifndef SLIME_BAND_MFMA
+#define SLIME_BAND_MFMA 1
+#endif
 
 namespace slime {
 namespace band {
@@ -176,6 +189,48 @@ static_assert(corner_mask(6) == 0 && corner_mask(13) == 0 && corner_mask(5) != 0
 // (gbit << 8) | (b << 1) | ph, gbit = 4 * group.
 __device__ __forceinline__ uint32_t group_code(int gbit, int b, int ph) {
     return ((uint32_t)gbit << 8) | ((uint32_t)b << 1) | (uint32_t)ph;
+}
+
+typedef int mfma_i8x16 __attribute__((ext_vector_type(4)));  // 16 int8 / 4 int32
+
+// Matrix-core window sums with two x-phases and NB (2..8) bands: QN = 2 NB
+// band-phases and H column ranges make the 16 rows of A (H QN <= 16 rows
+// used). Lane l of counting wave w holds row m = l % 16 (band-phase m % QN,
+// range m / QN) over plane word (m / QN) * kRangeWords + 4 w + l / 16, i.e.
+// a wave covers 32 nibble columns of each range; the waves past kWaves do
+// not count. D (lane l, register i): row 4 (l / 16) + i, group l % 16.
+template <int NB>
+struct MfmaRoles {
+    static constexpr int QN = 2 * NB;
+    static constexpr int H = QN > 8 ? 1 : QN > 4 ? 2 : 4;
+    static constexpr int kRows = QN * H;
+    static constexpr int kRangeWords = 16 / H;
+    static constexpr int kWaves = 4 / H;
+    static_assert(NB >= 2 && NB <= 8 && kRows <= 16, "2..8 bands");
+};
+
+// B of the window contraction for lane l: column n = l % 16 (the n-th group
+// of the operand's first plane word), k-block l / 16 (its (l / 16)-th plane
+// word): bytes 0-3 the word's even nibble columns' counts, 4-7 the odd ones,
+// 8-11 / 12-15 their corner counts. Group n's window: nibble columns n..n+4,
+// less the corner counts of columns n and n+4.
+__device__ __forceinline__ mfma_i8x16 window_matrix(int lane) {
+    const int n = lane & 15, j = lane >> 4;
+    int dw[4];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        uint32_t word = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int c = 8 * j + 2 * e + (d & 1);
+            int wgt;
+            if (d < 2) wgt = (c >= n && c <= n + 4) ? 1 : 0;
+            else wgt = (c == n || c == n + 4) ? -1 : 0;
+            word |= (uint32_t)(uint8_t)(int8_t)wgt << (8 * e);
+        }
+        dw[d] = (int)word;
+    }
+    return mfma_i8x16{dw[0], dw[1], dw[2], dw[3]};
 }
 
 // Row r (0..19) of the 20-bit window of a coded group: slime chunks of its
@@ -368,9 +423,17 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
     constexpr bool kCorner = UB && kWord;
     constexpr bool kWSplit = PH == 2 && NB <= 8;
     constexpr int kWThreads = 16 * NB * (kWSplit ? 2 : 1);  // threads that hold a band
-    const int wd = tid & 15;
-    const int wb = kWSplit ? tid >> 5 : tid >> 4;
-    const int wph = kWSplit ? (tid >> 4) & 1 : 0;
+    // Matrix-core window sums (MfmaRoles): the lane's plane word, band and
+    // phase follow its row of A instead.
+    constexpr bool kMfma = SLIME_BAND_MFMA && kCorner && PH == 2 && NB >= 2 && NB <= 8;
+    using MR = MfmaRoles<kMfma ? NB : 2>;
+    const int mrow = lane & 15;
+    const bool m_ok = kMfma && wave < MR::kWaves && mrow < MR::kRows;
+    const int wd = !kMfma ? (tid & 15)
+                 : m_ok   ? (mrow / MR::QN) * MR::kRangeWords + 4 * wave + (lane >> 4)
+                          : 0;
+    const int wb = !kMfma ? (kWSplit ? tid >> 5 : tid >> 4) : m_ok ? (mrow % MR::QN) >> 1 : NB;
+    const int wph = !kMfma ? (kWSplit ? (tid >> 4) & 1 : 0) : (mrow % MR::QN) & 1;
     const bool w_active = wb < NB;
     const int wbc = w_active ? wb : 0;  // idle threads read a valid plane
     // phase 0 of band b lies where e == (b & 1): E for odd b, ~E for even b;
@@ -392,7 +455,7 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
 #pragma unroll
         for (int j = 0; j < 5; ++j) he[ph][j] = ho[ph][j] = 0;
         wvalid[ph] = 0;
-        if constexpr (kWord) {
+        if constexpr (kWord && !kMfma) {
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
                 const int gi = 8 * wd + i;
@@ -406,6 +469,37 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
     // 0x8000 - min_size in both 16-bit lanes: bit 15 / 31 of square + this is
     // set when the lane's square reaches min_size
     const uint32_t wbias = (uint32_t)(0x8000 - p.min_size) * 0x10001u;
+    // kMfma: bit 4 f + i of `mvalid` says that D register i of the f-th
+    // contraction (row 4 (lane / 16) + i, group 16 f + lane % 16 of the row's
+    // first plane word) has a center inside the job; rows i = 0, 1 are band
+    // mband[0], rows 2, 3 band mband[1]. C is 0x8000 - min_size, so that bit
+    // 15 of a sum says it reaches min_size. Words 16 and 17 of the exchange
+    // (past the row) stay zero.
+    __shared__ mfma_i8x16 mxch[kMfma ? (kPlaneDw + 2) * 16 : 1];
+    uint32_t mvalid = 0;
+    int mband[2] = {NB, NB};
+    mfma_i8x16 mB = {0, 0, 0, 0}, mC = {0, 0, 0, 0};
+    if constexpr (kMfma) {
+        if (tid < 32) mxch[kPlaneDw * 16 + tid] = mfma_i8x16{0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int md = 4 * (lane >> 4) + i;
+            if (wave >= MR::kWaves || md >= MR::kRows) continue;
+            const int qd = md % MR::QN, hd = md / MR::QN;
+            if ((i & 1) == 0) mband[i >> 1] = qd >> 1;
+#pragma unroll
+            for (int f = 0; f < 2; ++f) {
+                const int g = 8 * (hd * MR::kRangeWords + 4 * wave) + 16 * f + (lane & 15);
+                const int64_t ox0 = block_off + (int64_t)(qd & 1) * kPeriod + col0 + 4 * g;
+                if (g < kGroupCols && col0 + 4 * g < p.phase_cols && ox0 < p.w &&
+                    ox0 + 3 >= p.ox_min)
+                    mvalid |= 1u << (4 * f + i);
+            }
+        }
+        mB = window_matrix(lane);
+        const int bias = 0x8000 - p.min_size;
+        mC = mfma_i8x16{bias, bias, bias, bias};
+    }
 
     const int32_t tile_s0 = p.s_begin + ts * p.tile_h;
     const int nbatch = p.tile_h / 4 + 4;
@@ -459,7 +553,65 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
         const int m = k - 4;
         const int32_t s_g = tile_s0 + 4 * m;  // first slot row of the groups
         uint32_t pass = 0;  // owned band-phases whose square reaches min_size
-        if constexpr (kWord) {
+        if constexpr (kMfma) {
+            // the vertical part as below (one phase a thread), then the lane's
+            // counts and corner counts as its 16 bytes of A
+            mfma_i8x16 a0 = {0, 0, 0, 0};
+            if (wave < MR::kWaves) {
+                uint32_t nhead = 0, ntail = 0;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const uint32_t* rowp = ring + ((4 * k + r) % kRing) * R::kRowDw;
+                    const uint32_t c =
+                        nibble_counts(rowp[(wbc + 1) * kPlaneDw + wd] & (rowp[wd] ^ wflip));
+                    if (r < 2) nhead += c;
+                    else ntail += c;
+                }
+                const uint32_t se = (nhead & 0x0F0F0F0Fu) + (ntail & 0x0F0F0F0Fu);
+                const uint32_t so = ((nhead >> 4) & 0x0F0F0F0Fu) + ((ntail >> 4) & 0x0F0F0F0Fu);
+                we[0] += se - he[0][0];
+                wo[0] += so - ho[0][0];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    he[0][j] = he[0][j + 1];
+                    ho[0][j] = ho[0][j + 1];
+                }
+                he[0][4] = se;
+                ho[0][4] = so;
+                const uint32_t old = hq[0][0];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) hq[0][j] = hq[0][j + 1];
+                hq[0][3] = nhead;
+                const uint32_t pe = (ntail & 0x0F0F0F0Fu) + (old & 0x0F0F0F0Fu);
+                const uint32_t po = ((ntail >> 4) & 0x0F0F0F0Fu) + ((old >> 4) & 0x0F0F0F0Fu);
+                if (m_ok) {
+                    a0 = mfma_i8x16{(int)we[0], (int)wo[0], (int)pe, (int)po};
+                    mxch[wd * 16 + mrow % MR::QN] = a0;
+                }
+            }
+            __syncthreads();
+            if (wave < MR::kWaves) {
+                // the second contraction starts two plane words on; the
+                // exchange is indexed by band-phase, since with several column
+                // ranges the rows of another range hold the word
+                mfma_i8x16 a1 = {0, 0, 0, 0};
+                if (m_ok) a1 = mxch[(wd + 2) * 16 + mrow % MR::QN];
+                const mfma_i8x16 d0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, mB, mC, 0, 0, 0);
+                const mfma_i8x16 d1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, mB, mC, 0, 0, 0);
+                const uint32_t lo =
+                    (((uint32_t)d0[0] & 0x8000u) | (((uint32_t)d0[1] & 0x8000u) << 1) |
+                     (((uint32_t)d0[2] & 0x8000u) << 2) | (((uint32_t)d0[3] & 0x8000u) << 3)) >> 15;
+                const uint32_t hi =
+                    (((uint32_t)d1[0] & 0x8000u) | (((uint32_t)d1[1] & 0x8000u) << 1) |
+                     (((uint32_t)d1[2] & 0x8000u) << 2) | (((uint32_t)d1[3] & 0x8000u) << 3)) >> 11;
+                pass = (lo | hi) & mvalid;
+                if (m < 0 || s_g >= p.s_end) pass = 0;
+                if (pass) {
+                    if ((int64_t)mband[0] * kPeriod + s_g >= p.h_valid) pass &= ~0x33u;
+                    if ((int64_t)mband[1] * kPeriod + s_g >= p.h_valid) pass &= ~0xCCu;
+                }
+            }
+        } else if constexpr (kWord) {
             // every thread of a wave that holds a band takes part (the next-lane
             // reads need whole rows); the waves past the last band skip it
             if (wave * 64 < kWThreads) {
@@ -602,7 +754,13 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
         }
         // ---- refinement by the whole wave ----------------------------------
         auto code_of = [&](int q) {
-            if constexpr (kWord)  // bit 8 ph + i: group 8 wd + i, band wb, phase wph + ph
+            if constexpr (kMfma) {  // bit 4 f + i: D register i of contraction f
+                const int md = 4 * (lane >> 4) + (q & 3);
+                const int qd = md % MR::QN, hd = md / MR::QN;
+                const int g =
+                    8 * (hd * MR::kRangeWords + 4 * wave) + 16 * (q >> 2) + (lane & 15);
+                return group_code(4 * g, qd >> 1, qd & 1);
+            } else if constexpr (kWord)  // bit 8 ph + i: group 8 wd + i, band wb, phase wph + ph
                 return group_code(4 * (8 * wd + (q & 7)), wb, wph + (q >> 3));
             else
                 return group_code(gbit, b_lo + q / C::kPhases, ph_lo + q % C::kPhases);
