@@ -42,14 +42,15 @@
 // Java's rejected first draws are not handled here; the engine recounts the
 // few affected centers.
 //
-// With the union bound, two x-phases and 2-8 bands (the full-world and the
-// largest batched launches), the 5-column window sums less the corner cells
-// are int8 contractions on the matrix cores instead: the 20-row counts and
-// corner counts of 16 band-phase rows x 32 nibble columns per wave are the A
-// operand of v_mfma_i32_16x16x64_i8, a constant 0/+1/-1 window matrix B,
-// the threshold bias C; two instructions per wave and step give 16 rows x 32
-// groups, the second one with the counts of the next two plane words through
-// LDS. Build with SLIME_BAND_MFMA=0 to count those windows on the VALU.
+// With the union bound, two x-phases and 2, 7 or 8 bands (the full world's
+// launches and the commonest batched class), the 5-column window sums less
+// the corner cells are int8 contractions on the matrix cores instead: the
+// 20-row counts and corner counts of 16 band-phase rows x 32 nibble columns
+// per wave are the A operand of v_mfma_i32_16x16x64_i8, a constant 0/+1/-1
+// window matrix B, the threshold bias C; two instructions per wave and step
+// give 16 rows x 32 groups, the second one with the counts of the next two
+// plane words through LDS. Build with SLIME_BAND_MFMA=0 to count those
+// windows on the VALU.
 #pragma once
 
 #include <type_traits>
@@ -425,7 +426,10 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
     constexpr int kWThreads = 16 * NB * (kWSplit ? 2 : 1);  // threads that hold a band
     // Matrix-core window sums (MfmaRoles): the lane's plane word, band and
     // phase follow its row of A instead.
-    constexpr bool kMfma = SLIME_BAND_MFMA && kCorner && PH == 2 && NB >= 2 && NB <= 8;
+    // (measured per class: faster with 2, 7 and 8 bands; with 3-6 bands the
+    // exchange's workgroup barrier, or a fourth counting wave where the VALU
+    // needs three, costs more than the window sums save)
+    constexpr bool kMfma = SLIME_BAND_MFMA && kCorner && PH == 2 && (NB == 2 || NB == 7 || NB == 8);
     using MR = MfmaRoles<kMfma ? NB : 2>;
     const int mrow = lane & 15;
     const bool m_ok = kMfma && wave < MR::kWaves && mrow < MR::kRows;
@@ -589,7 +593,14 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
                     mxch[wd * 16 + mrow % MR::QN] = a0;
                 }
             }
-            __syncthreads();
+            if constexpr (MR::kWaves == 1) {
+                // one counting wave: its own lanes wrote the words it reads
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            } else {
+                __syncthreads();
+            }
             if (wave < MR::kWaves) {
                 // the second contraction starts two plane words on; the
                 // exchange is indexed by band-phase, since with several column
