@@ -60,13 +60,13 @@ VALU_PEAK_T = 157.3 / 2
 # VALU lane instructions per center of the full-world search's banded kernels,
 # counted by rocprofv3 (SQ_INSTS_VALU x 64 / centers over all launches;
 # profiles/scan_band_kernel_full_world_thr60.md).
-VALU_INSTS_PER_CENTER = 5.27
+VALU_INSTS_PER_CENTER = 5.03
 CPU_SAMPLE_RADIUS = 131072
 # HBM bytes per scan_band_kernel launch (rocprofv3 FETCH_SIZE + WRITE_SIZE of
 # the full-world run, scaled by the same counters on a copy of known size;
-# profiles/scan_band_kernel_full_world_thr60.md): 80.0 KiB x 2.0 + 0.02 KiB x 1.0
+# profiles/scan_band_kernel_full_world_thr60.md): 79.9 KiB x 2.0 + 0.02 KiB x 1.0
 # per launch, averaged over the search's 33 launches.
-TRAFFIC_BYTES = 163900
+TRAFFIC_BYTES = 163700
 
 
 def host_cpu():
@@ -560,6 +560,32 @@ def served_leg(n, rate, with_plain=False):
     return out
 
 
+def band_classes_leg(reps=7):
+    """Kernel rate of each banded class with two x-phases and the union
+    bound: strided sets of nb = 2..8 bands of 16,384 rows over 4 x-blocks
+    (1,048,576 columns), threshold 55, seed 0x1234567; T centers per second
+    of kernel time (median of `reps` scans)."""
+    import statistics
+    import slime_amd
+    p = 1 << 17
+    out = {}
+    for nb in range(2, 9):
+        w, rows = 8 * p, 16384
+        args = (0x1234567, -3 * p, -2 * p, w, (nb - 1) * p + rows, 55, 221, rows,
+                slime_amd.MODE_BANDED_UNION, 4096)
+        slime_amd.scan_bands(*args)
+        rates = []
+        for _ in range(reps):
+            slime_amd.kernel_stats(slime_amd.KERNEL_BANDED, reset=True)
+            if slime_amd.scan_bands(*args)[0] < 0:
+                raise SystemExit("scan_bands failed")
+            _, ns, centers = slime_amd.kernel_stats(slime_amd.KERNEL_BANDED)
+            rates.append(centers / ns / 1e3)
+        out[str(nb)] = round(statistics.median(rates), 4)
+        print(f"band class {nb}: {out[str(nb)]} T centers/s", file=sys.stderr, flush=True)
+    return out
+
+
 def window_ab_leg(blocks=3072, steps=4096, min_size=60, seed=0x5EED):
     """Counting stage alone (slime_window_ab): VALU word counting (method 0)
     against int8 MFMA window sums (method 1) over the same hashed bitplane
@@ -615,6 +641,9 @@ def main():
                          "sites (for profiling the filter)")
     ap.add_argument("--only-valu-rates", action="store_true",
                     help="run only the VALU issue-rate microbenchmark (for profiling it)")
+    ap.add_argument("--only-band-classes", action="store_true",
+                    help="run only the per-class rate of the banded kernels (two x-phases, "
+                         "union bound, 2-8 bands)")
     ap.add_argument("--only-window-ab", action="store_true",
                     help="run only the counting-stage A/B (VALU word counting vs int8 MFMA "
                          "window sums) at 8 and 4 bands")
@@ -649,6 +678,9 @@ def main():
         rates = {name: slime_amd.valu_rate(kind, 16384) / 1e12
                  for kind, name in enumerate(slime_amd.VALU_KINDS)}
         print(json.dumps({"valu_rates_t_per_s": rates, "peak": VALU_PEAK_T}), flush=True)
+        return 0
+    if args.only_band_classes:
+        print(json.dumps({"band_classes": band_classes_leg()}), flush=True)
         return 0
     if args.only_window_ab:
         print(json.dumps({"window_ab": window_ab_leg()}), flush=True)
