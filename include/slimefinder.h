@@ -105,10 +105,12 @@ void set_y_scan_config(int32_t enabled, int32_t platform_y, int32_t y_min, int32
 void score_candidates_precise(int64_t seed, ExtChunkResult* results_buffer, int32_t count,
                               int32_t threads, int32_t scan_xz);
 /* refine_candidates_y (SlimeCoreGPU.cu:3958-3990; the CPU library's version,
- * SlimeCore.cpp:728-773, adds a `threads` argument, accepted and ignored):
- * the GPU library's rule - equal counts go to the first (point, height) in
- * order, so a row whose counts are all zero gets point (-16, -16) and the
- * first height; always packed. */
+ * SlimeCore.cpp:728-773, adds a `threads` argument, accepted and ignored).
+ * By default the GPU library's rule - equal counts go to the first (point,
+ * height) in order, so a row whose counts are all zero gets point (-16, -16)
+ * and the first height; with SLIME_REFINE_RULE=cpu in the environment (read
+ * at every call) the CPU library's - the best starts at (0, center, platform
+ * y) and only a strictly larger count replaces it. Always packed. */
 void refine_candidates_y(int64_t seed, ExtChunkResult* results_buffer, int32_t count,
                          int32_t threads);
 

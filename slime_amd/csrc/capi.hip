@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -114,6 +115,20 @@ void precise_gpu(int64_t seed, ExtChunkResult* out, int64_t total, int32_t max_r
     (void)slime::refine_rows(
         refine_spec(seed, 1, scan_y ? slime::kRuleFirstOrder : slime::kRuleKeepCenter, scan_y),
         reinterpret_cast<slime::ResultRow*>(out), std::min(n, 100));
+}
+
+// refine_candidates_y is exported by both reference libraries, with different
+// arity and a different rule for rows whose counts are all 0: SlimeCore's
+// (SlimeCore.cpp:728-773, with a threads argument) keeps the center and the
+// platform height, SlimeCoreGPU's (SlimeCoreGPU.cu:3958-3990) takes the first
+// point and height in its order. A C caller's arity cannot be seen, so the
+// GPU library's rule is the default and SLIME_REFINE_RULE=cpu in the
+// environment (read at every call) selects SlimeCore's, for callers that load
+// this library in place of the CPU one. The extensions that refine the same
+// way (the many-world and the allocation-free forms) follow the same switch.
+int32_t shared_refine_rule() {
+    const char* v = std::getenv("SLIME_REFINE_RULE");
+    return v && std::strcmp(v, "cpu") == 0 ? slime::kRuleKeepCenter : slime::kRuleFirstOrder;
 }
 
 constexpr int64_t kCpuRdLimit = (INT32_MAX - 17LL) / 2LL;
@@ -287,7 +302,7 @@ void refine_candidates_y(int64_t seed, ExtChunkResult* results_buffer, int32_t c
     (void)threads;
     if (!results_buffer || count <= 0) return;
     control().progress.store(90, std::memory_order_relaxed);
-    (void)slime::refine_rows(refine_spec(seed, 1, slime::kRuleFirstOrder, 1),
+    (void)slime::refine_rows(refine_spec(seed, 1, shared_refine_rule(), 1),
                              reinterpret_cast<slime::ResultRow*>(results_buffer), count);
     control().progress.store(99, std::memory_order_relaxed);
 }
@@ -298,7 +313,7 @@ int32_t refine_candidates_y_noalloc(int64_t seed, ExtChunkResult* results_buffer
     if (count > slime::kNoAllocMaxRows) return -2;
     if (count == 0) return 0;
     control().progress.store(90, std::memory_order_relaxed);
-    const int rc = slime::refine_rows_noalloc(refine_spec(seed, 1, slime::kRuleFirstOrder, 1),
+    const int rc = slime::refine_rows_noalloc(refine_spec(seed, 1, shared_refine_rule(), 1),
                                               reinterpret_cast<slime::ResultRow*>(results_buffer),
                                               count);
     control().progress.store(99, std::memory_order_relaxed);
@@ -318,7 +333,7 @@ int32_t refine_candidates_y_multi(int32_t nworlds, const int64_t* world_seeds, c
     std::vector<int64_t> seeds((size_t)count);
     for (int32_t w = 0; w < nworlds; ++w)
         std::fill(seeds.begin() + first[w], seeds.begin() + first[w + 1], world_seeds[w]);
-    return slime::refine_rows_multi(refine_spec(0, 1, slime::kRuleFirstOrder, 1), seeds.data(),
+    return slime::refine_rows_multi(refine_spec(0, 1, shared_refine_rule(), 1), seeds.data(),
                                     reinterpret_cast<slime::ResultRow*>(results_buffer), count);
 }
 

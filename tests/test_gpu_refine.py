@@ -79,6 +79,56 @@ def test_all_zero_rows_follow_each_rule(oracle, yscan):
         assert s == r[:3] + (oracle.pack_obs_y(0, -64), r[1], r[2])
 
 
+@pytest.fixture
+def cpu_rule(monkeypatch):
+    """refine_candidates_y with SlimeCore's rule (SLIME_REFINE_RULE=cpu)."""
+    monkeypatch.setenv("SLIME_REFINE_RULE", "cpu")
+    yield
+
+
+@pytest.mark.parametrize("yconf", [DEFAULT_Y, (1, -64, -64, 320, 4), (1, -64, 100, 300, 10)])
+def test_refine_candidates_y_cpu_rule(oracle, yscan, cpu_rule, yconf):
+    """Under SLIME_REFINE_RULE=cpu the shared name follows SlimeCore.cpp:728-773:
+    the keep-center rule, always packed (all-zero rows keep the center and the
+    platform height)."""
+    import slime_amd
+    rows = case_rows(oracle, seed=7)
+    print("refine_candidates_y cpu rule", yconf, len(rows), flush=True)
+    yscan(*yconf)
+    got = slime_amd.refine_candidates_y(SEED, rows)
+    assert got == oracle.oracle_refine(SEED, rows, 1, yconf, oracle.RULE_KEEP_CENTER, True)
+    if yconf[2] == 100:  # every height out of reach
+        for r, s in zip(rows, got):
+            assert s == r[:3] + (oracle.pack_obs_y(0, yconf[1]), r[1], r[2])
+
+
+def test_refine_candidates_y_cpu_rule_matches_compiled_reference(oracle, yscan, cpu_rule):
+    """Where SlimeCore.cpp was built: its refine_candidates_y on the same rows
+    (searched rows, random rows and all-zero rows)."""
+    import slime_amd
+    ref = oracle.ref_lib()
+    if ref is None:
+        pytest.skip("reference SlimeCore.cpp not built")
+    for yconf in ((1, -64, -64, 320, 4), (1, -64, 100, 300, 10)):
+        rows = case_rows(oracle, seed=8)
+        yscan(*yconf)
+        got = slime_amd.refine_candidates_y(SEED, rows)
+        want = oracle.ref_refine(ref, "refine_y", SEED, rows, yconf)
+        ref.set_y_scan_config(*DEFAULT_Y)
+        assert got == want
+
+
+def test_refine_multi_and_noalloc_follow_cpu_rule(oracle, yscan, cpu_rule):
+    import slime_amd
+    rows = case_rows(oracle, 6, 2, seed=9)
+    yscan(1, -64, 100, 300, 10)
+    want = oracle.oracle_refine(SEED, rows, 1, (1, -64, 100, 300, 10), oracle.RULE_KEEP_CENTER, True)
+    assert slime_amd.refine_multi([SEED], [rows])[0] == want
+    buf = slime_amd._row_buffer(rows)
+    assert slime_amd.lib().refine_candidates_y_noalloc(SEED, buf, len(rows)) == 0
+    assert slime_amd._rows(buf, len(rows), len(rows)) == want
+
+
 @pytest.mark.parametrize("yconf", [DEFAULT_Y, (1, -64, -64, 320, 4)])
 @pytest.mark.parametrize("family", ["cpu", "gpu"])
 def test_search_with_precise_afk(oracle, yscan, yconf, family):
