@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <mutex>
 #include <new>
 #include <thread>
@@ -400,13 +401,142 @@ void plan_plain_rows(const SearchSpec& spec, int32_t x0, int64_t w, std::deque<W
     }
 }
 
+// ---- autotuner: measured cost of the banded kernels ------------------------
+// A large search's band grouping, and whether its last, partial band is
+// scanned as a short group of its own, are chosen by measured cost, as the
+// reference times its kernel shapes on a saturated sample before a large
+// search (SlimeCoreGPU.cu:3517-3599). Per (x-phases, bands) of the
+// union-bound kernels, saturated launches of synthetic tiles at two tile
+// heights give the cost of a tile as T * (rows + 16 + o): T ns per generated
+// row, o fixed rows. Measured once per device at the first search of at
+// least kTuneMinCenters centers; smaller searches, and SLIME_BAND_TUNE=0 in
+// the environment, keep the even grouping.
+constexpr int64_t kTuneMinCenters = int64_t(1) << 37;
+constexpr int32_t kTileHaloRows = 16;  // generated rows beyond a tile's slot rows
+
+struct BandCost {
+    double T = 0, o = 0;
+};
+struct BandCosts {
+    int device = -1;
+    bool ready = false, failed = false;
+    BandCost c[3][kUnionBoundMaxBands + 1];  // [x-phases][bands]
+};
+BandCosts g_costs;
+
+bool band_tune_enabled() {
+    const char* v = std::getenv("SLIME_BAND_TUNE");
+    return !(v && std::strcmp(v, "0") == 0);
+}
+
+// ns of one launch of nb bands and ph phases over `tiles` synthetic tiles of
+// tile_h slot rows (nothing reaches min_size, so nothing is emitted).
+bool time_band_launch(int nb, int ph, int32_t tile_h, int64_t want_tiles, double& ns_per_tile) {
+    SearchSpec spec;
+    spec.seed = 0x5EED5EED5EEDll + 17 * nb + ph;
+    spec.x0 = -kPeriod;
+    spec.z0 = 12345;
+    spec.w = ph == 2 ? 2 * kPeriod : kPeriod;
+    const int64_t row_tiles = std::max<int64_t>(1, (want_tiles + kTilesPerPhase - 1) / kTilesPerPhase);
+    const int64_t slot_rows = std::min<int64_t>(kPeriod / tile_h * tile_h, row_tiles * tile_h);
+    spec.h = (int64_t)(nb - 1) * kPeriod + slot_rows;
+    spec.band_rows = (int32_t)slot_rows;
+    spec.min_size = kDiskCells;
+    spec.max_size = kDiskCells;
+    std::vector<BandJob> jobs;
+    band_jobs(spec, x_parts(spec, ph), std::vector<int>{nb}, tile_h, int64_t(1) << 62, jobs);
+    if (jobs.size() != 1) return false;
+    BandParams p = jobs[0].p;
+    p.emit_min = kDiskCells + 1;
+    p.hits = g_buf.d_hits;
+    p.hit_cap = 0;
+    p.emitted = g_buf.d_counters;
+    p.hist = g_buf.d_counters + 1;
+    float ms = 0.f;
+    if (hipEventRecord(g_buf.ev_start, g_buf.stream) != hipSuccess ||
+        launch_scan_band(p, nb, ph, true, g_buf.stream) != hipSuccess ||
+        hipEventRecord(g_buf.ev_stop, g_buf.stream) != hipSuccess ||
+        hipStreamSynchronize(g_buf.stream) != hipSuccess ||
+        hipEventElapsedTime(&ms, g_buf.ev_start, g_buf.ev_stop) != hipSuccess || p.tiles_total <= 0)
+        return false;
+    ns_per_tile = (double)ms * 1e6 / (double)p.tiles_total;
+    return true;
+}
+
+bool calibrate_bands() {
+    if (g_costs.device == g_buf.device && (g_costs.ready || g_costs.failed)) return g_costs.ready;
+    g_costs = BandCosts{};
+    g_costs.device = g_buf.device;
+    g_costs.failed = true;
+    const int64_t slots = 3 * (int64_t)std::max(1, g_buf.cus);  // workgroups resident at 3 per CU
+    for (int ph = 1; ph <= 2; ++ph)
+        for (int nb = 1; nb <= kUnionBoundMaxBands; ++nb) {
+            double warm = 0, a[2] = {0, 0};
+            const int32_t ths[2] = {64, 1024};
+            if (!time_band_launch(nb, ph, ths[0], slots, warm) ||
+                !time_band_launch(nb, ph, ths[0], 24 * slots, a[0]) ||
+                !time_band_launch(nb, ph, ths[1], 6 * slots, a[1]))
+                return false;
+            const double r0 = ths[0] + kTileHaloRows, r1 = ths[1] + kTileHaloRows;
+            double T = (a[1] - a[0]) / (r1 - r0);
+            if (!(T > 0)) T = a[1] / r1;
+            g_costs.c[ph][nb].T = T;
+            g_costs.c[ph][nb].o = std::max(0.0, a[0] / T - r0);
+        }
+    g_costs.failed = false;
+    g_costs.ready = true;
+    return true;
+}
+
+// Tile height: the largest (up to kBandTileH) that still gives every CU a few
+// tiles when `col_tiles` x-tiles share each slot row; small regions trade halo
+// rows for occupancy.
+int32_t band_tile_h(int32_t slot_rows, int64_t col_tiles) {
+    const int64_t want_tiles = 4 * (int64_t)g_buf.cus * 3;
+    const int64_t fit_h = (slot_rows * col_tiles + want_tiles - 1) / want_tiles;
+    return (int32_t)std::min<int64_t>(std::min<int64_t>(kBandTileH, (slot_rows + 3) / 4 * 4),
+                                      std::max<int64_t>(kMinBandTileH, (fit_h + 3) / 4 * 4));
+}
+
+// Measured ns of a group of nb bands over s_rows slot rows of every x-part.
+double group_ns(const std::vector<XPart>& parts, int nb, int32_t s_rows, int32_t tile_h) {
+    double t = 0;
+    for (const XPart& xp : parts) {
+        const BandCost& c = g_costs.c[xp.ph][nb];
+        const int64_t tiles =
+            (int64_t)xp.nblocks * xp.tiles_per_block() * ((s_rows + tile_h - 1) / tile_h);
+        t += (double)tiles * c.T * ((double)tile_h + kTileHaloRows + c.o);
+    }
+    return t;
+}
+
+// Cheapest split of n consecutive bands into groups of at most max_bands
+// (sizes largest first), each over s_rows slot rows; its measured ns.
+double cheapest_groups(const std::vector<XPart>& parts, int64_t n, int max_bands, int32_t s_rows,
+                       int32_t tile_h, std::vector<int>& sizes) {
+    std::vector<double> cost((size_t)n + 1, -1.0);
+    std::vector<int> pick((size_t)n + 1, 0);
+    cost[0] = 0;
+    for (int64_t k = 1; k <= n; ++k)
+        for (int nb = 1; nb <= max_bands && nb <= k; ++nb) {
+            const double v = cost[k - nb] + group_ns(parts, nb, s_rows, tile_h);
+            if (cost[k] < 0 || v < cost[k]) {
+                cost[k] = v;
+                pick[k] = nb;
+            }
+        }
+    sizes.clear();
+    for (int64_t k = n; k > 0; k -= pick[k]) sizes.push_back(pick[k]);
+    std::sort(sizes.begin(), sizes.end(), std::greater<int>());
+    return cost[n];
+}
+
 // Banded launches over every center of the spec: two x-phases above P
 // columns, one below (x_parts); bands grouped evenly by at most g_max_bands
 // per launch, and at most kUnionBoundMaxBands where the union bound applies
 // (its kernels exist up to that many bands; at thresholds near 50 it beats
-// the square bound on more bands). Tile height: the largest (up to
-// kBandTileH) that still gives every CU a few tiles of the plan; small regions
-// trade halo rows for occupancy.
+// the square bound on more bands); for a large search, the grouping of the
+// measured cost model above instead.
 void plan_banded(const SearchSpec& spec, std::deque<Work>& out) {
     const std::vector<XPart> parts = x_parts(spec, spec.w > kPeriod ? 2 : 1);
     const int64_t nbz = bands_of(spec);
@@ -422,14 +552,39 @@ void plan_banded(const SearchSpec& spec, std::deque<Work>& out) {
     const int32_t slot_rows = slot_rows_of(spec);
     int64_t col_tiles = 0;  // x-tiles of one slot row over all parts and groups
     for (const XPart& xp : parts) col_tiles += (int64_t)xp.nblocks * xp.tiles_per_block();
-    col_tiles *= ngroups;
-    const int64_t want_tiles = 4 * (int64_t)g_buf.cus * 3;
-    const int64_t fit_h = (slot_rows * col_tiles + want_tiles - 1) / want_tiles;
-    const int32_t tile_h = (int32_t)std::min<int64_t>(
-        std::min<int64_t>(kBandTileH, (slot_rows + 3) / 4 * 4),
-        std::max<int64_t>(kMinBandTileH, (fit_h + 3) / 4 * 4));
+    const int32_t tile_h = band_tile_h(slot_rows, col_tiles * ngroups);
     std::vector<BandJob> jobs;
-    band_jobs(spec, parts, groups, tile_h, kBandCentersPerLaunch, jobs);
+    const bool tune = union_bound && spec.w * spec.h >= kTuneMinCenters && band_tune_enabled() &&
+                      calibrate_bands();
+    if (tune) {
+        // (a) every band over slot_rows in the cheapest groups; (b) the full
+        // bands so and the last, partial band alone over its own rows
+        std::vector<int> ga, gb;
+        const double ca = cheapest_groups(parts, nbz, max_bands, slot_rows, tile_h, ga);
+        const int64_t nf = spec.h / kPeriod, rem = spec.h % kPeriod;
+        double cb = -1;
+        int32_t th_rem = 0;
+        if (spec.band_rows == 0 && nf >= 1 && rem > 0) {
+            th_rem = band_tile_h((int32_t)rem, col_tiles);
+            cb = cheapest_groups(parts, nf, max_bands, kPeriod, tile_h, gb);
+            cb += group_ns(parts, 1, (int32_t)rem, th_rem);
+        }
+        if (cb >= 0 && cb < ca) {
+            int64_t b_first = 0;
+            for (const int nb : gb) {
+                for (const XPart& xp : parts)
+                    band_jobs_part(spec, xp, b_first, nb, kPeriod, tile_h, kBandCentersPerLaunch,
+                                   jobs);
+                b_first += nb;
+            }
+            for (const XPart& xp : parts)
+                band_jobs_part(spec, xp, nf, 1, (int32_t)rem, th_rem, kBandCentersPerLaunch, jobs);
+        } else {
+            band_jobs(spec, parts, ga, tile_h, kBandCentersPerLaunch, jobs);
+        }
+    } else {
+        band_jobs(spec, parts, groups, tile_h, kBandCentersPerLaunch, jobs);
+    }
     for (const BandJob& j : jobs) {
         Work wk;
         wk.banded = true;

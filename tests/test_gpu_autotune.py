@@ -1,0 +1,35 @@
+"""The per-search engine's measured band grouping (searches of at least 2^37
+centers; SLIME_BAND_TUNE=0 keeps the even grouping): identical totals,
+222-bin histograms and rows either way, every center decided once, on
+regions whose last band is partial (the plan may scan it as a short group of
+its own) and on a region of whole bands."""
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+P = 1 << 17
+CASES = [
+    # (seed, x0, z0, w, h, min_size)
+    (0x51A11, -200_000, -200_000, 400_001, 400_001, 50),          # 3 bands + 6,785 rows
+    (-987654321, 1_000_000, -3_000_000, 2 * P * 3 + 777, 9 * P + 50_000, 58),  # 9 bands + a part
+    (42, -P, -4 * P, 4 * P, 8 * P, 56),                           # whole bands, 8 of them
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_tuned_plan_equals_even_grouping(gpu, monkeypatch, case):
+    import slime_amd
+    seed, x0, z0, w, h, thr = CASES[case]
+    assert w * h >= 1 << 37
+    print("autotune case", case, seed, x0, z0, w, h, thr, flush=True)
+    L = slime_amd.lib()
+    monkeypatch.setenv("SLIME_BAND_TUNE", "1")
+    L.reset_cancel()  # the processed-centers counter runs until reset, as in the reference
+    tuned = slime_amd.scan_rect(seed, x0, z0, w, h, thr, 221, 5000)
+    assert L.get_processed_centers() == w * h
+    monkeypatch.setenv("SLIME_BAND_TUNE", "0")
+    L.reset_cancel()
+    even = slime_amd.scan_rect(seed, x0, z0, w, h, thr, 221, 5000)
+    assert L.get_processed_centers() == w * h
+    assert tuned[0] == even[0] and tuned[2] == even[2] and tuned[1] == even[1]
+    assert sum(tuned[2]) == tuned[0] and tuned[0] > 0
