@@ -252,7 +252,8 @@ def batched_leg(repeats):
         st = slime_amd.batch_stats()
         out[key] = {"seconds": dt, "t_centers_per_s": centers / dt / 1e12,
                     "kernel_s": st["kernel_ns"] / 1e9, "launches": st["launches"],
-                    "jobs": st["jobs"], "fallbacks": st["fallbacks"]}
+                    "jobs": st["jobs"], "fallbacks": st["fallbacks"],
+                    "hits_stored": st["hits_stored"], "hits_copied": st["hits_copied"]}
         print(f"batched {key}: {dt:.3f} s", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     per = [slime_amd.search_slime_clusters_gpu_v1_centered(*s, max_results=k) for s in searches]

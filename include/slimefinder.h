@@ -247,10 +247,13 @@ int32_t slime_search_batch(int32_t n, const int64_t* seeds, const int64_t* rd_ma
 /* Extension: figures of the last batch: candidate centers, device time of the
  * sweep launches, host time of planning + fix-up + upload, of the sweep, and
  * of the whole call (ns), kernel launches, jobs, searches rerun alone after a
- * hit-region overflow. Any pointer may be NULL. */
+ * hit-region overflow, hits stored in the searches' device regions and how
+ * many of them came back to the host to be ranked (a search's best rows are
+ * selected on the device first: only the hits at or above the score its
+ * top max_results reach). Any pointer may be NULL. */
 void slime_batch_stats(int64_t* centers, int64_t* kernel_ns, int64_t* prepare_ns,
                        int64_t* sweep_ns, int64_t* wall_ns, int32_t* launches, int32_t* jobs,
-                       int32_t* fallbacks);
+                       int32_t* fallbacks, int64_t* hits_stored, int64_t* hits_copied);
 
 /* Extension: the autotuner's measured cost table, one entry per kernel class
  * (bands nb, x-phases ph): device ns per tile row with the whole GPU on that

@@ -138,7 +138,7 @@ def lib():
     L.slime_search_batch.argtypes = [i32, PI64, PI64, PI32, PI32, PI64, PI32, PI32, P, i32, PI64,
                                      i32]
     L.slime_search_batch.restype = i32
-    L.slime_batch_stats.argtypes = [PI64, PI64, PI64, PI64, PI64, PI32, PI32, PI32]
+    L.slime_batch_stats.argtypes = [PI64, PI64, PI64, PI64, PI64, PI32, PI32, PI32, PI64, PI64]
     L.slime_batch_stats.restype = None
     L.slime_sweep_costs.argtypes = [PD, PD, PI32, PI32, i32]
     L.slime_sweep_costs.restype = i32
@@ -410,11 +410,14 @@ def batch_stats():
     """Figures of the last batch as a dict (slime_batch_stats)."""
     v64 = [C.c_int64() for _ in range(5)]
     v32 = [C.c_int32() for _ in range(3)]
-    lib().slime_batch_stats(*[C.byref(x) for x in v64], *[C.byref(x) for x in v32])
+    hits = [C.c_int64() for _ in range(2)]
+    lib().slime_batch_stats(*[C.byref(x) for x in v64], *[C.byref(x) for x in v32],
+                            *[C.byref(x) for x in hits])
     keys64 = ("centers", "kernel_ns", "prepare_ns", "sweep_ns", "wall_ns")
     keys32 = ("launches", "jobs", "fallbacks")
     d = {k: x.value for k, x in zip(keys64, v64)}
     d.update({k: x.value for k, x in zip(keys32, v32)})
+    d["hits_stored"], d["hits_copied"] = hits[0].value, hits[1].value
     return d
 
 

@@ -432,7 +432,7 @@ int32_t slime_search_batch(int32_t n, const int64_t* seeds, const int64_t* rd_ma
 
 void slime_batch_stats(int64_t* centers, int64_t* kernel_ns, int64_t* prepare_ns,
                        int64_t* sweep_ns, int64_t* wall_ns, int32_t* launches, int32_t* jobs,
-                       int32_t* fallbacks) {
+                       int32_t* fallbacks, int64_t* hits_stored, int64_t* hits_copied) {
     slime::BatchStats st;
     slime::last_batch_stats(&st);
     if (centers) *centers = st.centers;
@@ -443,6 +443,8 @@ void slime_batch_stats(int64_t* centers, int64_t* kernel_ns, int64_t* prepare_ns
     if (launches) *launches = st.launches;
     if (jobs) *jobs = st.jobs;
     if (fallbacks) *fallbacks = st.fallbacks;
+    if (hits_stored) *hits_stored = st.hits_stored;
+    if (hits_copied) *hits_copied = st.hits_copied;
 }
 
 int32_t slime_sweep_costs(double* ns_row, double* extra_rows, int32_t* nb, int32_t* ph,

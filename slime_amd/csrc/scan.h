@@ -158,6 +158,11 @@ int sweep_blocks_per_cu(int kernel);
 hipError_t launch_sweep(const SweepArgs& a, int kernel, int grid, hipStream_t stream);
 // Sets n words to v with a kernel (usable inside a captured graph).
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t stream);
+// Copies the hits of hits[0, n) whose size is at least min_size into out (at
+// most cap of them, in no particular order); *count, cleared by the caller,
+// ends as the number that qualified (which may exceed cap).
+hipError_t launch_select_hits(const Hit* hits, int64_t n, int32_t min_size, Hit* out,
+                              uint32_t cap, unsigned int* count, hipStream_t stream);
 
 // ---- first-draw mismatches (Java's nextInt(10) rejection) ------------------
 //

@@ -237,6 +237,37 @@ hipError_t launch_batch(const BatchArgs& a, int kernel, int64_t blocks, hipStrea
     }
 }
 
+// One atomic per wave: the lanes that qualify take consecutive slots from a
+// wave-wide ballot.
+__global__ __launch_bounds__(256) void select_hits_kernel(const Hit* __restrict__ hits, int64_t n,
+                                                          int32_t min_size, Hit* __restrict__ out,
+                                                          uint32_t cap, unsigned int* count) {
+    const int lane = threadIdx.x & 63;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+        const int64_t i = i0 + threadIdx.x;
+        const bool take = i < n && hits[i].size >= min_size;
+        const unsigned long long m = __ballot(take);
+        if (m == 0) continue;
+        unsigned int base = 0;
+        if (lane == 0) base = atomicAdd(count, (unsigned int)__popcll(m));
+        base = (unsigned int)__shfl((int)base, 0);
+        if (take) {
+            const unsigned int idx = base + (unsigned int)__popcll(m & ((1ull << lane) - 1ull));
+            if (idx < cap) out[idx] = hits[i];
+        }
+    }
+}
+
+hipError_t launch_select_hits(const Hit* hits, int64_t n, int32_t min_size, Hit* out,
+                              uint32_t cap, unsigned int* count, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 2048);
+    hipLaunchKernelGGL(select_hits_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, hits, n,
+                       min_size, out, cap, count);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t stream) {
     if (n == 0) return hipSuccess;
     const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);

@@ -52,6 +52,8 @@ struct SweepRequest {
     int64_t total = -1;
     bool fallback = false;
     bool truncated = false;  // cancelled with more hits than its region held
+    int64_t hits_stored = 0;  // hits in the request's device region at the end
+    int64_t hits_copied = 0;  // of them, brought to the host to rank
     std::vector<ResultRow> rows;
 };
 
@@ -126,6 +128,8 @@ class Sweeper {
     TaggedMismatch* d_mm_ = nullptr;
     unsigned int* d_mm_count_ = nullptr;
     CenterFix* d_fixes_ = nullptr;
+    Hit* d_select_ = nullptr;  // a finishing request's hits that can still rank (kSelectCap)
+    unsigned int* d_select_count_ = nullptr;
     CostTable costs_;
     std::vector<int32_t> free_slots_;
     struct Ranges {
@@ -159,6 +163,8 @@ struct BatchStats {
     int32_t launches = 0;
     int32_t jobs = 0;
     int32_t fallbacks = 0;  // searches rerun alone after a hit-region overflow
+    int64_t hits_stored = 0;  // hits in the searches' device regions
+    int64_t hits_copied = 0;  // of them, brought to the host to rank (top-K selection)
 };
 
 // Runs n searches in one sweep. out holds n * max_results rows (search i at

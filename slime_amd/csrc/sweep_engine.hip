@@ -56,6 +56,8 @@ double center_tail(int t) {
 constexpr int64_t kMaxHitCap = int64_t(1) << 24;
 constexpr int32_t kMaxTileH = 2048;
 constexpr int32_t kTileHalo = 16;  // generated rows beyond the tile's slot rows
+// Hits a finishing request may bring back after the device selection (4 MiB).
+constexpr uint32_t kSelectCap = 1u << 18;
 
 }  // namespace
 
@@ -116,6 +118,8 @@ void Sweeper::release() {
     f(d_mm_);
     f(d_mm_count_);
     f(d_fixes_);
+    f(d_select_);
+    f(d_select_count_);
     if (h_done_) (void)hipHostFree(h_done_);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
@@ -132,6 +136,8 @@ void Sweeper::release() {
     d_mm_ = nullptr;
     d_mm_count_ = nullptr;
     d_fixes_ = nullptr;
+    d_select_ = nullptr;
+    d_select_count_ = nullptr;
     h_done_ = nullptr;
     ev0_ = ev1_ = nullptr;
     stream_ = nullptr;
@@ -178,6 +184,8 @@ bool Sweeper::init(int32_t req_cap, int64_t job_cap, int64_t hit_cap) {
         hipMalloc(&d_mm_, sizeof(TaggedMismatch) * kMismatchCap) == hipSuccess &&
         hipMalloc(&d_mm_count_, sizeof(unsigned int)) == hipSuccess &&
         hipMalloc(&d_fixes_, sizeof(CenterFix) * fix_cap_) == hipSuccess &&
+        hipMalloc(&d_select_, sizeof(Hit) * kSelectCap) == hipSuccess &&
+        hipMalloc(&d_select_count_, sizeof(unsigned int)) == hipSuccess &&
         hipMemsetAsync(d_status_, 0, sizeof(int32_t) * req_cap, stream_) == hipSuccess &&
         hipStreamSynchronize(stream_) == hipSuccess;
     if (!ok) {
@@ -788,17 +796,9 @@ bool Sweeper::finalize(SweepRequest& r, bool partial) {
     // hit_cap emitted if the region overflowed: then r.truncated).
     const size_t stored = (size_t)std::min<int64_t>((int64_t)emitted, r.hit_cap);
     r.truncated = (int64_t)emitted > r.hit_cap;
-    std::vector<Hit> hits(stored);
-    if (stored && (hipMemcpyAsync(hits.data(), d_hits_ + r.hit_off, sizeof(Hit) * stored,
-                                  hipMemcpyDeviceToHost, stream_) != hipSuccess ||
-                   hipStreamSynchronize(stream_) != hipSuccess))
-        return false;
     int64_t h[kHistBins];
     for (int s = 0; s < kHistBins; ++s) h[s] = (int64_t)hist[s];
     std::vector<Hit> cand;
-    cand.reserve(hits.size());
-    for (const Hit& x : hits)
-        if (!r.mm.affects(x.cx, x.cz)) cand.push_back(Hit{x.size, x.cx, x.cz, 0});
     // Replace the provisional first-draw counts of recounted centers: all of
     // them, or for a cancelled request those in finished tiles (every job's
     // tiles finish in index order, so its first done[j] tiles).
@@ -821,6 +821,52 @@ bool Sweeper::finalize(SweepRequest& r, bool partial) {
     int64_t total = 0;
     for (int s = 0; s < kHistBins; ++s) total += h[s];
     r.total = total;
+    // Top-K on the device: with the corrected histogram, the best `keep`
+    // centers all score at least t, the highest score with >= keep centers at
+    // or above it. Only the stored hits of score >= t come back (those near a
+    // Java-rejected chunk are dropped below and replaced by the fixes, which
+    // the histogram already counts). A truncated region, or more such hits
+    // than the staging buffer holds, brings every stored hit back.
+    int32_t t = r.spec.min_size;
+    for (int64_t s = kHistBins - 1, acc = 0; s > r.spec.min_size; --s) {
+        acc += h[s];
+        if (acc >= (int64_t)r.keep) {
+            t = (int32_t)s;
+            break;
+        }
+    }
+    std::vector<Hit> hits;
+    bool selected = false;
+    if (!r.truncated && t > r.spec.min_size && stored > (size_t)r.keep) {
+        unsigned int n_sel = 0;
+        if (launch_fill_u32(d_select_count_, 1, 0u, stream_) != hipSuccess ||
+            launch_select_hits(d_hits_ + r.hit_off, (int64_t)stored, t, d_select_, kSelectCap,
+                               d_select_count_, stream_) != hipSuccess ||
+            hipMemcpyAsync(&n_sel, d_select_count_, sizeof(n_sel), hipMemcpyDeviceToHost,
+                           stream_) != hipSuccess ||
+            hipStreamSynchronize(stream_) != hipSuccess)
+            return false;
+        if (n_sel <= kSelectCap) {
+            hits.resize(n_sel);
+            if (n_sel && (hipMemcpyAsync(hits.data(), d_select_, sizeof(Hit) * n_sel,
+                                         hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+                          hipStreamSynchronize(stream_) != hipSuccess))
+                return false;
+            selected = true;
+        }
+    }
+    if (!selected) {
+        hits.resize(stored);
+        if (stored && (hipMemcpyAsync(hits.data(), d_hits_ + r.hit_off, sizeof(Hit) * stored,
+                                      hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+                       hipStreamSynchronize(stream_) != hipSuccess))
+            return false;
+    }
+    r.hits_stored = (int64_t)stored;
+    r.hits_copied = (int64_t)hits.size();
+    cand.reserve(cand.size() + hits.size());
+    for (const Hit& x : hits)
+        if (!r.mm.affects(x.cx, x.cz)) cand.push_back(Hit{x.size, x.cx, x.cz, 0});
     r.rows.assign((size_t)std::min<int64_t>((int64_t)cand.size(), r.keep), ResultRow{});
     write_rows(cand, (size_t)r.keep, r.rows.data());
     return true;
@@ -908,6 +954,8 @@ int run_batch(const SearchSpec* specs, int32_t n, int32_t max_results, ResultRow
         st.centers += r.centers;
         st.fallbacks += r.fallback ? 1 : 0;
         st.jobs += (int32_t)r.jobs.size();
+        st.hits_stored += r.hits_stored;
+        st.hits_copied += r.hits_copied;
     }
     cleanup();
     st.kernel_ns = rs.kernel_ns;

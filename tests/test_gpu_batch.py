@@ -102,3 +102,52 @@ def test_batch_rejected_chunks(gpu, oracle):
     for s, (total, rows) in zip(searches, got):
         want_total, want_rows = oracle.oracle_search(*s, 400)
         assert (total, rows) == (want_total, want_rows), s
+
+
+def test_batch_top_k_selected_on_device(gpu):
+    """Dense thresholds and small K: only the hits at or above the score the
+    top K reach come back from the device; rows identical to separate calls."""
+    import slime_amd
+    rng = random.Random(31)
+    searches = [(rng.getrandbits(64) - (1 << 63), rd, thr, 221, 0,
+                 rng.randint(-10 ** 6, 10 ** 6), rng.randint(-10 ** 6, 10 ** 6))
+                for rd, thr in ((3000, 40), (20000, 44), (70000, 48), (150000, 50), (500, 36))]
+    for k in (1, 37, 1000):
+        got = run_batch(searches, k)
+        st = slime_amd.batch_stats()
+        print("top-k", k, st["hits_stored"], st["hits_copied"], flush=True)
+        assert st["hits_copied"] < st["hits_stored"]
+        for s, (total, rows) in zip(searches, got):
+            want = slime_amd.search_slime_clusters_gpu_v1_centered(*s, k)
+            assert (total, rows) == want, (s, k)
+
+
+def test_batch_top_k_with_rejected_chunks_vs_oracle(gpu, oracle):
+    """A Java-rejected chunk inside each disk of a dense search with K = 5:
+    the recounted centers compete with the device-selected hits."""
+    from test_gpu_banded import rejected_state, seed_with_state
+    rng = random.Random(77)
+    lib = oracle.oracle_lib()
+    searches = []
+    for i in range(4):
+        state1 = rejected_state(rng, i % 2 == 1)
+        cx, cz = rng.randint(-10 ** 5, 10 ** 5), rng.randint(-10 ** 5, 10 ** 5)
+        x, z = cx + rng.randint(-20, 20), cz + rng.randint(-20, 20)
+        seed = seed_with_state(state1, x, z, rng.getrandbits(16))
+        assert lib.oracle_first_draw(seed, x, z) != lib.oracle_is_slime(seed, x, z)
+        searches.append((seed, 300, 34, 221, 0, cx, cz))
+    got = run_batch(searches, 5)
+    for s, (total, rows) in zip(searches, got):
+        assert (total, rows) == oracle.oracle_search(*s, 5), s
+
+
+def test_batch_selection_larger_than_staging_falls_back(gpu):
+    """More hits at or above the K-th score than the 262,144-hit staging
+    buffer: every stored hit comes back, rows still identical."""
+    import slime_amd
+    s = (0x7A11BACC, 100000, 40, 221, 0, 12345, -54321)
+    k = 300000
+    (total, rows), = run_batch([s], k)
+    st = slime_amd.batch_stats()
+    assert total > k and st["hits_copied"] == st["hits_stored"]
+    assert (total, rows) == slime_amd.search_slime_clusters_gpu_v1_centered(*s, k)
