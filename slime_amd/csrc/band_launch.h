@@ -12,7 +12,7 @@ template <int NB, int PH, bool UB>
 __global__ __launch_bounds__(kThreads, 3) void scan_band_kernel(BandParams p) {
     __shared__ TileLds<NB> lds;
     for (int i = threadIdx.x; i < kHistBins; i += kThreads) lds.lhist[i] = 0;
-    band_tile<NB, PH, UB>(p, p.emit_min, blockIdx.x, lds);
+    band_tile<NB, PH, UB>(p, p.emit_min, blockIdx.x, lds, (int)threadIdx.x);
     flush_hist(lds.lhist, p.hist);
 }
 

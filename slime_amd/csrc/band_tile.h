@@ -348,15 +348,17 @@ __device__ __forceinline__ void refine_wave(uint32_t todo, const CodeOf& code_of
 // count at offset 16 + r is taken now and at offset r (four steps later) is
 // queued a byte per step. Two masked popcounts per row and band-phase, and no
 // refinement of the many groups whose square passes at thresholds near 50.
+// tid is the thread's index in the workgroup (threadIdx.x; a loop over tiles
+// passes a copy the compiler cannot see through, so that the tile's per-lane
+// constants are computed per tile rather than kept live across the loop).
 template <int NB, int PH, bool UB>
 __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min, int tile,
-                                          TileLds<NB>& lds) {
+                                          TileLds<NB>& lds, int tid) {
     static_assert(PH == 1 || PH == 2, "one or two x-phases");
     using R = Ring<NB>;
     using C = Roles<NB, PH>;
     uint32_t* ring = lds.ring;
     uint32_t* lhist = lds.lhist;
-    const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     if (tid < kRing) {

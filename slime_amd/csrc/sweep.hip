@@ -58,9 +58,11 @@ __device__ bool claim_tile(const SweepArgs& a, int& pos, int& job, int& tile) {
 // Waves per SIMD each class is built for: the occupancy its per-search kernel
 // reaches with the same tile code (fewer bands need fewer registers).
 // The union-bound kernels (index kSweepClasses + class) keep their class's
-// occupancy where the extra rolling state fits.
-constexpr int kSweepWaves[kSweepKernels] = {6, 5, 5, 5, 5, 5, 4, 4, 4, 3, 3,
-                                            6, 5, 4, 4, 5, 5, 4, 4, 3, 3};
+// occupancy where the extra rolling state fits. The 2-band, two-phase class
+// runs at four waves in both forms (at five, with the claim loop around the
+// tile and, in the union-bound form, the matrix-core window sums, it spilled).
+constexpr int kSweepWaves[kSweepKernels] = {6, 5, 5, 5, 5, 4, 4, 4, 4, 3, 3,
+                                            6, 5, 4, 4, 5, 4, 4, 4, 3, 3};
 
 template <int K>
 __global__ __launch_bounds__(kThreads, kSweepWaves[K]) void sweep_kernel(SweepArgs a) {
@@ -89,7 +91,13 @@ __global__ __launch_bounds__(kThreads, kSweepWaves[K]) void sweep_kernel(SweepAr
         if (job < 0) break;
         // read in place (scalar loads as needed) rather than copied into registers
         const BandParams& p = a.jobs[job].p;
-        band::band_tile<NB, PH, UB>(p, p.emit_min, tile, lds);
+        // a copy of the thread index the compiler cannot see through: the
+        // tile's per-lane constants are then computed per tile instead of
+        // being hoisted out of the claim loop (live across tiles, they cost
+        // registers and waves)
+        int tid;
+        asm volatile("v_mov_b32 %0, %1" : "=v"(tid) : "v"((int)threadIdx.x));
+        band::band_tile<NB, PH, UB>(p, p.emit_min, tile, lds, tid);
         band::flush_hist(lds.lhist, p.hist);
         if (threadIdx.x == 0) atomicAdd(&a.done[job], 1u);
         __syncthreads();
@@ -117,7 +125,8 @@ __global__ __launch_bounds__(kThreads, 3) void batch_kernel(BatchArgs a) {
     }
     const int job = a.order[lo];
     const BandParams p = a.jobs[job].p;
-    band::band_tile<NB, PH, UB>(p, p.emit_min, a.start[lo] + (int)(blk - a.first[lo]), lds);
+    band::band_tile<NB, PH, UB>(p, p.emit_min, a.start[lo] + (int)(blk - a.first[lo]), lds,
+                                (int)threadIdx.x);
     band::flush_hist(lds.lhist, p.hist);
     if (threadIdx.x == 0) atomicAdd(&a.done[job], 1u);
 }
