@@ -563,14 +563,17 @@ def served_leg(n, rate, with_plain=False):
 
 def band_classes_leg(reps=7):
     """Kernel rate of each banded class with two x-phases and the union
-    bound: strided sets of nb = 2..8 bands of 16,384 rows over 4 x-blocks
+    bound: strided sets of nb = 2..14 bands of 16,384 rows over 4 x-blocks
     (1,048,576 columns), threshold 55, seed 0x1234567; T centers per second
     of kernel time (median of `reps` scans)."""
     import statistics
     import slime_amd
     p = 1 << 17
     out = {}
-    for nb in range(2, 9):
+    # one launch per class: no measured regrouping (the classes above 12 bands
+    # also need SLIME_MAX_BANDS=14 in the environment)
+    os.environ["SLIME_BAND_TUNE"] = "0"
+    for nb in range(2, 15):
         w, rows = 8 * p, 16384
         args = (0x1234567, -3 * p, -2 * p, w, (nb - 1) * p + rows, 55, 221, rows,
                 slime_amd.MODE_BANDED_UNION, 4096)
@@ -644,7 +647,7 @@ def main():
                     help="run only the VALU issue-rate microbenchmark (for profiling it)")
     ap.add_argument("--only-band-classes", action="store_true",
                     help="run only the per-class rate of the banded kernels (two x-phases, "
-                         "union bound, 2-8 bands)")
+                         "union bound, 2-14 bands)")
     ap.add_argument("--only-window-ab", action="store_true",
                     help="run only the counting-stage A/B (VALU word counting vs int8 MFMA "
                          "window sums) at 8 and 4 bands")

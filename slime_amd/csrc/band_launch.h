@@ -18,9 +18,10 @@ __global__ __launch_bounds__(kThreads, 3) void scan_band_kernel(BandParams p) {
 
 template <int NB, int PH>
 hipError_t launch_one(const BandParams& p, bool ub, hipStream_t stream) {
-    if (ub && NB <= kUnionBoundMaxBands)
-        hipLaunchKernelGGL((scan_band_kernel<NB, PH, NB <= kUnionBoundMaxBands>),
-                           dim3(p.tiles_total), dim3(kThreads), 0, stream, p);
+    constexpr bool kUnion = NB <= union_bound_max_bands(PH);
+    if (ub && kUnion)
+        hipLaunchKernelGGL((scan_band_kernel<NB, PH, kUnion>), dim3(p.tiles_total),
+                           dim3(kThreads), 0, stream, p);
     else
         hipLaunchKernelGGL((scan_band_kernel<NB, PH, false>), dim3(p.tiles_total),
                            dim3(kThreads), 0, stream, p);

@@ -194,20 +194,25 @@ __device__ __forceinline__ uint32_t group_code(int gbit, int b, int ph) {
 
 typedef int mfma_i8x16 __attribute__((ext_vector_type(4)));  // 16 int8 / 4 int32
 
-// Matrix-core window sums with two x-phases and NB (2..8) bands: QN = 2 NB
-// band-phases and H column ranges make the 16 rows of A (H QN <= 16 rows
-// used). Lane l of counting wave w holds row m = l % 16 (band-phase m % QN,
-// range m / QN) over plane word (m / QN) * kRangeWords + 4 w + l / 16, i.e.
-// a wave covers 32 nibble columns of each range; the waves past kWaves do
-// not count. D (lane l, register i): row 4 (l / 16) + i, group l % 16.
+// Matrix-core window sums with two x-phases and NB (2..14) bands: QN = 2 NB
+// band-phases. Up to 8 bands, QN band-phases and H column ranges make the 16
+// rows of A (H QN <= 16 rows used); lane l of counting wave w holds row
+// m = l % 16 (band-phase m % QN, range m / QN) over plane word
+// (m / QN) * kRangeWords + 4 w + l / 16, i.e. a wave covers 32 nibble columns
+// of each range; the waves past kWaves do not count. Above 8 bands there are
+// two 16-row sets (kSets): lane l holds row m of set s, band-phase 16 s + m
+// (set 1 has QN - 16 rows), both over plane word 4 w + l / 16. D (lane l,
+// register i): row 4 (l / 16) + i, group l % 16.
 template <int NB>
 struct MfmaRoles {
     static constexpr int QN = 2 * NB;
-    static constexpr int H = QN > 8 ? 1 : QN > 4 ? 2 : 4;
-    static constexpr int kRows = QN * H;
+    static constexpr int kSets = QN > 16 ? 2 : 1;
+    static constexpr int H = kSets == 2 || QN > 8 ? 1 : QN > 4 ? 2 : 4;
+    static constexpr int kRows = kSets == 2 ? 16 : QN * H;  // rows of set 0
+    static constexpr int kRows1 = kSets == 2 ? QN - 16 : 0;  // rows of set 1
     static constexpr int kRangeWords = 16 / H;
     static constexpr int kWaves = 4 / H;
-    static_assert(NB >= 2 && NB <= 8 && kRows <= 16, "2..8 bands");
+    static_assert(NB >= 2 && NB <= 16 && kRows <= 16 && kRows1 <= 16, "2..16 bands");
 };
 
 // B of the window contraction for lane l: column n = l % 16 (the n-th group
@@ -428,10 +433,11 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
     constexpr int kWThreads = 16 * NB * (kWSplit ? 2 : 1);  // threads that hold a band
     // Matrix-core window sums (MfmaRoles): the lane's plane word, band and
     // phase follow its row of A instead.
-    // (measured per class: faster with 2, 7 and 8 bands; with 3-6 bands the
-    // exchange's workgroup barrier, or a fourth counting wave where the VALU
-    // needs three, costs more than the window sums save)
-    constexpr bool kMfma = SLIME_BAND_MFMA && kCorner && PH == 2 && (NB == 2 || NB == 7 || NB == 8);
+    // (measured per class: faster with 2 and 7-8 bands, and the only
+    // union-bound form above 8; with 3-6 bands the exchange's workgroup
+    // barrier, or a fourth counting wave where the VALU needs three, costs
+    // more than the window sums save)
+    constexpr bool kMfma = SLIME_BAND_MFMA && kCorner && PH == 2 && (NB == 2 || NB >= 7);
     using MR = MfmaRoles<kMfma ? NB : 2>;
     const int mrow = lane & 15;
     const bool m_ok = kMfma && wave < MR::kWaves && mrow < MR::kRows;
@@ -475,33 +481,55 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
     // 0x8000 - min_size in both 16-bit lanes: bit 15 / 31 of square + this is
     // set when the lane's square reaches min_size
     const uint32_t wbias = (uint32_t)(0x8000 - p.min_size) * 0x10001u;
-    // kMfma: bit 4 f + i of `mvalid` says that D register i of the f-th
-    // contraction (row 4 (lane / 16) + i, group 16 f + lane % 16 of the row's
-    // first plane word) has a center inside the job; rows i = 0, 1 are band
-    // mband[0], rows 2, 3 band mband[1]. C is 0x8000 - min_size, so that bit
-    // 15 of a sum says it reaches min_size. Words 16 and 17 of the exchange
-    // (past the row) stay zero.
-    __shared__ mfma_i8x16 mxch[kMfma ? (kPlaneDw + 2) * 16 : 1];
+    // kMfma: bit 8 s + 4 f + i of `mvalid` says that D register i of the f-th
+    // contraction of set s (row 4 (lane / 16) + i, group 16 f + lane % 16 of
+    // the row's first plane word) has a center inside the job; rows i = 0, 1
+    // are band mband[s][0], rows 2, 3 band mband[s][1]. C is 0x8000 -
+    // min_size, so that bit 15 of a sum says it reaches min_size. Each set has
+    // its exchange (18 words of 16 band-phases); words 16 and 17 (past the
+    // row) stay zero. With two sets, set 1's band is 8 more than set 0's (the
+    // same phase and E / ~E selection); the lanes past its rows read set 0's
+    // plane and give zero rows.
+    constexpr int kXchSet = (kPlaneDw + 2) * 16;
+    __shared__ mfma_i8x16 mxch[kMfma ? kXchSet * MR::kSets : 1];
+    const bool m_ok1 = kMfma && MR::kSets == 2 && wave < MR::kWaves && mrow < MR::kRows1;
+    const int mplane1 = m_ok1 ? wbc + 9 : wbc + 1;
+    const int mq = kMfma && MR::kSets == 1 ? mrow % MR::QN : mrow;  // row within its set
     uint32_t mvalid = 0;
-    int mband[2] = {NB, NB};
+    int mband[MR::kSets][2];
+#pragma unroll
+    for (int s = 0; s < MR::kSets; ++s) mband[s][0] = mband[s][1] = NB;
     mfma_i8x16 mB = {0, 0, 0, 0}, mC = {0, 0, 0, 0};
     if constexpr (kMfma) {
-        if (tid < 32) mxch[kPlaneDw * 16 + tid] = mfma_i8x16{0, 0, 0, 0};
+        static_assert(MR::kSets == kWPh, "one rolling state per operand set");
+        if (tid < 32 * MR::kSets)
+            mxch[(tid >> 5) * kXchSet + kPlaneDw * 16 + (tid & 31)] = mfma_i8x16{0, 0, 0, 0};
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int md = 4 * (lane >> 4) + i;
-            if (wave >= MR::kWaves || md >= MR::kRows) continue;
-            const int qd = md % MR::QN, hd = md / MR::QN;
-            if ((i & 1) == 0) mband[i >> 1] = qd >> 1;
+        for (int s = 0; s < MR::kSets; ++s)
 #pragma unroll
-            for (int f = 0; f < 2; ++f) {
-                const int g = 8 * (hd * MR::kRangeWords + 4 * wave) + 16 * f + (lane & 15);
-                const int64_t ox0 = block_off + (int64_t)(qd & 1) * kPeriod + col0 + 4 * g;
-                if (g < kGroupCols && col0 + 4 * g < p.phase_cols && ox0 < p.w &&
-                    ox0 + 3 >= p.ox_min)
-                    mvalid |= 1u << (4 * f + i);
+            for (int i = 0; i < 4; ++i) {
+                const int md = 4 * (lane >> 4) + i;
+                if (wave >= MR::kWaves) continue;
+                int qd, hd;
+                if constexpr (MR::kSets == 2) {
+                    qd = 16 * s + md;
+                    hd = 0;
+                    if (qd >= MR::QN) continue;
+                } else {
+                    if (md >= MR::kRows) continue;
+                    qd = md % MR::QN;
+                    hd = md / MR::QN;
+                }
+                if ((i & 1) == 0) mband[s][i >> 1] = qd >> 1;
+#pragma unroll
+                for (int f = 0; f < 2; ++f) {
+                    const int g = 8 * (hd * MR::kRangeWords + 4 * wave) + 16 * f + (lane & 15);
+                    const int64_t ox0 = block_off + (int64_t)(qd & 1) * kPeriod + col0 + 4 * g;
+                    if (g < kGroupCols && col0 + 4 * g < p.phase_cols && ox0 < p.w &&
+                        ox0 + 3 >= p.ox_min)
+                        mvalid |= 1u << (8 * s + 4 * f + i);
+                }
             }
-        }
         mB = window_matrix(lane);
         const int bias = 0x8000 - p.min_size;
         mC = mfma_i8x16{bias, bias, bias, bias};
@@ -560,39 +588,53 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
         const int32_t s_g = tile_s0 + 4 * m;  // first slot row of the groups
         uint32_t pass = 0;  // owned band-phases whose square reaches min_size
         if constexpr (kMfma) {
-            // the vertical part as below (one phase a thread), then the lane's
-            // counts and corner counts as its 16 bytes of A
-            mfma_i8x16 a0 = {0, 0, 0, 0};
+            // the vertical part as below (one phase a thread, per operand
+            // set), then the lane's counts and corner counts as its 16 bytes
+            // of A
+            mfma_i8x16 a0[MR::kSets];
+#pragma unroll
+            for (int s = 0; s < MR::kSets; ++s) a0[s] = mfma_i8x16{0, 0, 0, 0};
             if (wave < MR::kWaves) {
-                uint32_t nhead = 0, ntail = 0;
+                uint32_t nhead[MR::kSets], ntail[MR::kSets];
+#pragma unroll
+                for (int s = 0; s < MR::kSets; ++s) nhead[s] = ntail[s] = 0;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const uint32_t* rowp = ring + ((4 * k + r) % kRing) * R::kRowDw;
-                    const uint32_t c =
-                        nibble_counts(rowp[(wbc + 1) * kPlaneDw + wd] & (rowp[wd] ^ wflip));
-                    if (r < 2) nhead += c;
-                    else ntail += c;
-                }
-                const uint32_t se = (nhead & 0x0F0F0F0Fu) + (ntail & 0x0F0F0F0Fu);
-                const uint32_t so = ((nhead >> 4) & 0x0F0F0F0Fu) + ((ntail >> 4) & 0x0F0F0F0Fu);
-                we[0] += se - he[0][0];
-                wo[0] += so - ho[0][0];
+                    const uint32_t sel = rowp[wd] ^ wflip;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    he[0][j] = he[0][j + 1];
-                    ho[0][j] = ho[0][j + 1];
+                    for (int s = 0; s < MR::kSets; ++s) {
+                        const int plane = s == 0 ? wbc + 1 : mplane1;
+                        const uint32_t c = nibble_counts(rowp[plane * kPlaneDw + wd] & sel);
+                        if (r < 2) nhead[s] += c;
+                        else ntail[s] += c;
+                    }
                 }
-                he[0][4] = se;
-                ho[0][4] = so;
-                const uint32_t old = hq[0][0];
 #pragma unroll
-                for (int j = 0; j < 3; ++j) hq[0][j] = hq[0][j + 1];
-                hq[0][3] = nhead;
-                const uint32_t pe = (ntail & 0x0F0F0F0Fu) + (old & 0x0F0F0F0Fu);
-                const uint32_t po = ((ntail >> 4) & 0x0F0F0F0Fu) + ((old >> 4) & 0x0F0F0F0Fu);
-                if (m_ok) {
-                    a0 = mfma_i8x16{(int)we[0], (int)wo[0], (int)pe, (int)po};
-                    mxch[wd * 16 + mrow % MR::QN] = a0;
+                for (int s = 0; s < MR::kSets; ++s) {
+                    const uint32_t se = (nhead[s] & 0x0F0F0F0Fu) + (ntail[s] & 0x0F0F0F0Fu);
+                    const uint32_t so =
+                        ((nhead[s] >> 4) & 0x0F0F0F0Fu) + ((ntail[s] >> 4) & 0x0F0F0F0Fu);
+                    we[s] += se - he[s][0];
+                    wo[s] += so - ho[s][0];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        he[s][j] = he[s][j + 1];
+                        ho[s][j] = ho[s][j + 1];
+                    }
+                    he[s][4] = se;
+                    ho[s][4] = so;
+                    const uint32_t old = hq[s][0];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) hq[s][j] = hq[s][j + 1];
+                    hq[s][3] = nhead[s];
+                    const uint32_t pe = (ntail[s] & 0x0F0F0F0Fu) + (old & 0x0F0F0F0Fu);
+                    const uint32_t po =
+                        ((ntail[s] >> 4) & 0x0F0F0F0Fu) + ((old >> 4) & 0x0F0F0F0Fu);
+                    if (s == 0 ? m_ok : m_ok1) {
+                        a0[s] = mfma_i8x16{(int)we[s], (int)wo[s], (int)pe, (int)po};
+                        mxch[s * kXchSet + wd * 16 + mq] = a0[s];
+                    }
                 }
             }
             if constexpr (MR::kWaves == 1) {
@@ -607,21 +649,34 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
                 // the second contraction starts two plane words on; the
                 // exchange is indexed by band-phase, since with several column
                 // ranges the rows of another range hold the word
-                mfma_i8x16 a1 = {0, 0, 0, 0};
-                if (m_ok) a1 = mxch[(wd + 2) * 16 + mrow % MR::QN];
-                const mfma_i8x16 d0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, mB, mC, 0, 0, 0);
-                const mfma_i8x16 d1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, mB, mC, 0, 0, 0);
-                const uint32_t lo =
-                    (((uint32_t)d0[0] & 0x8000u) | (((uint32_t)d0[1] & 0x8000u) << 1) |
-                     (((uint32_t)d0[2] & 0x8000u) << 2) | (((uint32_t)d0[3] & 0x8000u) << 3)) >> 15;
-                const uint32_t hi =
-                    (((uint32_t)d1[0] & 0x8000u) | (((uint32_t)d1[1] & 0x8000u) << 1) |
-                     (((uint32_t)d1[2] & 0x8000u) << 2) | (((uint32_t)d1[3] & 0x8000u) << 3)) >> 11;
-                pass = (lo | hi) & mvalid;
+#pragma unroll
+                for (int s = 0; s < MR::kSets; ++s) {
+                    mfma_i8x16 a1 = {0, 0, 0, 0};
+                    if (s == 0 ? m_ok : m_ok1) a1 = mxch[s * kXchSet + (wd + 2) * 16 + mq];
+                    const mfma_i8x16 d0 =
+                        __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[s], mB, mC, 0, 0, 0);
+                    const mfma_i8x16 d1 =
+                        __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, mB, mC, 0, 0, 0);
+                    const uint32_t lo = (((uint32_t)d0[0] & 0x8000u) |
+                                         (((uint32_t)d0[1] & 0x8000u) << 1) |
+                                         (((uint32_t)d0[2] & 0x8000u) << 2) |
+                                         (((uint32_t)d0[3] & 0x8000u) << 3)) >> 15;
+                    const uint32_t hi = (((uint32_t)d1[0] & 0x8000u) |
+                                         (((uint32_t)d1[1] & 0x8000u) << 1) |
+                                         (((uint32_t)d1[2] & 0x8000u) << 2) |
+                                         (((uint32_t)d1[3] & 0x8000u) << 3)) >> 11;
+                    pass |= (lo | hi) << (8 * s);
+                }
+                pass &= mvalid;
                 if (m < 0 || s_g >= p.s_end) pass = 0;
                 if (pass) {
-                    if ((int64_t)mband[0] * kPeriod + s_g >= p.h_valid) pass &= ~0x33u;
-                    if ((int64_t)mband[1] * kPeriod + s_g >= p.h_valid) pass &= ~0xCCu;
+#pragma unroll
+                    for (int s = 0; s < MR::kSets; ++s) {
+                        if ((int64_t)mband[s][0] * kPeriod + s_g >= p.h_valid)
+                            pass &= ~(0x33u << (8 * s));
+                        if ((int64_t)mband[s][1] * kPeriod + s_g >= p.h_valid)
+                            pass &= ~(0xCCu << (8 * s));
+                    }
                 }
             }
         } else if constexpr (kWord) {
@@ -767,11 +822,12 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
         }
         // ---- refinement by the whole wave ----------------------------------
         auto code_of = [&](int q) {
-            if constexpr (kMfma) {  // bit 4 f + i: D register i of contraction f
+            if constexpr (kMfma) {  // bit 8 s + 4 f + i: D register i of set s, contraction f
                 const int md = 4 * (lane >> 4) + (q & 3);
-                const int qd = md % MR::QN, hd = md / MR::QN;
+                const int qd = MR::kSets == 2 ? 16 * (q >> 3) + md : md % MR::QN;
+                const int hd = MR::kSets == 2 ? 0 : md / MR::QN;
                 const int g =
-                    8 * (hd * MR::kRangeWords + 4 * wave) + 16 * (q >> 2) + (lane & 15);
+                    8 * (hd * MR::kRangeWords + 4 * wave) + 16 * ((q >> 2) & 1) + (lane & 15);
                 return group_code(4 * g, qd >> 1, qd & 1);
             } else if constexpr (kWord)  // bit 8 ph + i: group 8 wd + i, band wb, phase wph + ph
                 return group_code(4 * (8 * wd + (q & 7)), wb, wph + (q >> 3));

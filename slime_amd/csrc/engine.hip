@@ -46,14 +46,14 @@ constexpr int64_t kPlainCentersPerLaunch = int64_t(1) << 37;
 constexpr int64_t kBandCentersPerLaunch = int64_t(1) << 39;
 constexpr int kBandTileH = 2048;
 constexpr int kMinBandTileH = 64;
-// Bands per banded launch (kernel instantiations: 1, 2, 4, ..., 16; up to 14
-// fit three waves per SIMD without spills); the
-// SLIME_MAX_BANDS environment variable overrides it, like the reference's
-// SLIME_GPU_V1_* path switches (SlimeCoreGPU.cu:3267-3354).
+// Bands per banded launch at most (kernel instantiations 1..14, all at three
+// or more waves per SIMD without spills); the SLIME_MAX_BANDS environment
+// variable lowers it, like the reference's SLIME_GPU_V1_* path switches
+// (SlimeCoreGPU.cu:3267-3354).
 int g_max_bands = [] {
     const char* v = std::getenv("SLIME_MAX_BANDS");
     const int n = v ? std::atoi(v) : 0;
-    return n >= 1 && n <= kMaxBands ? n : 12;
+    return n >= 1 && n <= kMaxBands ? n : kMaxBands;
 }();
 
 // Host array of center fixes, in pinned memory where it can be had (the copies
@@ -420,7 +420,7 @@ struct BandCost {
 struct BandCosts {
     int device = -1;
     bool ready = false, failed = false;
-    BandCost c[3][kUnionBoundMaxBands + 1];  // [x-phases][bands]
+    BandCost c[3][kUnionBoundMaxBands2 + 1];  // [x-phases][bands]
 };
 BandCosts g_costs;
 
@@ -470,7 +470,7 @@ bool calibrate_bands() {
     g_costs.failed = true;
     const int64_t slots = 3 * (int64_t)std::max(1, g_buf.cus);  // workgroups resident at 3 per CU
     for (int ph = 1; ph <= 2; ++ph)
-        for (int nb = 1; nb <= kUnionBoundMaxBands; ++nb) {
+        for (int nb = 1; nb <= union_bound_max_bands(ph); ++nb) {
             double warm = 0, a[2] = {0, 0};
             const int32_t ths[2] = {64, 1024};
             if (!time_band_launch(nb, ph, ths[0], slots, warm) ||
@@ -533,15 +533,17 @@ double cheapest_groups(const std::vector<XPart>& parts, int64_t n, int max_bands
 
 // Banded launches over every center of the spec: two x-phases above P
 // columns, one below (x_parts); bands grouped evenly by at most g_max_bands
-// per launch, and at most kUnionBoundMaxBands where the union bound applies
-// (its kernels exist up to that many bands; at thresholds near 50 it beats
-// the square bound on more bands); for a large search, the grouping of the
-// measured cost model above instead.
+// per launch, and at most union_bound_max_bands(ph) where the union bound
+// applies (its kernels exist up to that many bands; at thresholds near 50 it
+// beats the square bound on more bands); for a large search, the grouping of
+// the measured cost model above instead.
 void plan_banded(const SearchSpec& spec, std::deque<Work>& out) {
-    const std::vector<XPart> parts = x_parts(spec, spec.w > kPeriod ? 2 : 1);
+    const int ph = spec.w > kPeriod ? 2 : 1;
+    const std::vector<XPart> parts = x_parts(spec, ph);
     const int64_t nbz = bands_of(spec);
     const bool union_bound = union_bound_for(spec.mode, spec.min_size);
-    const int max_bands = union_bound ? std::min(g_max_bands, kUnionBoundMaxBands) : g_max_bands;
+    const int max_bands =
+        union_bound ? std::min(g_max_bands, union_bound_max_bands(ph)) : g_max_bands;
     const int64_t ngroups = (nbz + max_bands - 1) / max_bands;
     std::vector<int> groups;
     for (int64_t gi = 0, left = nbz; gi < ngroups; ++gi) {

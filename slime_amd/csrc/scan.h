@@ -80,15 +80,21 @@ struct BandParams {
     unsigned long long* hist;
 };
 // One workgroup per tile; nb bands (1..kMaxBands), ph x-phases (1 or 2);
-// union_bound selects the rolling union-of-disks bound (for nb <=
-// kUnionBoundMaxBands; the square bound otherwise).
+// union_bound selects the rolling union-of-disks bound (for nb up to
+// union_bound_max_bands(ph); the square bound otherwise).
 hipError_t launch_scan_band(const BandParams& p, int nb, int ph, bool union_bound,
                             hipStream_t stream);
-// The union-bound kernels exist up to 8 bands: there a thread counts one
-// phase of one band, so all four waves share the words; with more bands a
-// thread takes both phases and the full world's 10-band launches ran slower
-// (1.73 s) than 8-band ones (1.56 s).
+// The union-bound kernels exist up to 8 bands, and with two x-phases up to
+// 14: counting on the VALU, a thread counts one phase of one band up to 8
+// bands, so all four waves share the words (with more, a thread took both
+// phases and the full world's 10-band launches ran slower, 1.73 s, than 8-band
+// ones, 1.56 s); with the window sums on the matrix cores, a lane holds one
+// band-phase row of each of two 16-row operand sets above 8 bands.
 constexpr int kUnionBoundMaxBands = 8;
+constexpr int kUnionBoundMaxBands2 = 14;
+constexpr int union_bound_max_bands(int ph) {
+    return ph == 2 ? kUnionBoundMaxBands2 : kUnionBoundMaxBands;
+}
 
 // ---- multi-search sweep -----------------------------------------------------
 //
