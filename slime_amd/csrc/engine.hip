@@ -559,28 +559,54 @@ void plan_banded(const SearchSpec& spec, std::deque<Work>& out) {
     const bool tune = union_bound && spec.w * spec.h >= kTuneMinCenters && band_tune_enabled() &&
                       calibrate_bands();
     if (tune) {
-        // (a) every band over slot_rows in the cheapest groups; (b) the full
-        // bands so and the last, partial band alone over its own rows
-        std::vector<int> ga, gb;
+        // (a) every band over slot_rows in the cheapest groups. With a last,
+        // partial band of rem rows after nf full ones, for m = 0..nf: (b_m)
+        // the first nf - m bands over all P rows; the last m full bands and
+        // the partial one over rows [0, rem); those m bands alone over rows
+        // [rem, P) (m = 0: the partial band as a short group of its own). No
+        // launch then scans the partial band's missing rows, and every group
+        // can keep a high band count.
+        std::vector<int> ga;
         const double ca = cheapest_groups(parts, nbz, max_bands, slot_rows, tile_h, ga);
         const int64_t nf = spec.h / kPeriod, rem = spec.h % kPeriod;
-        double cb = -1;
-        int32_t th_rem = 0;
+        double best = ca;
+        int64_t best_m = -1;
+        std::vector<int> g_head, g_low, g_high;
+        int32_t th_low = 0, th_high = 0;
         if (spec.band_rows == 0 && nf >= 1 && rem > 0) {
-            th_rem = band_tile_h((int32_t)rem, col_tiles);
-            cb = cheapest_groups(parts, nf, max_bands, kPeriod, tile_h, gb);
-            cb += group_ns(parts, 1, (int32_t)rem, th_rem);
-        }
-        if (cb >= 0 && cb < ca) {
-            int64_t b_first = 0;
-            for (const int nb : gb) {
-                for (const XPart& xp : parts)
-                    band_jobs_part(spec, xp, b_first, nb, kPeriod, tile_h, kBandCentersPerLaunch,
-                                   jobs);
-                b_first += nb;
+            const int32_t thl = band_tile_h((int32_t)rem, col_tiles);
+            const int32_t thh = band_tile_h((int32_t)(kPeriod - rem), col_tiles);
+            for (int64_t m = 0; m <= nf; ++m) {
+                std::vector<int> h, l, u;
+                double c = cheapest_groups(parts, nf - m, max_bands, kPeriod, tile_h, h) +
+                           cheapest_groups(parts, m + 1, max_bands, (int32_t)rem, thl, l);
+                if (m > 0)
+                    c += cheapest_groups(parts, m, max_bands, (int32_t)(kPeriod - rem), thh, u);
+                if (c < best) {
+                    best = c;
+                    best_m = m;
+                    g_head = h;
+                    g_low = l;
+                    g_high = u;
+                    th_low = thl;
+                    th_high = thh;
+                }
             }
-            for (const XPart& xp : parts)
-                band_jobs_part(spec, xp, nf, 1, (int32_t)rem, th_rem, kBandCentersPerLaunch, jobs);
+        }
+        if (best_m >= 0) {
+            auto emit = [&](const std::vector<int>& sizes, int64_t b_first, int32_t s_lo,
+                            int32_t s_hi, int32_t th) {
+                for (const int nb : sizes) {
+                    for (const XPart& xp : parts)
+                        band_jobs_part(spec, xp, b_first, nb, s_hi, th, kBandCentersPerLaunch,
+                                       jobs, s_lo);
+                    b_first += nb;
+                }
+            };
+            const int64_t b_tail = nf - best_m;
+            emit(g_head, 0, 0, kPeriod, tile_h);
+            emit(g_low, b_tail, 0, (int32_t)rem, th_low);
+            emit(g_high, b_tail, (int32_t)rem, kPeriod, th_high);
         } else {
             band_jobs(spec, parts, ga, tile_h, kBandCentersPerLaunch, jobs);
         }

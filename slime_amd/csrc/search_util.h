@@ -166,15 +166,15 @@ struct BandJob {
 };
 
 // Jobs of one x-part and one group of nb consecutive bands from band b_first,
-// each over slot rows [0, s_rows), cut into slot-row ranges of about
+// each over slot rows [s_lo, s_rows), cut into slot-row ranges of about
 // max_centers centers. Output pointers are left null.
 inline void band_jobs_part(const SearchSpec& spec, const XPart& xp, int64_t b_first, int nb,
                            int32_t s_rows, int32_t tile_h, int64_t max_centers,
-                           std::vector<BandJob>& out) {
+                           std::vector<BandJob>& out, int32_t s_lo = 0) {
     const int32_t tpb = xp.tiles_per_block();
     int64_t rows = max_centers / std::max<int64_t>(1, nb * xp.centers_per_row);
     rows = std::max<int64_t>(tile_h, rows / tile_h * tile_h);
-    for (int64_t s = 0; s < s_rows; s += rows) {
+    for (int64_t s = s_lo; s < s_rows; s += rows) {
         BandJob j{};
         j.nb = nb;
         j.ph = xp.ph;
