@@ -60,13 +60,13 @@ VALU_PEAK_T = 157.3 / 2
 # VALU lane instructions per center of the full-world search's banded kernels,
 # counted by rocprofv3 (SQ_INSTS_VALU x 64 / centers over all launches;
 # profiles/scan_band_kernel_full_world_thr60.md).
-VALU_INSTS_PER_CENTER = 5.03
+VALU_INSTS_PER_CENTER = 4.61
 CPU_SAMPLE_RADIUS = 131072
 # HBM bytes per scan_band_kernel launch (rocprofv3 FETCH_SIZE + WRITE_SIZE of
 # the full-world run, scaled by the same counters on a copy of known size;
-# profiles/scan_band_kernel_full_world_thr60.md): 79.9 KiB x 2.0 + 0.02 KiB x 1.0
-# per launch, averaged over the search's 33 launches.
-TRAFFIC_BYTES = 163700
+# profiles/scan_band_kernel_full_world_thr60.md): 105.0 KiB x 2.0 + 0.02 KiB x 1.0
+# per launch, averaged over the search's 31 launches.
+TRAFFIC_BYTES = 215000
 
 
 def host_cpu():

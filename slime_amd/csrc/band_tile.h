@@ -42,15 +42,16 @@
 // Java's rejected first draws are not handled here; the engine recounts the
 // few affected centers.
 //
-// With the union bound, two x-phases and 2, 7 or 8 bands (the full world's
+// With the union bound, two x-phases and 2 or 7-14 bands (the full world's
 // launches and the commonest batched class), the 5-column window sums less
 // the corner cells are int8 contractions on the matrix cores instead: the
 // 20-row counts and corner counts of 16 band-phase rows x 32 nibble columns
 // per wave are the A operand of v_mfma_i32_16x16x64_i8, a constant 0/+1/-1
 // window matrix B, the threshold bias C; two instructions per wave and step
 // give 16 rows x 32 groups, the second one with the counts of the next two
-// plane words through LDS. Build with SLIME_BAND_MFMA=0 to count those
-// windows on the VALU.
+// plane words through LDS; above 8 bands a second 16-row operand set takes
+// bands 8 and up. Build with SLIME_BAND_MFMA=0 to count those windows on the
+// VALU.
 #pragma once
 
 #include <type_traits>
