@@ -173,7 +173,10 @@ int64_t slime_scan_rect(int64_t seed, int32_t x0, int32_t z0, int64_t w, int64_t
  * 1 plain per-chunk scan, 2 banded scan (one x-phase up to 2^17 columns,
  * two above), 3 banded with the square bound before refinement, 4 banded
  * with the union bound (the square minus cells outside the union of disks;
- * modes 0 and 2 take it). Lets tests and callers pick the kernel explicitly. */
+ * modes 0 and 2 take it), 5 as 4 with three x-phases (x, x + 2^17,
+ * x + 2^18) over the columns past the last whole 2^18-column block of a
+ * region wider than 2^18 (at least two bands). Lets tests and callers pick
+ * the kernel explicitly. */
 int64_t slime_scan_bands(int64_t seed, int32_t x0, int32_t z0, int64_t w, int64_t h,
                          int32_t band_rows, int32_t mode, int32_t min_size, int32_t max_size,
                          ExtChunkResult* results_buffer, int32_t max_results, int64_t* hist222);
@@ -181,8 +184,9 @@ int64_t slime_scan_bands(int64_t seed, int32_t x0, int32_t z0, int64_t w, int64_
 /* Extension with no counterpart in the reference: device time of the scan
  * kernel launches since the last reset, measured with HIP events on the
  * engine's stream, for one kernel kind (0 plain per-chunk scan, 1 banded
- * scan, 2 both), with the number of candidate centers those launches decided.
- * reset != 0 zeroes every counter after the read. */
+ * scan, 2 both, 3 the banded launches with three x-phases, a part of 1),
+ * with the number of candidate centers those launches decided. reset != 0
+ * zeroes every counter after the read. */
 void slime_kernel_stats(int32_t kind, int64_t* launches, int64_t* kernel_ns, int64_t* centers,
                         int32_t reset);
 
@@ -254,6 +258,12 @@ int32_t slime_search_batch(int32_t n, const int64_t* seeds, const int64_t* rd_ma
 void slime_batch_stats(int64_t* centers, int64_t* kernel_ns, int64_t* prepare_ns,
                        int64_t* sweep_ns, int64_t* wall_ns, int32_t* launches, int32_t* jobs,
                        int32_t* fallbacks, int64_t* hits_stored, int64_t* hits_copied);
+
+/* Extension with no counterpart in the reference: the centers the last
+ * slime_search_batch call decided per kernel class (the order of
+ * slime_sweep_costs). Writes at most cap entries; returns how many, or -1 for
+ * a null buffer. */
+int32_t slime_batch_class_centers(int64_t* centers, int32_t cap);
 
 /* Extension: the autotuner's measured cost table, one entry per kernel class
  * (bands nb, x-phases ph): device ns per tile row with the whole GPU on that

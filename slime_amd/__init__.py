@@ -36,7 +36,7 @@ EXPORTS = (
     "slime_session_release", "slime_session_stats", "score_candidates_precise",
     "refine_candidates_y", "slime_refine_stats", "refine_candidates_y_multi",
     "slime_valu_rate", "slime_session_create_ex", "refine_candidates_y_noalloc",
-    "slime_window_ab",
+    "slime_window_ab", "slime_batch_class_centers",
 )
 
 # Every symbol include/mcgpu.h declares (the biome filter).
@@ -142,6 +142,8 @@ def lib():
     L.slime_batch_stats.restype = None
     L.slime_sweep_costs.argtypes = [PD, PD, PI32, PI32, i32]
     L.slime_sweep_costs.restype = i32
+    L.slime_batch_class_centers.argtypes = [PI64, i32]
+    L.slime_batch_class_centers.restype = i32
     VP = C.c_void_p
     L.slime_session_create.argtypes = [i64, i32, i32]
     L.slime_session_create.restype = VP
@@ -313,7 +315,7 @@ def device_info():
     return cu.value, clk.value, mem.value
 
 
-KERNEL_PLAIN, KERNEL_BANDED, KERNEL_ALL = 0, 1, 2
+KERNEL_PLAIN, KERNEL_BANDED, KERNEL_ALL, KERNEL_BANDED_THREE = 0, 1, 2, 3
 
 
 VALU_KINDS = ("v_add_u32", "v_xor_b32", "v_mul_lo_u32", "v_bcnt_u32_b32", "v_mad_u32_u24",
@@ -351,6 +353,7 @@ def kernel_stats(kind=KERNEL_ALL, reset=False):
 
 
 MODE_AUTO, MODE_PLAIN, MODE_BANDED, MODE_BANDED_SQUARE, MODE_BANDED_UNION = 0, 1, 2, 3, 4
+MODE_BANDED_UNION3 = 5  # the union bound with three x-phases past the whole 2^18 blocks
 
 
 def scan_bands(seed, x0, z0, w, h, min_size, max_size=221, band_rows=0, mode=MODE_AUTO,
@@ -430,6 +433,17 @@ def sweep_costs():
     if n < 0:
         raise RuntimeError("slime_sweep_costs failed")
     return [(nb[i], ph[i], a[i], b[i]) for i in range(n)]
+
+
+def batch_class_centers():
+    """Centers the last search_batch decided per kernel class, keyed (nb, ph)."""
+    cap = 32
+    out = (C.c_int64 * cap)()
+    n = lib().slime_batch_class_centers(out, cap)
+    if n < 0:
+        raise RuntimeError("slime_batch_class_centers failed")
+    classes = [(c[0], c[1]) for c in sweep_costs()]
+    return {classes[i]: out[i] for i in range(min(n, len(classes))) if out[i]}
 
 
 QUEUED, RUNNING, PAUSED, DONE, CANCELLED, FAILED = range(6)

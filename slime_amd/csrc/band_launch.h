@@ -10,7 +10,7 @@ namespace band {
 // stream of the generation with the LDS waits of the counting.
 template <int NB, int PH, bool UB>
 __global__ __launch_bounds__(kThreads, 3) void scan_band_kernel(BandParams p) {
-    __shared__ TileLds<NB> lds;
+    __shared__ TileLds<NB, PH> lds;
     for (int i = threadIdx.x; i < kHistBins; i += kThreads) lds.lhist[i] = 0;
     band_tile<NB, PH, UB>(p, p.emit_min, blockIdx.x, lds, (int)threadIdx.x);
     flush_hist(lds.lhist, p.hist);
@@ -48,6 +48,15 @@ hipError_t launch_phases(const BandParams& p, int nb, bool ub, hipStream_t strea
         default: return hipErrorInvalidValue;
     }
 }
+
+// Three x-phases: 2..5 bands, union bound only (scan_band3.hip).
+template <int NB>
+hipError_t launch_three(const BandParams& p, hipStream_t stream) {
+    hipLaunchKernelGGL((scan_band_kernel<NB, 3, true>), dim3(p.tiles_total), dim3(kThreads), 0,
+                       stream, p);
+    return hipGetLastError();
+}
+hipError_t launch_phases3(const BandParams& p, int nb, bool ub, hipStream_t stream);
 
 // Instantiated in scan_band.hip (two phases) and scan_band1.hip (one phase).
 extern template hipError_t launch_phases<1>(const BandParams&, int, bool, hipStream_t);

@@ -114,17 +114,29 @@ __device__ __forceinline__ void shift_in_above(uint32_t& acc, uint32_t v, uint32
         : "vcc");
 }
 
-template <int NB>
+// Bitplanes of a ring row: E, one plane per band for x-phases 0 and 1 (with
+// one phase, the band's own test), and with three phases one plane per band
+// pair (2k, 2k + 1) for phase 2 (the last band alone when NB is odd).
+template <int NB, int PH>
 struct Ring {
-    static constexpr int kPlanes = NB + 1;
+    static constexpr int kPlanes = 1 + NB + (PH == 3 ? (NB + 1) / 2 : 0);
     static constexpr int kRowDw = kPlanes * kPlaneDw + 2;  // + pad for window reads
     static constexpr int kDwords = kRing * kRowDw;
 };
 
+// The plane of band b, phase ph: phases 0 and 1 share their band's plane (E
+// tells which of them is the even one), phase 2 shares a plane per band pair
+// (E tells which band). Either way band b, phase ph is the plane's bitmap
+// under E where (b ^ ph) is odd and under ~E where it is even.
+template <int NB, int PH>
+__host__ __device__ constexpr int plane_of(int b, int ph) {
+    return PH < 3 || ph < 2 ? 1 + b : 1 + NB + (b >> 1);
+}
+
 // LDS of one workgroup: the bitplane ring and the score histogram.
-template <int NB>
+template <int NB, int PH>
 struct TileLds {
-    uint32_t ring[Ring<NB>::kDwords];
+    uint32_t ring[Ring<NB, PH>::kDwords];
     uint32_t lhist[kHistBins];
 };
 
@@ -188,9 +200,9 @@ static_assert(corner_mask(6) == 0 && corner_mask(13) == 0 && corner_mask(5) != 0
               "corner rows");
 
 // A group of one band-phase, as a lane hands it to its wave:
-// (gbit << 8) | (b << 1) | ph, gbit = 4 * group.
+// (gbit << 8) | (b << 2) | ph, gbit = 4 * group.
 __device__ __forceinline__ uint32_t group_code(int gbit, int b, int ph) {
-    return ((uint32_t)gbit << 8) | ((uint32_t)b << 1) | (uint32_t)ph;
+    return ((uint32_t)gbit << 8) | ((uint32_t)b << 2) | (uint32_t)ph;
 }
 
 typedef int mfma_i8x16 __attribute__((ext_vector_type(4)));  // 16 int8 / 4 int32
@@ -242,14 +254,17 @@ __device__ __forceinline__ mfma_i8x16 window_matrix(int lane) {
 
 // Row r (0..19) of the 20-bit window of a coded group: slime chunks of its
 // band-phase starting at group row `first` of the ring.
-template <int NB>
+template <int NB, int PH>
 __device__ __forceinline__ uint32_t group_row(const uint32_t* ring, int first, uint32_t code,
                                               int r) {
-    const int gbit = (int)(code >> 8), b = (int)((code >> 1) & 15u), ph = (int)(code & 1u);
+    const int gbit = (int)(code >> 8), b = (int)((code >> 2) & 15u);
+    const int ph = (int)(code & (PH == 3 ? 3u : 1u));
     const uint32_t flip = ((b ^ ph) & 1) ? 0u : 0xFFFFFFFFu;  // phase mask is E or ~E
     // first >= 0 wherever a group is refined: an unsigned remainder is cheaper
-    const uint32_t* row = ring + ((uint32_t)(first + r) % (uint32_t)kRing) * Ring<NB>::kRowDw;
-    return window20(row + (b + 1) * kPlaneDw, gbit) & (window20(row, gbit) ^ flip);
+    const uint32_t* row =
+        ring + ((uint32_t)(first + r) % (uint32_t)kRing) * Ring<NB, PH>::kRowDw;
+    return window20(row + plane_of<NB, PH>(b, ph) * kPlaneDw, gbit) &
+           (window20(row, gbit) ^ flip);
 }
 
 // The wave refines the groups its lanes found (todo: a bit per owned
@@ -264,11 +279,11 @@ __device__ __forceinline__ uint32_t group_row(const uint32_t* ring, int first, u
 // wave: lane r < 20 reads window row r once; lane 4c + part takes the disk
 // rows dz = -8 + part + 4t of center c from those lanes (ds_bpermute), a quad
 // reduction gives the count, and lane 4c emits it.
-template <int NB, typename EmitCenter>
+template <int NB, int PH, typename EmitCenter>
 __device__ __forceinline__ void exact_group(uint32_t code, const uint32_t* ring, int first,
                                             const EmitCenter& emit) {
     const int lane = threadIdx.x & 63;
-    const uint32_t row = group_row<NB>(ring, first, code, lane < 20 ? lane : 19);
+    const uint32_t row = group_row<NB, PH>(ring, first, code, lane < 20 ? lane : 19);
     const int center = lane >> 2, part = lane & 3;
     const int i = center & 3, jr = center >> 2;
     uint32_t cnt = 0;
@@ -289,7 +304,7 @@ __device__ __forceinline__ void exact_group(uint32_t code, const uint32_t* ring,
     if (part == 0) emit(code, i, jr, cnt);
 }
 
-template <int NB, bool UB, typename CodeOf, typename EmitCenter>
+template <int NB, int PH, bool UB, typename CodeOf, typename EmitCenter>
 __device__ __forceinline__ void refine_wave(uint32_t todo, const CodeOf& code_of,
                                             const uint32_t* ring, int first, int32_t min_size,
                                             const EmitCenter& emit) {
@@ -302,7 +317,7 @@ __device__ __forceinline__ void refine_wave(uint32_t todo, const CodeOf& code_of
             const uint32_t mine = todo ? code_of(__builtin_ctz(todo)) : 0u;
             const uint32_t code = __builtin_amdgcn_readlane(mine, l);
             if ((threadIdx.x & 63) == l) todo &= todo - 1;
-            exact_group<NB>(code, ring, first, emit);
+            exact_group<NB, PH>(code, ring, first, emit);
         }
         return;
     }
@@ -329,7 +344,8 @@ __device__ __forceinline__ void refine_wave(uint32_t todo, const CodeOf& code_of
         if ((taken >> lane) & 1ull) todo &= todo - 1;
         const uint32_t my_code = slot == 0 ? code[0] : slot == 1 ? code[1] : code[2];
         uint32_t c = 0;
-        if (slot < nsel) c = __popc(group_row<NB>(ring, first, my_code, r) & union_row_mask(r));
+        if (slot < nsel)
+            c = __popc(group_row<NB, PH>(ring, first, my_code, r) & union_row_mask(r));
         uint64_t bits[5];
 #pragma unroll
         for (int j = 0; j < 5; ++j) bits[j] = __ballot((c >> j) & 1u);
@@ -339,7 +355,7 @@ __device__ __forceinline__ void refine_wave(uint32_t todo, const CodeOf& code_of
 #pragma unroll
             for (int j = 0; j < 5; ++j) sum += (uint32_t)__popcll(bits[j] & m) << j;
             if (sum < (uint32_t)min_size) continue;
-            exact_group<NB>(code[s], ring, first, emit);
+            exact_group<NB, PH>(code[s], ring, first, emit);
         }
     }
 }
@@ -359,9 +375,12 @@ __device__ __forceinline__ void refine_wave(uint32_t todo, const CodeOf& code_of
 // constants are computed per tile rather than kept live across the loop).
 template <int NB, int PH, bool UB>
 __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min, int tile,
-                                          TileLds<NB>& lds, int tid) {
-    static_assert(PH == 1 || PH == 2, "one or two x-phases");
-    using R = Ring<NB>;
+                                          TileLds<NB, PH>& lds, int tid) {
+    static_assert(PH >= 1 && PH <= 3, "one to three x-phases");
+    // three phases: 2..5 bands, the union bound, counting by plane words
+    // with a thread group per band-phase
+    static_assert(PH < 3 || (NB >= 2 && NB <= 5 && UB), "three-phase classes");
+    using R = Ring<NB, PH>;
     using C = Roles<NB, PH>;
     uint32_t* ring = lds.ring;
     uint32_t* lhist = lds.lhist;
@@ -376,21 +395,27 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
     const int ts = tile / p.tiles_x;
     const int xblock = tx / p.tiles_per_block;
     const int xtile = tx % p.tiles_per_block;
-    const int64_t block_off = (int64_t)xblock * 2 * kPeriod;  // center offset of phase 0
+    // center offset of phase 0 (x-blocks of PH P columns; one block with PH = 1)
+    const int64_t block_off = (int64_t)xblock * (PH == 1 ? 2 : PH) * kPeriod;
     const int32_t col0 = p.slot_lo + xtile * kBandTileOut;      // slot column offset in phase
     const uint32_t chunk_x0 = (uint32_t)p.x0 + (uint32_t)(block_off + col0) - 8u;
 
     // Lane l evaluates the chunks kChunksPerLane*l + j of every row.
-    uint32_t xlo[kChunksPerLane], xh0[kChunksPerLane], xh1[PH == 2 ? kChunksPerLane : 1];
+    uint32_t xlo[kChunksPerLane], xh0[kChunksPerLane], xh1[PH >= 2 ? kChunksPerLane : 1];
+    uint32_t xh2[PH == 3 ? kChunksPerLane : 1];
 #pragma unroll
     for (int j = 0; j < kChunksPerLane; ++j) {
         const uint32_t xc = chunk_x0 + (uint32_t)(kChunksPerLane * lane + j);
         const uint64_t X0 = (uint64_t)p.seed + x_term((int32_t)xc);
         xlo[j] = (uint32_t)X0 << 16;
         xh0[j] = (uint32_t)(X0 >> 16);
-        if constexpr (PH == 2) {
+        if constexpr (PH >= 2) {
             const uint64_t X1 = (uint64_t)p.seed + x_term((int32_t)(xc + (uint32_t)kPeriod));
             xh1[j] = (uint32_t)(X1 >> 16);
+        }
+        if constexpr (PH == 3) {
+            const uint64_t X2 = (uint64_t)p.seed + x_term((int32_t)(xc + 2u * (uint32_t)kPeriod));
+            xh2[j] = (uint32_t)(X2 >> 16);
         }
     }
 
@@ -430,8 +455,10 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
     // in window rows 0, 1, 18 and 19 (32 cells, all outside the union; 368
     // remain). Groups that pass it are refined as with the square bound.
     constexpr bool kCorner = UB && kWord;
-    constexpr bool kWSplit = PH == 2 && NB <= 8;
-    constexpr int kWThreads = 16 * NB * (kWSplit ? 2 : 1);  // threads that hold a band
+    // a thread group of 16 per band-phase: two phases and up to 8 bands, or
+    // three phases
+    constexpr bool kWSplit = (PH == 2 && NB <= 8) || PH == 3;
+    constexpr int kWThreads = 16 * NB * (kWSplit ? PH : 1);  // threads that hold a band
     // Matrix-core window sums (MfmaRoles): the lane's plane word, band and
     // phase follow its row of A instead.
     // (measured per class: faster with 2 and 7-8 bands, and the only
@@ -445,8 +472,16 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
     const int wd = !kMfma ? (tid & 15)
                  : m_ok   ? (mrow / MR::QN) * MR::kRangeWords + 4 * wave + (lane >> 4)
                           : 0;
-    const int wb = !kMfma ? (kWSplit ? tid >> 5 : tid >> 4) : m_ok ? (mrow % MR::QN) >> 1 : NB;
-    const int wph = !kMfma ? (kWSplit ? (tid >> 4) & 1 : 0) : (mrow % MR::QN) & 1;
+    // three phases: band-phase t / 16 is (band, phase 0 or 1) below 2 NB,
+    // then (band, phase 2)
+    const int wbp = tid >> 4;
+    const int wb = PH == 3 ? (wbp < 2 * NB ? wbp >> 1 : wbp - 2 * NB)
+                 : !kMfma  ? (kWSplit ? tid >> 5 : tid >> 4)
+                 : m_ok    ? (mrow % MR::QN) >> 1
+                           : NB;
+    const int wph = PH == 3 ? (wbp < 2 * NB ? wbp & 1 : 2)
+                  : !kMfma  ? (kWSplit ? (tid >> 4) & 1 : 0)
+                            : (mrow % MR::QN) & 1;
     const bool w_active = wb < NB;
     const int wbc = w_active ? wb : 0;  // idle threads read a valid plane
     // phase 0 of band b lies where e == (b & 1): E for odd b, ~E for even b;
@@ -577,7 +612,7 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
             static_for<0, kChunksPerLane>([&](auto jc) {
                 constexpr int j = kChunksPerLane - 1 - decltype(jc)::value;  // high bits first
                 const BandSlot sl =
-                    band_slot(xlo[j], xh0[j], PH == 2 ? xh1[PH == 2 ? j : 0] : xh0[j], zlo, zhi[0]);
+                    band_slot(xlo[j], xh0[j], PH >= 2 ? xh1[PH >= 2 ? j : 0] : xh0[j], zlo, zhi[0]);
                 acc[0] = acc[0] + acc[0] + (sl.e ? 1u : 0u);
                 // all values first, so the multiplies of independent bands overlap
                 uint32_t v[NB];
@@ -599,6 +634,24 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
                     constexpr int b = decltype(bc)::value;
                     shift_in_above(acc[b + 1], v[b], fifth_limit);
                 });
+                if constexpr (PH == 3) {
+                    // phase 2 (x + 2P) has phase 0's parity: of the band pair
+                    // (2k, 2k + 1) the even chunk is band 2k's where e == 0 and
+                    // band 2k + 1's where e == 1 (an odd NB's last band is
+                    // tested alone; the plane's mask keeps it where it is even)
+                    constexpr int kPairs = (NB + 1) / 2;
+                    uint32_t v2[kPairs];
+                    static_for<0, kPairs>([&](auto kc) {
+                        constexpr int k = decltype(kc)::value;
+                        uint32_t zs = zhi[2 * k];
+                        if constexpr (2 * k + 1 < NB) zs = sl.e ? zhi[2 * k + 1] : zhi[2 * k];
+                        v2[k] = ((xh2[j] + zs + sl.carry) ^ kHighScramble) * kMulInv5 + sl.bias;
+                    });
+                    static_for<0, kPairs>([&](auto kc) {
+                        constexpr int k = decltype(kc)::value;
+                        shift_in_above(acc[1 + NB + k], v2[k], fifth_limit);
+                    });
+                }
             });
             uint8_t* dst = reinterpret_cast<uint8_t*>(ring + (row % kRing) * R::kRowDw);
             dst[lane] = (uint8_t)acc[0];
@@ -712,7 +765,7 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
                 for (int r = 0; r < 4; ++r) {
                     const uint32_t* rowp = ring + ((4 * k + r) % kRing) * R::kRowDw;
                     const uint32_t e = rowp[wd];
-                    const uint32_t h = rowp[(wbc + 1) * kPlaneDw + wd];
+                    const uint32_t h = rowp[plane_of<NB, PH>(wbc, wph) * kPlaneDw + wd];
                     uint32_t c[kWPh];
                     c[0] = nibble_counts(h & (e ^ wflip));
                     if constexpr (kWPh == 2) c[1] = nibble_counts(h) - c[0];  // the rest
@@ -859,7 +912,8 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
                 return group_code(gbit, b_lo + q / C::kPhases, ph_lo + q % C::kPhases);
         };
         auto emit = [&](uint32_t code, int i, int jr, uint32_t c) {
-            const int gb = (int)(code >> 8), b = (int)((code >> 1) & 15u), ph = (int)(code & 1u);
+            const int gb = (int)(code >> 8), b = (int)((code >> 2) & 15u);
+            const int ph = (int)(code & (PH == 3 ? 3u : 1u));
             const int64_t ox = block_off + (int64_t)ph * kPeriod + col0 + gb + i;
             const int32_t s_c = s_g + jr;
             if (c < (uint32_t)p.min_size || c > (uint32_t)p.max_size) return;
@@ -881,7 +935,7 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
         // a wave-uniform test first: the refinement's per-lane row addresses are
         // then computed only in the (rare) steps where some lane has a group
         if (__ballot(pass != 0) != 0)
-            refine_wave<NB, UB && !kWord>(pass, code_of, ring, 4 * m, p.min_size, emit);
+            refine_wave<NB, PH, UB && !kWord>(pass, code_of, ring, 4 * m, p.min_size, emit);
     }
 }
 

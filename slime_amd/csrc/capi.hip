@@ -217,7 +217,7 @@ int64_t slime_scan_bands(int64_t seed, int32_t x0, int32_t z0, int64_t w, int64_
                          ExtChunkResult* results_buffer, int32_t max_results, int64_t* hist222) {
     if (!results_buffer || max_results <= 0 || w <= 0 || h <= 0 || w > INT32_MAX ||
         h > INT32_MAX || !slime::within_search_limit(w, h) || band_rows < 0 || mode < 0 ||
-        mode > 4 || min_size < 0 || min_size > 221 || max_size < min_size)
+        mode > 5 || min_size < 0 || min_size > 221 || max_size < min_size)
         return -1;
     slime::SearchSpec spec;
     spec.seed = seed;
@@ -445,6 +445,15 @@ void slime_batch_stats(int64_t* centers, int64_t* kernel_ns, int64_t* prepare_ns
     if (fallbacks) *fallbacks = st.fallbacks;
     if (hits_stored) *hits_stored = st.hits_stored;
     if (hits_copied) *hits_copied = st.hits_copied;
+}
+
+int32_t slime_batch_class_centers(int64_t* centers, int32_t cap) {
+    if (!centers || cap < 0) return -1;
+    slime::BatchStats st;
+    slime::last_batch_stats(&st);
+    const int32_t n = std::min<int32_t>(cap, slime::kSweepClasses);
+    for (int32_t c = 0; c < n; ++c) centers[c] = st.class_centers[c];
+    return n;
 }
 
 int32_t slime_sweep_costs(double* ns_row, double* extra_rows, int32_t* nb, int32_t* ph,

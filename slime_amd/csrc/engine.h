@@ -29,6 +29,7 @@ enum class ScanMode : int32_t {
     kBanded = 2,        // banded scan (same as kAuto)
     kBandedSquare = 3,  // banded scan, square bound before refinement
     kBandedUnion = 4,   // banded scan, union bound (corner cells off the square)
+    kBandedUnion3 = 5,  // as kBandedUnion, three x-phases wherever wider than 2^18
 };
 
 // The bound the banded scan uses when the mode does not force one: the union

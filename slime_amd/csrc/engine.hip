@@ -135,7 +135,8 @@ FixList g_fixes;  // the host fixes of the current search (guarded by g_mutex)
 Buffers g_buf;
 std::mutex g_mutex;
 // Per kernel kind (0 plain, 1 banded): launches, device ns, centers decided.
-std::atomic<int64_t> g_kstat[2][3];
+// [plain, banded, banded with three x-phases (also counted as banded)][launches, ns, centers]
+std::atomic<int64_t> g_kstat[3][3];
 
 void free_buffers() {
     if (g_buf.device >= 0) (void)hipSetDevice(g_buf.device);
@@ -296,6 +297,11 @@ RunResult run_work(Work& wk, Ranker& rk) {
             st[0].fetch_add(1, std::memory_order_relaxed);
             st[1].fetch_add((int64_t)((double)ms * 1e6), std::memory_order_relaxed);
             st[2].fetch_add(wk.centers, std::memory_order_relaxed);
+            if (wk.banded && wk.ph == 3) {
+                g_kstat[2][0].fetch_add(1, std::memory_order_relaxed);
+                g_kstat[2][1].fetch_add((int64_t)((double)ms * 1e6), std::memory_order_relaxed);
+                g_kstat[2][2].fetch_add(wk.centers, std::memory_order_relaxed);
+            }
         }
         const unsigned long long emitted = g_buf.h_counters[0];
         const unsigned long long* slab_hist = g_buf.h_counters + 1;
@@ -420,12 +426,18 @@ struct BandCost {
 struct BandCosts {
     int device = -1;
     bool ready = false, failed = false;
-    BandCost c[3][kUnionBoundMaxBands2 + 1];  // [x-phases][bands]
+    BandCost c[4][kUnionBoundMaxBands2 + 1];  // [x-phases][bands]
 };
 BandCosts g_costs;
 
 bool band_tune_enabled() {
     const char* v = std::getenv("SLIME_BAND_TUNE");
+    return !(v && std::strcmp(v, "0") == 0);
+}
+
+// SLIME_THREE_PHASE=0: the measured plan keeps the two-phase layout.
+bool three_phase_enabled() {
+    const char* v = std::getenv("SLIME_THREE_PHASE");
     return !(v && std::strcmp(v, "0") == 0);
 }
 
@@ -436,7 +448,7 @@ bool time_band_launch(int nb, int ph, int32_t tile_h, int64_t want_tiles, double
     spec.seed = 0x5EED5EED5EEDll + 17 * nb + ph;
     spec.x0 = -kPeriod;
     spec.z0 = 12345;
-    spec.w = ph == 2 ? 2 * kPeriod : kPeriod;
+    spec.w = ph * kPeriod;
     const int64_t row_tiles = std::max<int64_t>(1, (want_tiles + kTilesPerPhase - 1) / kTilesPerPhase);
     const int64_t slot_rows = std::min<int64_t>(kPeriod / tile_h * tile_h, row_tiles * tile_h);
     spec.h = (int64_t)(nb - 1) * kPeriod + slot_rows;
@@ -469,8 +481,8 @@ bool calibrate_bands() {
     g_costs.device = g_buf.device;
     g_costs.failed = true;
     const int64_t slots = 3 * (int64_t)std::max(1, g_buf.cus);  // workgroups resident at 3 per CU
-    for (int ph = 1; ph <= 2; ++ph)
-        for (int nb = 1; nb <= union_bound_max_bands(ph); ++nb) {
+    for (int ph = 1; ph <= 3; ++ph)
+        for (int nb = min_bands(ph); nb <= union_bound_max_bands(ph); ++nb) {
             double warm = 0, a[2] = {0, 0};
             const int32_t ths[2] = {64, 1024};
             if (!time_band_launch(nb, ph, ths[0], slots, warm) ||
@@ -498,10 +510,12 @@ int32_t band_tile_h(int32_t slot_rows, int64_t col_tiles) {
                                       std::max<int64_t>(kMinBandTileH, (fit_h + 3) / 4 * 4));
 }
 
-// Measured ns of a group of nb bands over s_rows slot rows of every x-part.
+// Measured ns of a group of nb bands over s_rows slot rows of every x-part
+// (negative where a part has no kernel of nb bands).
 double group_ns(const std::vector<XPart>& parts, int nb, int32_t s_rows, int32_t tile_h) {
     double t = 0;
     for (const XPart& xp : parts) {
+        if (nb < min_bands(xp.ph)) return -1.0;
         const BandCost& c = g_costs.c[xp.ph][nb];
         const int64_t tiles =
             (int64_t)xp.nblocks * xp.tiles_per_block() * ((s_rows + tile_h - 1) / tile_h);
@@ -511,7 +525,8 @@ double group_ns(const std::vector<XPart>& parts, int nb, int32_t s_rows, int32_t
 }
 
 // Cheapest split of n consecutive bands into groups of at most max_bands
-// (sizes largest first), each over s_rows slot rows; its measured ns.
+// (sizes largest first), each over s_rows slot rows; its measured ns, or -1
+// when no split exists (three phases have no 1-band kernel).
 double cheapest_groups(const std::vector<XPart>& parts, int64_t n, int max_bands, int32_t s_rows,
                        int32_t tile_h, std::vector<int>& sizes) {
     std::vector<double> cost((size_t)n + 1, -1.0);
@@ -519,98 +534,161 @@ double cheapest_groups(const std::vector<XPart>& parts, int64_t n, int max_bands
     cost[0] = 0;
     for (int64_t k = 1; k <= n; ++k)
         for (int nb = 1; nb <= max_bands && nb <= k; ++nb) {
-            const double v = cost[k - nb] + group_ns(parts, nb, s_rows, tile_h);
+            const double g = group_ns(parts, nb, s_rows, tile_h);
+            if (g < 0 || cost[k - nb] < 0) continue;
+            const double v = cost[k - nb] + g;
             if (cost[k] < 0 || v < cost[k]) {
                 cost[k] = v;
                 pick[k] = nb;
             }
         }
     sizes.clear();
+    if (cost[n] < 0) return -1.0;
     for (int64_t k = n; k > 0; k -= pick[k]) sizes.push_back(pick[k]);
     std::sort(sizes.begin(), sizes.end(), std::greater<int>());
     return cost[n];
+}
+
+// The tuned plan of one x-part (section (a)/(b_m) below), its jobs appended to
+// `jobs`; its measured ns, or -1 when the part has no such plan.
+double plan_part_tuned(const SearchSpec& spec, const XPart& xp, int max_bands, int32_t tile_h,
+                       int64_t col_tiles, std::vector<BandJob>& jobs) {
+    const std::vector<XPart> parts{xp};
+    const int32_t slot_rows = slot_rows_of(spec);
+    const int64_t nbz = bands_of(spec);
+    // (a) every band over slot_rows in the cheapest groups. With a last,
+    // partial band of rem rows after nf full ones, for m = 0..nf: (b_m) the
+    // first nf - m bands over all P rows; the last m full bands and the
+    // partial one over rows [0, rem); those m bands alone over rows [rem, P)
+    // (m = 0: the partial band as a short group of its own). No launch then
+    // scans the partial band's missing rows, and every group can keep a high
+    // band count.
+    std::vector<int> ga;
+    const double ca = cheapest_groups(parts, nbz, max_bands, slot_rows, tile_h, ga);
+    const int64_t nf = spec.h / kPeriod, rem = spec.h % kPeriod;
+    double best = ca;
+    int64_t best_m = -1;
+    std::vector<int> g_head, g_low, g_high;
+    int32_t th_low = 0, th_high = 0;
+    if (spec.band_rows == 0 && nf >= 1 && rem > 0) {
+        const int32_t thl = band_tile_h((int32_t)rem, col_tiles);
+        const int32_t thh = band_tile_h((int32_t)(kPeriod - rem), col_tiles);
+        for (int64_t m = 0; m <= nf; ++m) {
+            std::vector<int> h, l, u;
+            const double c0 = cheapest_groups(parts, nf - m, max_bands, kPeriod, tile_h, h);
+            const double c1 = cheapest_groups(parts, m + 1, max_bands, (int32_t)rem, thl, l);
+            const double c2 =
+                m > 0 ? cheapest_groups(parts, m, max_bands, (int32_t)(kPeriod - rem), thh, u) : 0;
+            if (c0 < 0 || c1 < 0 || c2 < 0) continue;
+            const double c = c0 + c1 + c2;
+            if (best < 0 || c < best) {
+                best = c;
+                best_m = m;
+                g_head = h;
+                g_low = l;
+                g_high = u;
+                th_low = thl;
+                th_high = thh;
+            }
+        }
+    }
+    if (best < 0) return -1.0;
+    if (best_m >= 0) {
+        auto emit = [&](const std::vector<int>& sizes, int64_t b_first, int32_t s_lo, int32_t s_hi,
+                        int32_t th) {
+            for (const int nb : sizes) {
+                band_jobs_part(spec, xp, b_first, nb, s_hi, th, kBandCentersPerLaunch, jobs, s_lo);
+                b_first += nb;
+            }
+        };
+        const int64_t b_tail = nf - best_m;
+        emit(g_head, 0, 0, kPeriod, tile_h);
+        emit(g_low, b_tail, 0, (int32_t)rem, th_low);
+        emit(g_high, b_tail, (int32_t)rem, kPeriod, th_high);
+    } else {
+        band_jobs(spec, parts, ga, tile_h, kBandCentersPerLaunch, jobs);
+    }
+    return best;
 }
 
 // Banded launches over every center of the spec: two x-phases above P
 // columns, one below (x_parts); bands grouped evenly by at most g_max_bands
 // per launch, and at most union_bound_max_bands(ph) where the union bound
 // applies (its kernels exist up to that many bands; at thresholds near 50 it
-// beats the square bound on more bands); for a large search, the grouping of
-// the measured cost model above instead.
+// beats the square bound on more bands). Where the measured costs are known
+// (the first search of at least kTuneMinCenters centers on a device measures
+// them), each x-part takes the cheapest grouping of the cost model instead,
+// and a region of at most 5 bands, wider than 2P and not a whole number of 2P
+// blocks wide, may take the three-phase layout (x_parts_three: no masked
+// columns) when its measured cost is lower (SLIME_THREE_PHASE=0 keeps two
+// phases). kBandedUnion3 forces that layout (tests).
 void plan_banded(const SearchSpec& spec, std::deque<Work>& out) {
     const int ph = spec.w > kPeriod ? 2 : 1;
-    const std::vector<XPart> parts = x_parts(spec, ph);
-    const int64_t nbz = bands_of(spec);
     const bool union_bound = union_bound_for(spec.mode, spec.min_size);
-    const int max_bands =
-        union_bound ? std::min(g_max_bands, union_bound_max_bands(ph)) : g_max_bands;
-    const int64_t ngroups = (nbz + max_bands - 1) / max_bands;
-    std::vector<int> groups;
-    for (int64_t gi = 0, left = nbz; gi < ngroups; ++gi) {
-        const int64_t nb = (left + (ngroups - gi) - 1) / (ngroups - gi);
-        groups.push_back((int)nb);
-        left -= nb;
-    }
+    const int64_t nbz = bands_of(spec);
+    const bool force3 = spec.mode == ScanMode::kBandedUnion3 && spec.w > 2 * kPeriod &&
+                        nbz >= kThreePhaseMinBands;
+    auto max_bands_of = [&](const std::vector<XPart>& parts) {
+        int mb = g_max_bands;
+        if (union_bound)
+            for (const XPart& xp : parts) mb = std::min(mb, union_bound_max_bands(xp.ph));
+        return mb;
+    };
+    auto col_tiles_of = [](const std::vector<XPart>& parts) {
+        int64_t t = 0;  // x-tiles of one slot row over all parts
+        for (const XPart& xp : parts) t += (int64_t)xp.nblocks * xp.tiles_per_block();
+        return t;
+    };
     const int32_t slot_rows = slot_rows_of(spec);
-    int64_t col_tiles = 0;  // x-tiles of one slot row over all parts and groups
-    for (const XPart& xp : parts) col_tiles += (int64_t)xp.nblocks * xp.tiles_per_block();
-    const int32_t tile_h = band_tile_h(slot_rows, col_tiles * ngroups);
     std::vector<BandJob> jobs;
-    const bool tune = union_bound && spec.w * spec.h >= kTuneMinCenters && band_tune_enabled() &&
-                      calibrate_bands();
+    const bool tune = union_bound && !force3 && band_tune_enabled() &&
+                      ((g_costs.device == g_buf.device && g_costs.ready) ||
+                       (spec.w * spec.h >= kTuneMinCenters && calibrate_bands()));
+    bool planned = false;
     if (tune) {
-        // (a) every band over slot_rows in the cheapest groups. With a last,
-        // partial band of rem rows after nf full ones, for m = 0..nf: (b_m)
-        // the first nf - m bands over all P rows; the last m full bands and
-        // the partial one over rows [0, rem); those m bands alone over rows
-        // [rem, P) (m = 0: the partial band as a short group of its own). No
-        // launch then scans the partial band's missing rows, and every group
-        // can keep a high band count.
-        std::vector<int> ga;
-        const double ca = cheapest_groups(parts, nbz, max_bands, slot_rows, tile_h, ga);
-        const int64_t nf = spec.h / kPeriod, rem = spec.h % kPeriod;
-        double best = ca;
-        int64_t best_m = -1;
-        std::vector<int> g_head, g_low, g_high;
-        int32_t th_low = 0, th_high = 0;
-        if (spec.band_rows == 0 && nf >= 1 && rem > 0) {
-            const int32_t thl = band_tile_h((int32_t)rem, col_tiles);
-            const int32_t thh = band_tile_h((int32_t)(kPeriod - rem), col_tiles);
-            for (int64_t m = 0; m <= nf; ++m) {
-                std::vector<int> h, l, u;
-                double c = cheapest_groups(parts, nf - m, max_bands, kPeriod, tile_h, h) +
-                           cheapest_groups(parts, m + 1, max_bands, (int32_t)rem, thl, l);
-                if (m > 0)
-                    c += cheapest_groups(parts, m, max_bands, (int32_t)(kPeriod - rem), thh, u);
-                if (c < best) {
-                    best = c;
-                    best_m = m;
-                    g_head = h;
-                    g_low = l;
-                    g_high = u;
-                    th_low = thl;
-                    th_high = thh;
+        std::vector<std::vector<XPart>> layouts{x_parts(spec, ph)};
+        // the three-phase layout for regions of at most 5 bands: with more, the
+        // two-phase parts keep wide band groups and the model's gain for the
+        // three-phase part (at most 5 bands a group) did not hold (full world:
+        // 9.86 vs 9.88 T centers/s of kernels)
+        if (spec.w > 2 * kPeriod && spec.w % (2 * kPeriod) != 0 && three_phase_enabled() &&
+            nbz <= kThreePhaseMaxBands)
+            layouts.push_back(x_parts(spec, 3));
+        double best = -1;
+        for (const std::vector<XPart>& parts : layouts) {
+            const int64_t col_tiles = col_tiles_of(parts);
+            const int64_t ngroups = (nbz + max_bands_of(parts) - 1) / max_bands_of(parts);
+            const int32_t tile_h = band_tile_h(slot_rows, col_tiles * ngroups);
+            std::vector<BandJob> lj;
+            double total = 0;
+            for (const XPart& xp : parts) {
+                const int mb = std::min(g_max_bands, union_bound_max_bands(xp.ph));
+                const double c = plan_part_tuned(spec, xp, mb, tile_h, col_tiles, lj);
+                if (c < 0) {
+                    total = -1;
+                    break;
                 }
+                total += c;
+            }
+            if (total >= 0 && (best < 0 || total < best)) {
+                best = total;
+                jobs.swap(lj);
             }
         }
-        if (best_m >= 0) {
-            auto emit = [&](const std::vector<int>& sizes, int64_t b_first, int32_t s_lo,
-                            int32_t s_hi, int32_t th) {
-                for (const int nb : sizes) {
-                    for (const XPart& xp : parts)
-                        band_jobs_part(spec, xp, b_first, nb, s_hi, th, kBandCentersPerLaunch,
-                                       jobs, s_lo);
-                    b_first += nb;
-                }
-            };
-            const int64_t b_tail = nf - best_m;
-            emit(g_head, 0, 0, kPeriod, tile_h);
-            emit(g_low, b_tail, 0, (int32_t)rem, th_low);
-            emit(g_high, b_tail, (int32_t)rem, kPeriod, th_high);
-        } else {
-            band_jobs(spec, parts, ga, tile_h, kBandCentersPerLaunch, jobs);
+        planned = best >= 0;
+    }
+    if (!planned) {
+        jobs.clear();
+        const std::vector<XPart> parts = x_parts(spec, force3 ? 3 : ph);
+        const int max_bands = max_bands_of(parts);
+        const int64_t ngroups = (nbz + max_bands - 1) / max_bands;
+        std::vector<int> groups;
+        for (int64_t gi = 0, left = nbz; gi < ngroups; ++gi) {
+            const int64_t nb = (left + (ngroups - gi) - 1) / (ngroups - gi);
+            groups.push_back((int)nb);
+            left -= nb;
         }
-    } else {
+        const int32_t tile_h = band_tile_h(slot_rows, col_tiles_of(parts) * ngroups);
         band_jobs(spec, parts, groups, tile_h, kBandCentersPerLaunch, jobs);
     }
     for (const BandJob& j : jobs) {
@@ -681,15 +759,16 @@ void set_max_bands(int nb) { g_max_bands = std::max(1, std::min(kMaxBands, nb));
 
 void kernel_stats(int kind, int64_t* launches, int64_t* kernel_ns, int64_t* centers, bool reset) {
     int64_t v[3] = {0, 0, 0};
-    for (int k = 0; k < 2; ++k) {
-        if (kind != 2 && kind != k) continue;
+    for (int k = 0; k < 3; ++k) {
+        // kind 0 plain, 1 banded, 2 both, 3 banded with three x-phases
+        if (!(kind == k && k < 2) && !(kind == 2 && k < 2) && !(kind == 3 && k == 2)) continue;
         for (int i = 0; i < 3; ++i) v[i] += g_kstat[k][i].load();
     }
     if (launches) *launches = v[0];
     if (kernel_ns) *kernel_ns = v[1];
     if (centers) *centers = v[2];
     if (reset)
-        for (int k = 0; k < 2; ++k)
+        for (int k = 0; k < 3; ++k)
             for (int i = 0; i < 3; ++i) g_kstat[k][i].store(0);
 }
 

@@ -79,9 +79,10 @@ struct BandParams {
     unsigned long long* emitted;
     unsigned long long* hist;
 };
-// One workgroup per tile; nb bands (1..kMaxBands), ph x-phases (1 or 2);
-// union_bound selects the rolling union-of-disks bound (for nb up to
-// union_bound_max_bands(ph); the square bound otherwise).
+// One workgroup per tile; nb bands (1..kMaxBands), ph x-phases (1 or 2; or 3
+// with 2..5 bands and the union bound); union_bound selects the rolling
+// union-of-disks bound (for nb up to union_bound_max_bands(ph); the square
+// bound otherwise).
 hipError_t launch_scan_band(const BandParams& p, int nb, int ph, bool union_bound,
                             hipStream_t stream);
 // The union-bound kernels exist up to 8 bands, and with two x-phases up to
@@ -90,11 +91,15 @@ hipError_t launch_scan_band(const BandParams& p, int nb, int ph, bool union_boun
 // phases and the full world's 10-band launches ran slower, 1.73 s, than 8-band
 // ones, 1.56 s); with the window sums on the matrix cores, a lane holds one
 // band-phase row of each of two 16-row operand sets above 8 bands.
+// Three x-phases (x, x + P, x + 2P) exist for 2..5 bands with the union bound
+// only: 3 NB band-phases, a thread group of 16 each.
 constexpr int kUnionBoundMaxBands = 8;
 constexpr int kUnionBoundMaxBands2 = 14;
+constexpr int kThreePhaseMinBands = 2, kThreePhaseMaxBands = 5;
 constexpr int union_bound_max_bands(int ph) {
-    return ph == 2 ? kUnionBoundMaxBands2 : kUnionBoundMaxBands;
+    return ph == 2 ? kUnionBoundMaxBands2 : ph == 3 ? kThreePhaseMaxBands : kUnionBoundMaxBands;
 }
+constexpr int min_bands(int ph) { return ph == 3 ? kThreePhaseMinBands : 1; }
 
 // ---- multi-search sweep -----------------------------------------------------
 //
@@ -105,10 +110,10 @@ constexpr int union_bound_max_bands(int ph) {
 struct SweepClass {
     int nb, ph;
 };
-constexpr int kSweepClasses = 11;
-constexpr SweepClass kSweepClass[kSweepClasses] = {{1, 1}, {2, 1}, {3, 1}, {4, 1},
-                                                   {1, 2}, {2, 2}, {3, 2}, {4, 2},
-                                                   {6, 2}, {8, 2}, {12, 2}};
+constexpr int kSweepClasses = 15;
+constexpr SweepClass kSweepClass[kSweepClasses] = {{1, 1}, {2, 1}, {3, 1}, {4, 1}, {1, 2},
+                                                   {2, 2}, {3, 2}, {4, 2}, {6, 2}, {8, 2},
+                                                   {12, 2}, {2, 3}, {3, 3}, {4, 3}, {5, 3}};
 constexpr int kSweepMaxBands = 12;
 // Class index of (nb, ph), or -1.
 inline int sweep_class_of(int nb, int ph) {
@@ -116,18 +121,41 @@ inline int sweep_class_of(int nb, int ph) {
         if (kSweepClass[c].nb == nb && kSweepClass[c].ph == ph) return c;
     return -1;
 }
-// Kernels: one per class with the square bound (index = class), and one per
-// class of at most kUnionBoundMaxBands bands with the union bound
-// (index = kSweepClasses + class).
+// Kernels: the square bound for the one- and two-phase classes (index =
+// class, classes 0..10); the union bound for those of at most
+// kUnionBoundMaxBands bands (index 11 + class, classes 0..9) and for the
+// three-phase classes, which have no square-bound kernel (index 21.., classes
+// 11..14).
+constexpr int kSweepSquareClasses = 11;
 constexpr int kSweepUnionClasses = 10;
-constexpr int kSweepKernels = kSweepClasses + kSweepUnionClasses;
+constexpr int kSweepThreeFirst = 11;
+constexpr int kSweepKernels =
+    kSweepSquareClasses + kSweepUnionClasses + (kSweepClasses - kSweepThreeFirst);
 static_assert(kSweepClass[kSweepUnionClasses - 1].nb <= kUnionBoundMaxBands &&
-                  kSweepClass[kSweepUnionClasses].nb > kUnionBoundMaxBands,
-              "union-bound sweep kernels: the classes within the band limit");
-inline int sweep_kernel_of(int cls, bool union_bound) {
-    return union_bound && cls < kSweepUnionClasses ? kSweepClasses + cls : cls;
+                  kSweepClass[kSweepUnionClasses].nb > kUnionBoundMaxBands &&
+                  kSweepClass[kSweepThreeFirst].ph == 3 &&
+                  kSweepClass[kSweepThreeFirst - 1].ph == 2,
+              "sweep kernels: union bound within the band limit, three phases last");
+// Whether class c has a kernel with the union bound (ub) or the square bound.
+constexpr bool sweep_class_has(int c, bool ub) {
+    return ub ? (c < kSweepUnionClasses || c >= kSweepThreeFirst) : c < kSweepSquareClasses;
 }
-constexpr int sweep_class_of_kernel(int k) { return k < kSweepClasses ? k : k - kSweepClasses; }
+// The kernel of class c: with the union bound where the class has it, else
+// the square bound (-1 for a three-phase class without the union bound).
+inline int sweep_kernel_of(int cls, bool union_bound) {
+    if (cls >= kSweepThreeFirst)
+        return union_bound ? kSweepSquareClasses + kSweepUnionClasses + (cls - kSweepThreeFirst)
+                           : -1;
+    return union_bound && cls < kSweepUnionClasses ? kSweepSquareClasses + cls : cls;
+}
+constexpr int sweep_class_of_kernel(int k) {
+    return k < kSweepSquareClasses                        ? k
+           : k < kSweepSquareClasses + kSweepUnionClasses ? k - kSweepSquareClasses
+                                                          : k - kSweepSquareClasses -
+                                                                kSweepUnionClasses +
+                                                                kSweepThreeFirst;
+}
+constexpr bool sweep_kernel_union(int k) { return k >= kSweepSquareClasses; }
 
 constexpr int32_t kReqActive = 1;  // request status: tiles may be claimed
 

@@ -13,6 +13,7 @@ hipError_t launch_scan_band(const BandParams& p, int nb, int ph, bool union_boun
     if (p.tiles_total <= 0) return hipSuccess;
     if (ph == 2) return band::launch_phases<2>(p, nb, union_bound, stream);
     if (ph == 1) return band::launch_phases<1>(p, nb, union_bound, stream);
+    if (ph == 3) return band::launch_phases3(p, nb, union_bound, stream);
     return hipErrorInvalidValue;
 }
 

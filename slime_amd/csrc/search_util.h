@@ -119,6 +119,8 @@ inline void fill_filters(const SearchSpec& spec, BandParams& p) {
 // center, with the already covered centers masked out; a region between P and
 // 2P wide is one block whose phase-1 columns past the edge are masked out.
 // One phase (ph = 1): a single part spanning the region.
+// Three phases (ph = 3, regions wider than 2P): phases x, x + P, x + 2P over
+// 3P blocks; see x_parts_three.
 struct XPart {
     int32_t x0, w, ox_min, slot_lo, nblocks, phase_cols, ph;
     int64_t centers_per_row;
@@ -128,8 +130,11 @@ struct XPart {
     }
 };
 
+inline std::vector<XPart> x_parts_three(const SearchSpec& spec);
+
 inline std::vector<XPart> x_parts(const SearchSpec& spec, int ph) {
     const int64_t W = spec.w;
+    if (ph == 3) return x_parts_three(spec);
     std::vector<XPart> parts;
     if (ph == 2) {
         const int64_t nfull = W / (2 * kPeriod), rem = W % (2 * kPeriod);
@@ -147,6 +152,42 @@ inline std::vector<XPart> x_parts(const SearchSpec& spec, int ph) {
     } else {
         parts.push_back(XPart{spec.x0, (int32_t)W, 0, 0, 1, (int32_t)W, 1, W});
     }
+    return parts;
+}
+
+// Three phases for a region wider than 2P, every column decided once: with
+// W = 2P n + r, the last 2P + r columns (0 < r <= P) are three phases over
+// slot columns [0, r) and two over [r, P); for P < r < 2P a full 3P block
+// takes P + r of them first (the rest as above, or with n = 1 a one-phase
+// strip of r - P); whole 2P blocks before them keep two phases. With r = 0
+// the two-phase layout is already exact.
+inline std::vector<XPart> x_parts_three(const SearchSpec& spec) {
+    const int64_t W = spec.w, P = kPeriod;
+    const int64_t n = W / (2 * P), r = W % (2 * P);
+    std::vector<XPart> parts;
+    if (n == 0 || r == 0) return x_parts(spec, W > P ? 2 : 1);
+    auto at = [&](int64_t off) { return (int32_t)((uint32_t)spec.x0 + (uint32_t)off); };
+    const int64_t n2 = r <= P ? n - 1 : n - 2;  // whole two-phase blocks first
+    if (n2 > 0)
+        parts.push_back(XPart{spec.x0, (int32_t)(n2 * 2 * P), 0, 0, (int32_t)n2, (int32_t)P, 2,
+                              n2 * 2 * P});
+    int64_t off = std::max<int64_t>(0, n2) * 2 * P;
+    int64_t rr = r;  // columns past a 2P block still to cover with three phases
+    if (r > P) {
+        parts.push_back(XPart{at(off), (int32_t)(3 * P), 0, 0, 1, (int32_t)P, 3, 3 * P});
+        off += 3 * P;
+        rr = r - P;
+        if (n == 1) {  // no 2P block left to pair with: one phase over the rest
+            parts.push_back(XPart{at(off), (int32_t)rr, 0, 0, 1, (int32_t)rr, 1, rr});
+            return parts;
+        }
+    }
+    // the last 2P + rr columns: three phases over slot columns [0, rr), two
+    // over [rr, P)
+    parts.push_back(XPart{at(off), (int32_t)(2 * P + rr), 0, 0, 1, (int32_t)rr, 3, 3 * rr});
+    if (rr < P)
+        parts.push_back(XPart{at(off), (int32_t)(2 * P), 0, (int32_t)rr, 1, (int32_t)P, 2,
+                              2 * (P - rr)});
     return parts;
 }
 
@@ -223,16 +264,15 @@ inline void band_jobs(const SearchSpec& spec, const std::vector<XPart>& parts,
 // job's tiles (what band_tile's `tile` argument and the sweep's done counts
 // refer to), or -1 when the job does not cover that center. Tile t holds
 // x-tile tx = t % tiles_x (x-block tx / tiles_per_block, 496 slot columns
-// from slot_lo + (tx % tiles_per_block) * 496, both phases) and slot rows
+// from slot_lo + (tx % tiles_per_block) * 496, every phase) and slot rows
 // s_begin + (t / tiles_x) * tile_h onwards, in every band of the job.
 inline int64_t band_tile_of(const BandParams& p, int nb, int ph, int64_t cx, int64_t cz) {
     const int64_t ox = cx - (int64_t)p.x0;
     if (ox < p.ox_min || ox >= p.w) return -1;
     int64_t xblock = 0, c = ox;
-    if (ph == 2) {
-        xblock = ox / (2 * (int64_t)kPeriod);
-        c = ox - xblock * 2 * kPeriod;
-        if (c >= kPeriod) c -= kPeriod;
+    if (ph >= 2) {  // x-blocks of ph P columns, the slot column is ox mod P
+        xblock = ox / (ph * (int64_t)kPeriod);
+        c = (ox - xblock * ph * kPeriod) % kPeriod;
     }
     if (c < p.slot_lo || c >= p.phase_cols) return -1;
     const int64_t oz = cz - (int64_t)p.z0;

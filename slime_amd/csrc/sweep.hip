@@ -57,19 +57,20 @@ __device__ bool claim_tile(const SweepArgs& a, int& pos, int& job, int& tile) {
 // (a runtime switch over all classes in one kernel spills).
 // Waves per SIMD each class is built for: the occupancy its per-search kernel
 // reaches with the same tile code (fewer bands need fewer registers).
-// The union-bound kernels (index kSweepClasses + class) keep their class's
-// occupancy where the extra rolling state fits. The 2-band, two-phase class
-// runs at four waves in both forms (at five, with the claim loop around the
-// tile and, in the union-bound form, the matrix-core window sums, it spilled).
-constexpr int kSweepWaves[kSweepKernels] = {6, 5, 5, 5, 5, 4, 4, 4, 4, 3, 3,
-                                            6, 5, 4, 4, 5, 4, 4, 4, 3, 3};
+// The union-bound kernels (index kSweepSquareClasses + class) keep their
+// class's occupancy where the extra rolling state fits. The 2-band, two-phase
+// class runs at four waves in both forms (at five, with the claim loop around
+// the tile and, in the union-bound form, the matrix-core window sums, it
+// spilled). The three-phase kernels (union bound only) run at four.
+constexpr int kSweepWaves[kSweepKernels] = {6, 5, 5, 5, 5, 4, 4, 4, 4, 3, 3, 6, 5,
+                                            4, 4, 5, 4, 4, 4, 3, 3, 4, 4, 4, 4};
 
 template <int K>
 __global__ __launch_bounds__(kThreads, kSweepWaves[K]) void sweep_kernel(SweepArgs a) {
     constexpr int C = sweep_class_of_kernel(K);
     constexpr int NB = kSweepClass[C].nb, PH = kSweepClass[C].ph;
-    constexpr bool UB = K >= kSweepClasses;
-    __shared__ band::TileLds<NB> lds;
+    constexpr bool UB = sweep_kernel_union(K);
+    __shared__ band::TileLds<NB, PH> lds;
     __shared__ int s_job, s_tile;
     for (int i = threadIdx.x; i < kHistBins; i += kThreads) lds.lhist[i] = 0;
     int pos = 0;
@@ -113,8 +114,8 @@ template <int K>
 __global__ __launch_bounds__(kThreads, 3) void batch_kernel(BatchArgs a) {
     constexpr int C = sweep_class_of_kernel(K);
     constexpr int NB = kSweepClass[C].nb, PH = kSweepClass[C].ph;
-    constexpr bool UB = K >= kSweepClasses;
-    __shared__ band::TileLds<NB> lds;
+    constexpr bool UB = sweep_kernel_union(K);
+    __shared__ band::TileLds<NB, PH> lds;
     for (int i = threadIdx.x; i < kHistBins; i += kThreads) lds.lhist[i] = 0;
     const int64_t blk = blockIdx.x;
     int lo = 0, hi = a.n - 1;  // the last i with first[i] <= blk
@@ -184,6 +185,10 @@ int sweep_blocks_per_cu(int kernel) {
         case 18: return blocks_per_cu<18>();
         case 19: return blocks_per_cu<19>();
         case 20: return blocks_per_cu<20>();
+        case 21: return blocks_per_cu<21>();
+        case 22: return blocks_per_cu<22>();
+        case 23: return blocks_per_cu<23>();
+        case 24: return blocks_per_cu<24>();
         default: return 1;
     }
 }
@@ -212,10 +217,14 @@ hipError_t launch_sweep(const SweepArgs& a, int kernel, int grid, hipStream_t st
         case 18: return launch_class<18>(a, grid, stream);
         case 19: return launch_class<19>(a, grid, stream);
         case 20: return launch_class<20>(a, grid, stream);
+        case 21: return launch_class<21>(a, grid, stream);
+        case 22: return launch_class<22>(a, grid, stream);
+        case 23: return launch_class<23>(a, grid, stream);
+        case 24: return launch_class<24>(a, grid, stream);
         default: return hipErrorInvalidValue;
     }
 }
-static_assert(kSweepKernels == 21, "one case per kernel");
+static_assert(kSweepKernels == 25, "one case per kernel");
 
 hipError_t launch_batch(const BatchArgs& a, int kernel, int64_t blocks, hipStream_t stream) {
     if (a.n <= 0 || blocks <= 0) return hipSuccess;
@@ -242,6 +251,10 @@ hipError_t launch_batch(const BatchArgs& a, int kernel, int64_t blocks, hipStrea
         case 18: return launch_batch_kernel<18>(a, blocks, stream);
         case 19: return launch_batch_kernel<19>(a, blocks, stream);
         case 20: return launch_batch_kernel<20>(a, blocks, stream);
+        case 21: return launch_batch_kernel<21>(a, blocks, stream);
+        case 22: return launch_batch_kernel<22>(a, blocks, stream);
+        case 23: return launch_batch_kernel<23>(a, blocks, stream);
+        case 24: return launch_batch_kernel<24>(a, blocks, stream);
         default: return hipErrorInvalidValue;
     }
 }

@@ -86,7 +86,8 @@ class Sweeper {
     void set_costs(const CostTable& c) { costs_ = c; }
 
     bool calibrate();                         // fills the cost table (once)
-    double tile_ns(int cls, int32_t tile_h) const;
+    double class_tile_ns(int cls, int32_t tile_h) const;  // measured ns of a tile of a class
+    double tile_ns(int kernel, int32_t tile_h) const;     // of a tile of a kernel
     bool plan(SweepRequest& r) const;         // schedule + hit capacity
     static int64_t footprint(const SweepRequest& r);  // device bytes it holds
     bool reserve(SweepRequest& r);            // slot, job range, hit region
@@ -165,6 +166,7 @@ struct BatchStats {
     int32_t fallbacks = 0;  // searches rerun alone after a hit-region overflow
     int64_t hits_stored = 0;  // hits in the searches' device regions
     int64_t hits_copied = 0;  // of them, brought to the host to rank (top-K selection)
+    int64_t class_centers[kSweepClasses] = {};  // centers decided per kernel class
 };
 
 // Runs n searches in one sweep. out holds n * max_results rows (search i at

@@ -231,8 +231,8 @@ bool Sweeper::calibrate() {
             spec.seed = 0x5EED5EED5EEDll + c;
             spec.x0 = -kPeriod;
             spec.z0 = 12345;
-            spec.w = ph == 2 ? 2 * kPeriod : kPeriod;
-            const int64_t tiles_x = ph == 2 ? kTilesPerPhase : kTilesPerPhase;
+            spec.w = ph * kPeriod;
+            const int64_t tiles_x = kTilesPerPhase;
             const int64_t want = (k < 0 ? 1 : (th == 64 ? 24 : 6)) * (int64_t)grid_[kern];
             const int64_t row_tiles = std::max<int64_t>(1, (want + tiles_x - 1) / tiles_x);
             const int64_t slot_rows = std::min<int64_t>(kPeriod / th * th, row_tiles * th);
@@ -292,11 +292,14 @@ bool Sweeper::calibrate() {
     return true;
 }
 
-// Costs are measured per class (square-bound kernels); a union-bound kernel
-// is costed as its class.
-double Sweeper::tile_ns(int kernel, int32_t tile_h) const {
-    const int cls = sweep_class_of_kernel(kernel);
+// Costs are measured per class (with its union-bound kernel where it has
+// one); a kernel is costed as its class.
+double Sweeper::class_tile_ns(int cls, int32_t tile_h) const {
     return costs_.ns_row[cls] * ((double)tile_h + kTileHalo + costs_.extra_rows[cls]);
+}
+
+double Sweeper::tile_ns(int kernel, int32_t tile_h) const {
+    return class_tile_ns(sweep_class_of_kernel(kernel), tile_h);
 }
 
 // The schedule of one search under the measured costs. Choices:
@@ -319,12 +322,12 @@ bool Sweeper::plan(SweepRequest& r) const {
         if (c < 0) return -1.0;
         const int32_t th = tile_for(s_rows);
         const int64_t tiles = (int64_t)xp.nblocks * xp.tiles_per_block() * ((s_rows + th - 1) / th);
-        return (double)tiles * tile_ns(c, th);
+        return (double)tiles * class_tile_ns(c, th);
     };
     // the union bound has kernels for the classes of up to
-    // kUnionBoundMaxBands bands: where it applies, only those cover the bands
+    // kUnionBoundMaxBands bands and the three-phase classes: where it
+    // applies, only those cover the bands
     const bool ub = union_bound_for(spec.mode, spec.min_size);
-    const int ncls = ub ? kSweepUnionClasses : kSweepClasses;
     // cheapest cover of bands [b0, b0 + n) over s_rows slot rows
     auto cover = [&](const XPart& xp, int64_t b0, int64_t n, int32_t s_rows,
                      std::vector<Group>& out) {
@@ -332,9 +335,11 @@ bool Sweeper::plan(SweepRequest& r) const {
         std::vector<int> pick(n + 1, 0);
         cost[0] = 0;
         for (int64_t k = 1; k <= n; ++k)
-            for (int c = 0; c < ncls; ++c) {
+            for (int c = 0; c < kSweepClasses; ++c) {
                 const int nb = kSweepClass[c].nb;
-                if (kSweepClass[c].ph != xp.ph || nb > k || cost[k - nb] < 0) continue;
+                if (!sweep_class_has(c, ub) || kSweepClass[c].ph != xp.ph || nb > k ||
+                    cost[k - nb] < 0)
+                    continue;
                 const double v = cost[k - nb] + group_cost(xp, nb, s_rows);
                 if (cost[k] < 0 || v < cost[k]) {
                     cost[k] = v;
@@ -374,8 +379,10 @@ bool Sweeper::plan(SweepRequest& r) const {
     std::vector<std::vector<XPart>> xopts;
     xopts.push_back(x_parts(spec, 1));
     if (spec.w > kPeriod) xopts.push_back(x_parts(spec, 2));
-    if (spec.w > 2 * kPeriod && spec.w % (2 * kPeriod) != 0)
+    if (spec.w > 2 * kPeriod && spec.w % (2 * kPeriod) != 0) {
         xopts.push_back(x_parts_one_phase_tail(spec));
+        if (ub) xopts.push_back(x_parts(spec, 3));
+    }
     double best = -1;
     std::vector<XPart> best_parts;
     std::vector<std::vector<Group>> best_groups;
@@ -956,6 +963,8 @@ int run_batch(const SearchSpec* specs, int32_t n, int32_t max_results, ResultRow
         st.jobs += (int32_t)r.jobs.size();
         st.hits_stored += r.hits_stored;
         st.hits_copied += r.hits_copied;
+        for (size_t j = 0; j < r.jobs.size(); ++j)
+            st.class_centers[sweep_class_of_kernel(r.job_cls[j])] += r.jobs[j].centers;
     }
     cleanup();
     st.kernel_ns = rs.kernel_ns;

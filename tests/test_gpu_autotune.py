@@ -33,3 +33,22 @@ def test_tuned_plan_equals_even_grouping(gpu, monkeypatch, case):
     assert L.get_processed_centers() == w * h
     assert tuned[0] == even[0] and tuned[2] == even[2] and tuned[1] == even[1]
     assert sum(tuned[2]) == tuned[0] and tuned[0] > 0
+
+
+def test_tuned_plan_takes_three_phases(gpu, monkeypatch):
+    """A region 2P + 117,857 columns wide: the measured costs take the
+    three-phase layout past the first 2P block (no masked columns), with the
+    same totals, histogram and rows as the even two-phase plan."""
+    import slime_amd
+    seed, x0, z0, w, h, thr = 0x3F3F3F, -190_000, -190_000, 380_001, 380_001, 55
+    assert w * h >= 1 << 37
+    monkeypatch.setenv("SLIME_BAND_TUNE", "1")
+    slime_amd.kernel_stats(slime_amd.KERNEL_BANDED_THREE, reset=True)
+    tuned = slime_amd.scan_rect(seed, x0, z0, w, h, thr, 221, 5000)
+    launches, _, centers = slime_amd.kernel_stats(slime_amd.KERNEL_BANDED_THREE)
+    print("three-phase launches", launches, "centers", centers, flush=True)
+    assert launches > 0 and centers > 0
+    monkeypatch.setenv("SLIME_BAND_TUNE", "0")
+    even = slime_amd.scan_rect(seed, x0, z0, w, h, thr, 221, 5000)
+    assert tuned[0] == even[0] and tuned[2] == even[2] and tuned[1] == even[1]
+    assert sum(tuned[2]) == tuned[0] and tuned[0] > 0

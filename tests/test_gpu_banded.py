@@ -116,6 +116,50 @@ def test_band_sets_vs_oracle(gpu, oracle, case):
     assert rows == want_rows
 
 
+@pytest.mark.parametrize("case", range(7))
+def test_three_phase_band_sets_vs_oracle(gpu, oracle, case):
+    """Three x-phases (x, x + P, x + 2P; phase 2 paired across band pairs)
+    forced past the last whole 2P block: widths 2P + 700 (three phases over
+    700 slot columns, two over the rest of the block), 3P (one 3P block),
+    3P + 1000 (a 3P block and a one-phase strip), 5P - 3 and 7P + 5 (two-phase
+    blocks before them); odd and even band counts, 6 bands as two groups of
+    three; thresholds low enough that the refinement and emits run."""
+    import slime_amd
+    rng = random.Random(4242 + case)
+    seed = s64(rng.getrandbits(64))
+    nb = [2, 3, 4, 5, 6, 3, 2][case]
+    band_rows = [20, 12, 10, 8, 6, 10, 16][case]
+    w = [2 * P + 700, 3 * P, 3 * P + 1000, 5 * P - 3, 2 * P + 5000, 7 * P + 5, 2 * P + P // 2][case]
+    x0 = rng.randint(-10 ** 8, 10 ** 8)
+    z0 = rng.randint(-10 ** 8, 10 ** 8)
+    h = (nb - 1) * P + band_rows
+    thr, k = [36, 38, 35, 37, 36, 38, 34][case], 500
+    print("case", case, seed, x0, z0, w, h, band_rows, flush=True)
+    slime_amd.kernel_stats(slime_amd.KERNEL_BANDED_THREE, reset=True)
+    total, rows, hist = slime_amd.scan_bands(seed, x0, z0, w, h, thr, 221, band_rows,
+                                             slime_amd.MODE_BANDED_UNION3, k)
+    launches, _, centers = slime_amd.kernel_stats(slime_amd.KERNEL_BANDED_THREE)
+    assert launches > 0 and centers > 0
+    want_total, want_hist, want_rows = oracle_bands(oracle, seed, x0, z0, w, h, band_rows, thr, k)
+    assert total == want_total
+    assert hist == want_hist
+    assert rows == want_rows
+
+
+@pytest.mark.parametrize("radius,thr", [(140000, 52), (190000, 56), (262143, 58)])
+def test_three_phase_equals_plain_large(gpu, radius, thr):
+    """Squares between 2P and 4P wide through the forced three-phase layout
+    and the plain per-chunk kernel: identical totals, histograms, rows."""
+    import slime_amd
+    seed = s64(0x0FED_CBA9_8765_4321 * (radius + 3))
+    x0 = z0 = -radius
+    w = h = 2 * radius + 1
+    print("square", radius, thr, flush=True)
+    b = slime_amd.scan_bands(seed, x0, z0, w, h, thr, 221, 0, slime_amd.MODE_BANDED_UNION3, 2000)
+    p = slime_amd.scan_bands(seed, x0, z0, w, h, thr, 221, 0, slime_amd.MODE_PLAIN, 2000)
+    assert b[0] == p[0] and b[2] == p[2] and b[1] == p[1]
+
+
 @pytest.mark.parametrize("radius,thr", [(3000, 45), (20000, 52), (65536, 54), (70000, 54),
                                         (131100, 55), (200000, 58), (400000, 57)])
 def test_banded_equals_plain_large(gpu, radius, thr):

@@ -12,7 +12,8 @@ pytestmark = pytest.mark.gpu
 def mixed_batch(seed, n):
     """Searches of every schedule class: tiny to two-phase, annuli, ties."""
     rng = random.Random(seed)
-    radii = [0, 1, 5, 40, 300, 3000, 20000, 65535, 65536, 70000, 100000, 150000]
+    radii = [0, 1, 5, 40, 300, 3000, 20000, 65535, 65536, 70000, 100000, 150000, 140000,
+             180000]
     out = []
     for i in range(n):
         rd = radii[i % len(radii)]
@@ -151,3 +152,23 @@ def test_batch_selection_larger_than_staging_falls_back(gpu):
     st = slime_amd.batch_stats()
     assert total > k and st["hits_copied"] == st["hits_stored"]
     assert (total, rows) == slime_amd.search_slime_clusters_gpu_v1_centered(*s, k)
+
+
+def test_batch_three_phase_regions(gpu):
+    """Regions between 2^18 and 3 * 2^17 columns wide: the sweep planner takes
+    the three-phase classes (every column decided once, no masked phase), and
+    each search equals its separate call."""
+    import slime_amd
+    rng = random.Random(31)
+    searches = [(rng.getrandbits(64) - (1 << 63), rd, rng.randint(52, 58), 221, 0,
+                 rng.randint(-1_500_000, 1_500_000), rng.randint(-1_500_000, 1_500_000))
+                for rd in (135000, 150000, 170000, 196000)]
+    got = run_batch(searches, 100)
+    used = slime_amd.batch_class_centers()
+    print("class centers", used, flush=True)
+    assert sum(v for (nb, ph), v in used.items() if ph == 3) > 0
+    for s, (total, rows) in zip(searches, got):
+        print("search", s, flush=True)
+        want = slime_amd.search_slime_clusters_gpu_v1_centered(s[0], s[1], s[2], s[3], s[4],
+                                                               s[5], s[6], 100)
+        assert (total, rows) == want

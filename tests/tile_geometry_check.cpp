@@ -2,8 +2,8 @@
 // job that decides a center, which the engines use to apply exact recounts to
 // the centers a cancelled search has already decided. For random regions, x
 // layouts (one phase; two phases with whole blocks, a right-aligned tail
-// block or one masked block; two phases plus a one-phase strip), band groups
-// and slot-row cuts:
+// block or one masked block; two phases plus a one-phase strip; three phases
+// over part of the width), band groups and slot-row cuts:
 //   * every center of the region lies in exactly one job, and band_tile_of
 //     names a tile whose slot columns and slot rows, decomposed as the kernel
 //     does (tile % tiles_x -> x-block and x-tile, tile / tiles_x -> z-tile),
@@ -28,6 +28,9 @@ struct TileExtent {
     int64_t xblock, col0, s0;
 };
 
+// Columns of an x-block: ph P (one block of 2P span with one phase).
+int64_t block_cols(int ph) { return (ph == 3 ? 3 : 2) * kPeriod; }
+
 TileExtent extent(const BandParams& p, int64_t t) {
     const int64_t tx = t % p.tiles_x, ts = t / p.tiles_x;
     return TileExtent{tx / p.tiles_per_block, p.slot_lo + (tx % p.tiles_per_block) * kBandTileOut,
@@ -41,7 +44,7 @@ bool tile_holds(const BandParams& p, int nb, int ph, int64_t t, int64_t ox, int6
     if (ox < p.ox_min || ox >= p.w) return false;
     bool col = false;
     for (int phase = 0; phase < ph; ++phase) {
-        const int64_t c = ox - e.xblock * 2 * kPeriod - phase * kPeriod;
+        const int64_t c = ox - e.xblock * block_cols(ph) - phase * kPeriod;
         col = col || (c >= e.col0 && c < e.col0 + kBandTileOut && c < p.phase_cols);
     }
     if (!col) return false;
@@ -61,16 +64,20 @@ int main() {
     for (int it = 0; it < 400; ++it) {
         SearchSpec spec;
         spec.seed = (int64_t)rng();
-        const int64_t widths[] = {600, kPeriod, kPeriod + 1, kPeriod + 5000, 2 * kPeriod,
-                                  2 * kPeriod + 700, 3 * kPeriod + 1000, 5 * kPeriod - 3};
-        spec.w = widths[rng() % 8];
+        const int64_t widths[] = {600,           kPeriod,          kPeriod + 1,
+                                  kPeriod + 5000, 2 * kPeriod,      2 * kPeriod + 700,
+                                  3 * kPeriod + 1000, 5 * kPeriod - 3, 7 * kPeriod + 5,
+                                  6 * kPeriod + 3, 4 * kPeriod,     3 * kPeriod};
+        spec.w = widths[rng() % 12];
         spec.h = 1 + (int64_t)(rng() % (4 * (uint64_t)kPeriod));
         spec.x0 = (int32_t)((int64_t)(rng() % 2000000000ull) - 1000000000);
         spec.z0 = (int32_t)((int64_t)(rng() % 2000000000ull) - 1000000000);
         std::vector<XPart> parts;
-        const int layout = (int)(rng() % 3);
+        const int layout = (int)(rng() % 4);
         if (layout == 2 && spec.w > 2 * kPeriod && spec.w % (2 * kPeriod) != 0)
             parts = x_parts_one_phase_tail(spec);
+        else if (layout == 3 && spec.w > 2 * kPeriod)
+            parts = x_parts(spec, 3);
         else
             parts = x_parts(spec, (layout >= 1 && spec.w > kPeriod) ? 2 : 1);
         const int64_t nbz = bands_of(spec);
@@ -117,7 +124,7 @@ int main() {
                 // a random center of the tile's extent that the job decides
                 for (int tries = 0; tries < 64; ++tries) {
                     const int phase = (int)(rng() % (uint64_t)j.ph);
-                    const int64_t ox = e.xblock * 2 * kPeriod + phase * kPeriod + e.col0 +
+                    const int64_t ox = e.xblock * block_cols(j.ph) + phase * kPeriod + e.col0 +
                                        (int64_t)(rng() % kBandTileOut);
                     const int64_t oz = (int64_t)(rng() % (uint64_t)j.nb) * kPeriod + e.s0 +
                                        (int64_t)(rng() % (uint64_t)j.p.tile_h);
