@@ -58,7 +58,7 @@ struct SweepRequest {
 };
 
 struct RoundPolicy {
-    int64_t round_ns = 0;         // device time per class launch (0: unbounded)
+    int64_t round_ns = 0;         // device time per round, over its class launches (0: unbounded)
     bool shortest_first = false;  // claim order: least work left first (else largest tiles first)
     bool use_graph = false;       // replay the launch sequence from a captured hipGraph
     // one workgroup per tile over a range of tiles per job, no claims (the

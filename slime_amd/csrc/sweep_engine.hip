@@ -381,7 +381,10 @@ bool Sweeper::plan(SweepRequest& r) const {
     if (spec.w > kPeriod) xopts.push_back(x_parts(spec, 2));
     if (spec.w > 2 * kPeriod && spec.w % (2 * kPeriod) != 0) {
         xopts.push_back(x_parts_one_phase_tail(spec));
-        if (ub) xopts.push_back(x_parts(spec, 3));
+        // three phases past the last whole 2P block, for regions of at most 5
+        // bands (as the per-search planner: with more, the two-phase parts'
+        // wide groups measured faster than the model's three-phase part)
+        if (ub && bands_of(spec) <= kThreePhaseMaxBands) xopts.push_back(x_parts(spec, 3));
     }
     double best = -1;
     std::vector<XPart> best_parts;
@@ -634,30 +637,48 @@ bool Sweeper::round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& p
         int64_t blocks;
     };
     std::vector<Launch> launches;
+    // classes holding the most urgent request go first, and the round's device
+    // time is handed out in that order: a class takes what its tiles need, up
+    // to what is left of round_ns, and at least one tile per resident slot
+    // (so every class moves on every round)
+    struct ClassWork {
+        int c;
+        double min_key, per_tile, need_ns;
+        int64_t tiles;
+    };
+    std::vector<ClassWork> work;
     for (int c = 0; c < kSweepKernels; ++c) {
         std::vector<Item>& v = items[c];
         if (v.empty()) continue;
         std::sort(v.begin(), v.end(), [](const Item& a, const Item& b) {
             return a.key != b.key ? a.key < b.key : a.seq < b.seq;
         });
-        Launch l;
-        l.cls = c;
-        l.off = (int32_t)order.size();
-        l.min_key = v.front().key;
         int64_t tiles = 0;
         double tile_sum = 0;
         for (const Item& it : v) {
             tiles += it.left;
             tile_sum += tile_ns(c, it.tile_h) * (double)it.left;
         }
-        // tiles (claims) that take about round_ns of the whole device, at
-        // least one workgroup per resident slot
+        work.push_back(ClassWork{c, v.front().key, tile_sum / (double)std::max<int64_t>(1, tiles),
+                                 tile_sum, tiles});
+    }
+    std::stable_sort(work.begin(), work.end(),
+                     [](const ClassWork& a, const ClassWork& b) { return a.min_key < b.min_key; });
+    double left_ns = (double)pol.round_ns;
+    for (const ClassWork& cw : work) {
+        const int c = cw.c;
+        std::vector<Item>& v = items[c];
+        Launch l;
+        l.cls = c;
+        l.off = (int32_t)order.size();
+        l.min_key = cw.min_key;
         int64_t want = INT64_MAX;
         if (pol.round_ns > 0) {
-            const double per_tile = tile_sum / (double)std::max<int64_t>(1, tiles);
-            want = (int64_t)std::min<double>(
-                4e9, std::max<double>((double)grid_[c], (double)pol.round_ns /
-                                                            std::max(1e-3, per_tile)));
+            const double per_tile = std::max(1e-3, cw.per_tile);
+            const double give =
+                std::max(std::min(cw.need_ns, left_ns), per_tile * (double)grid_[c]);
+            left_ns -= std::min(give, cw.need_ns);
+            want = (int64_t)std::min<double>(4e9, std::max<double>((double)grid_[c], give / per_tile));
         }
         l.budget = pol.round_ns > 0 ? (uint32_t)want : 0xFFFFFFFFu;
         l.first_off = (int32_t)firsts.size();
@@ -682,9 +703,6 @@ bool Sweeper::round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& p
         launches.push_back(l);
     }
     if (launches.empty()) return true;
-    // classes holding the most urgent request go first
-    std::stable_sort(launches.begin(), launches.end(),
-                     [](const Launch& a, const Launch& b) { return a.min_key < b.min_key; });
     if (hipMemcpyAsync(d_order_, order.data(), sizeof(int32_t) * order.size(),
                        hipMemcpyHostToDevice, stream_) != hipSuccess ||
         (direct && (hipMemcpyAsync(d_first_, firsts.data(), sizeof(int64_t) * firsts.size(),

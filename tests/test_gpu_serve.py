@@ -30,7 +30,7 @@ def searches(seed, n, radii):
 def test_served_mix_equals_alone(gpu, persistent):
     """Direct rounds (the default) and the persistent work-queue kernels."""
     import slime_amd
-    reqs = searches(3, 18, [40, 300, 3000, 20000, 50000, 100000, 150000, 5])
+    reqs = searches(3, 18, [40, 300, 3000, 20000, 50000, 100000, 150000, 5, 180000])
     # a last one large enough (2e12 centers) to be still running when paused
     reqs.append((0x5EEDF00D, 700000, 56, 221, 0, 123456, -654321))
     k = 300
