@@ -114,23 +114,25 @@ __device__ __forceinline__ void shift_in_above(uint32_t& acc, uint32_t v, uint32
         : "vcc");
 }
 
-// Bitplanes of a ring row: E, one plane per band for x-phases 0 and 1 (with
-// one phase, the band's own test), and with three phases one plane per band
-// pair (2k, 2k + 1) for phase 2 (the last band alone when NB is odd).
+// Bitplanes of a ring row: E, one plane per band for x-phases 0 and 1, and
+// for the phase without a partner (phase 0 with one phase, phase 2 with
+// three) one plane per band pair (2k, 2k + 1), the last band alone when NB is
+// odd.
 template <int NB, int PH>
 struct Ring {
-    static constexpr int kPlanes = 1 + NB + (PH == 3 ? (NB + 1) / 2 : 0);
+    static constexpr int kPlanes = 1 + (PH >= 2 ? NB : 0) + (PH != 2 ? (NB + 1) / 2 : 0);
     static constexpr int kRowDw = kPlanes * kPlaneDw + 2;  // + pad for window reads
     static constexpr int kDwords = kRing * kRowDw;
 };
 
 // The plane of band b, phase ph: phases 0 and 1 share their band's plane (E
-// tells which of them is the even one), phase 2 shares a plane per band pair
-// (E tells which band). Either way band b, phase ph is the plane's bitmap
-// under E where (b ^ ph) is odd and under ~E where it is even.
+// tells which of them is the even one); the phase without a partner shares a
+// plane per band pair (E tells which band). Either way band b, phase ph is
+// the plane's bitmap under E where (b ^ ph) is odd and under ~E where it is
+// even.
 template <int NB, int PH>
 __host__ __device__ constexpr int plane_of(int b, int ph) {
-    return PH < 3 || ph < 2 ? 1 + b : 1 + NB + (b >> 1);
+    return PH == 1 ? 1 + (b >> 1) : PH == 2 || ph < 2 ? 1 + b : 1 + NB + (b >> 1);
 }
 
 // LDS of one workgroup: the bitplane ring and the score histogram.
@@ -615,8 +617,10 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
                     band_slot(xlo[j], xh0[j], PH >= 2 ? xh1[PH >= 2 ? j : 0] : xh0[j], zlo, zhi[0]);
                 acc[0] = acc[0] + acc[0] + (sl.e ? 1u : 0u);
                 // all values first, so the multiplies of independent bands overlap
-                uint32_t v[NB];
-                if constexpr (kPreCarry) {
+                constexpr int kXPairs = PH >= 2 ? NB : 0;  // tests of x-phase pairs
+                uint32_t v[PH >= 2 ? NB : 1];
+                if constexpr (PH == 1) {
+                } else if constexpr (kPreCarry) {
                     uint32_t xa = sl.xa + sl.carry, xb = sl.xb + sl.carry;
                     asm("" : "+v"(xa), "+v"(xb));  // the bands add to these sums
                     static_for<0, NB>([&](auto bc) {
@@ -630,26 +634,29 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
                     });
                 }
                 // shift in the complement of each hit: [a*K + bias > limit]
-                static_for<0, NB>([&](auto bc) {
+                static_for<0, kXPairs>([&](auto bc) {
                     constexpr int b = decltype(bc)::value;
                     shift_in_above(acc[b + 1], v[b], fifth_limit);
                 });
-                if constexpr (PH == 3) {
-                    // phase 2 (x + 2P) has phase 0's parity: of the band pair
-                    // (2k, 2k + 1) the even chunk is band 2k's where e == 0 and
-                    // band 2k + 1's where e == 1 (an odd NB's last band is
-                    // tested alone; the plane's mask keeps it where it is even)
+                if constexpr (PH != 2) {
+                    // the phase without a partner (phase 0 with one phase;
+                    // phase 2 = x + 2P, which has phase 0's parity, with
+                    // three): of the band pair (2k, 2k + 1) the even chunk is
+                    // band 2k's where e == 0 and band 2k + 1's where e == 1 (an
+                    // odd NB's last band is tested alone; the plane's mask
+                    // keeps it where it is even)
                     constexpr int kPairs = (NB + 1) / 2;
+                    const uint32_t xs = PH == 1 ? xh0[j] : xh2[PH == 3 ? j : 0];
                     uint32_t v2[kPairs];
                     static_for<0, kPairs>([&](auto kc) {
                         constexpr int k = decltype(kc)::value;
                         uint32_t zs = zhi[2 * k];
                         if constexpr (2 * k + 1 < NB) zs = sl.e ? zhi[2 * k + 1] : zhi[2 * k];
-                        v2[k] = ((xh2[j] + zs + sl.carry) ^ kHighScramble) * kMulInv5 + sl.bias;
+                        v2[k] = ((xs + zs + sl.carry) ^ kHighScramble) * kMulInv5 + sl.bias;
                     });
                     static_for<0, kPairs>([&](auto kc) {
                         constexpr int k = decltype(kc)::value;
-                        shift_in_above(acc[1 + NB + k], v2[k], fifth_limit);
+                        shift_in_above(acc[1 + kXPairs + k], v2[k], fifth_limit);
                     });
                 }
             });
@@ -856,7 +863,7 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
                 for (int hb = 0; hb < C::kBands; ++hb) {
                     const int b = b_lo + hb;
                     if (b >= NB) continue;  // odd NB: the second half is one band short
-                    const uint32_t hw = window20(rowp + (b + 1) * kPlaneDw, gbit);
+                    const uint32_t hw = window20(rowp + plane_of<NB, PH>(b, 0) * kPlaneDw, gbit);
                     uint32_t w_first = 0;
 #pragma unroll
                     for (int pp = 0; pp < C::kPhases; ++pp) {

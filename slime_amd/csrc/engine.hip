@@ -654,6 +654,10 @@ void plan_banded(const SearchSpec& spec, std::deque<Work>& out) {
         if (spec.w > 2 * kPeriod && spec.w % (2 * kPeriod) != 0 && three_phase_enabled() &&
             nbz <= kThreePhaseMaxBands)
             layouts.push_back(x_parts(spec, 3));
+        // two phases where both lie inside the region, one phase (bands
+        // paired) over the remaining columns
+        if (spec.w > kPeriod && spec.w % (2 * kPeriod) != 0)
+            layouts.push_back(x_parts_one_phase_tail(spec));
         double best = -1;
         for (const std::vector<XPart>& parts : layouts) {
             const int64_t col_tiles = col_tiles_of(parts);

@@ -301,17 +301,29 @@ inline bool within_search_limit(int64_t w, int64_t h) {
     return w > 0 && h > 0 && w <= kMaxSearchCenters / h;
 }
 
-// Two phases over the whole 2P blocks from x0 and one phase over the
-// remaining columns (for regions wider than 2P).
+// Two phases where both phases of a slot column lie inside the region, one
+// phase (its bands paired) elsewhere: whole 2P blocks from x0; then, for the
+// remainder r = W mod 2P, a one-phase strip of its r columns when r <= P, or
+// else two phases over its slot columns [0, r - P) and one phase over slot
+// columns [r - P, P). For regions wider than P that are not a whole number of
+// 2P blocks wide.
 inline std::vector<XPart> x_parts_one_phase_tail(const SearchSpec& spec) {
-    const int64_t W = spec.w;
-    const int64_t nfull = W / (2 * kPeriod), rem = W % (2 * kPeriod);
+    const int64_t W = spec.w, P = kPeriod;
+    const int64_t nfull = W / (2 * P), rem = W % (2 * P);
     std::vector<XPart> parts;
-    parts.push_back(XPart{spec.x0, (int32_t)(nfull * 2 * kPeriod), 0, 0, (int32_t)nfull, kPeriod,
-                          2, nfull * 2 * kPeriod});
-    if (rem > 0)
-        parts.push_back(XPart{(int32_t)((uint32_t)spec.x0 + (uint32_t)(nfull * 2 * kPeriod)),
-                              (int32_t)rem, 0, 0, 1, (int32_t)rem, 1, rem});
+    if (nfull > 0)
+        parts.push_back(XPart{spec.x0, (int32_t)(nfull * 2 * P), 0, 0, (int32_t)nfull, (int32_t)P,
+                              2, nfull * 2 * P});
+    if (rem == 0) return parts;
+    const int32_t at = (int32_t)((uint32_t)spec.x0 + (uint32_t)(nfull * 2 * P));
+    if (rem <= P) {
+        parts.push_back(XPart{at, (int32_t)rem, 0, 0, 1, (int32_t)rem, 1, rem});
+    } else {
+        const int64_t r2 = rem - P;  // slot columns with both phases inside
+        parts.push_back(XPart{at, (int32_t)rem, 0, 0, 1, (int32_t)r2, 2, 2 * r2});
+        parts.push_back(XPart{(int32_t)((uint32_t)at + (uint32_t)r2), (int32_t)(P - r2), 0, 0, 1,
+                              (int32_t)(P - r2), 1, P - r2});
+    }
     return parts;
 }
 

@@ -305,7 +305,9 @@ double Sweeper::tile_ns(int kernel, int32_t tile_h) const {
 // The schedule of one search under the measured costs. Choices:
 //  * x: one phase over the whole width; two phases (whole 2P blocks plus a
 //    right-aligned block for the remainder, or one masked block between P and
-//    2P); or two-phase blocks plus a one-phase strip for the remainder;
+//    2P); two phases where both phases of a slot column are inside and one
+//    phase elsewhere (x_parts_one_phase_tail); or, with the union bound and
+//    at most 5 bands, three phases past the last whole 2P block;
 //  * z, per x-part: the bands grouped into kernel classes (cheapest cover),
 //    either all over P slot rows, or the full bands so and the last, partial
 //    band as its own job over just its rows.
@@ -379,12 +381,13 @@ bool Sweeper::plan(SweepRequest& r) const {
     std::vector<std::vector<XPart>> xopts;
     xopts.push_back(x_parts(spec, 1));
     if (spec.w > kPeriod) xopts.push_back(x_parts(spec, 2));
-    if (spec.w > 2 * kPeriod && spec.w % (2 * kPeriod) != 0) {
+    if (spec.w > kPeriod && spec.w % (2 * kPeriod) != 0) {
         xopts.push_back(x_parts_one_phase_tail(spec));
         // three phases past the last whole 2P block, for regions of at most 5
         // bands (as the per-search planner: with more, the two-phase parts'
         // wide groups measured faster than the model's three-phase part)
-        if (ub && bands_of(spec) <= kThreePhaseMaxBands) xopts.push_back(x_parts(spec, 3));
+        if (ub && spec.w > 2 * kPeriod && bands_of(spec) <= kThreePhaseMaxBands)
+            xopts.push_back(x_parts(spec, 3));
     }
     double best = -1;
     std::vector<XPart> best_parts;

@@ -52,3 +52,19 @@ def test_tuned_plan_takes_three_phases(gpu, monkeypatch):
     even = slime_amd.scan_rect(seed, x0, z0, w, h, thr, 221, 5000)
     assert tuned[0] == even[0] and tuned[2] == even[2] and tuned[1] == even[1]
     assert sum(tuned[2]) == tuned[0] and tuned[0] > 0
+
+
+def test_tuned_plan_pairs_one_phase_columns(gpu, monkeypatch):
+    """A region 1.6 P wide: where the phase-1 columns would fall outside, the
+    measured plan may scan one phase with paired bands instead of a masked
+    two-phase block; the same totals, histogram and rows either way."""
+    import slime_amd
+    seed, w, h, thr = -0x77AA55, int(1.6 * P) | 1, 2 * P - 3000, 54
+    x0 = z0 = -(w // 2)
+    monkeypatch.setenv("SLIME_BAND_TUNE", "1")
+    tuned = slime_amd.scan_rect(seed, x0, z0, w, h, thr, 221, 5000)
+    monkeypatch.setenv("SLIME_BAND_TUNE", "0")
+    even = slime_amd.scan_rect(seed, x0, z0, w, h, thr, 221, 5000)
+    plain = slime_amd.scan_bands(seed, x0, z0, w, h, thr, 221, 0, slime_amd.MODE_PLAIN, 5000)
+    assert tuned[0] == even[0] == plain[0]
+    assert tuned[2] == even[2] == plain[2] and tuned[1] == even[1] == plain[1]

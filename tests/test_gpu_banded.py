@@ -116,6 +116,31 @@ def test_band_sets_vs_oracle(gpu, oracle, case):
     assert rows == want_rows
 
 
+@pytest.mark.parametrize("case", range(4))
+def test_one_phase_paired_bands_vs_oracle(gpu, oracle, case):
+    """One x-phase with its bands paired (bands 2k and 2k + 1 of a column have
+    opposite parity: one test decides both), counted by plane words (7 bands,
+    union bound; 11 bands, square bound) and by group windows (3 bands), with
+    an odd last band."""
+    import slime_amd
+    rng = random.Random(9090 + case)
+    seed = s64(rng.getrandbits(64))
+    nb, band_rows, w, mode = [(7, 10, 600, slime_amd.MODE_BANDED),
+                              (11, 6, P - 5, slime_amd.MODE_BANDED_SQUARE),
+                              (3, 24, 3000, slime_amd.MODE_BANDED),
+                              (8, 8, P, slime_amd.MODE_BANDED_UNION)][case]
+    x0 = rng.randint(-10 ** 8, 10 ** 8)
+    z0 = rng.randint(-10 ** 8, 10 ** 8)
+    h = (nb - 1) * P + band_rows
+    thr, k = 35, 500
+    print("case", case, seed, x0, z0, w, h, band_rows, flush=True)
+    total, rows, hist = slime_amd.scan_bands(seed, x0, z0, w, h, thr, 221, band_rows, mode, k)
+    want_total, want_hist, want_rows = oracle_bands(oracle, seed, x0, z0, w, h, band_rows, thr, k)
+    assert total == want_total
+    assert hist == want_hist
+    assert rows == want_rows
+
+
 @pytest.mark.parametrize("case", range(7))
 def test_three_phase_band_sets_vs_oracle(gpu, oracle, case):
     """Three x-phases (x, x + P, x + 2P; phase 2 paired across band pairs)

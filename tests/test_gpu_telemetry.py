@@ -20,8 +20,9 @@ def test_counters_after_searches(gpu):
     assert L.get_gpu_v1_algorithm() == 17            # banded, one x-phase (width < 2^17)
     assert L.get_gpu_v1_shape() == 3 and L.get_gpu_v34_shape() == 3
     # counters accumulate until reset_cancel, as in the reference
-    slime_amd.search_slime_clusters_centered(42, 70000, 60, 221, 0, 0, 0, 64)
-    assert L.get_processed_centers() == 2001 * 2001 + 140001 * 140001
+    # 2^18 - 1 columns: all but one slot column hold both phases
+    slime_amd.search_slime_clusters_centered(42, 131071, 60, 221, 0, 0, 0, 64)
+    assert L.get_processed_centers() == 2001 * 2001 + 262143 * 262143
     assert L.get_gpu_v1_algorithm() == 18            # two x-phases
     L.reset_cancel()
     assert L.get_processed_centers() == 0 and L.get_gpu_scan_work_ns() == 0

@@ -74,7 +74,7 @@ int main() {
         spec.z0 = (int32_t)((int64_t)(rng() % 2000000000ull) - 1000000000);
         std::vector<XPart> parts;
         const int layout = (int)(rng() % 4);
-        if (layout == 2 && spec.w > 2 * kPeriod && spec.w % (2 * kPeriod) != 0)
+        if (layout == 2 && spec.w > kPeriod && spec.w % (2 * kPeriod) != 0)
             parts = x_parts_one_phase_tail(spec);
         else if (layout == 3 && spec.w > 2 * kPeriod)
             parts = x_parts(spec, 3);
