@@ -110,52 +110,46 @@ constexpr int min_bands(int ph) { return ph == 3 ? kThreePhaseMinBands : 1; }
 struct SweepClass {
     int nb, ph;
 };
-constexpr int kSweepClasses = 15;
-constexpr SweepClass kSweepClass[kSweepClasses] = {{1, 1}, {2, 1}, {3, 1}, {4, 1}, {1, 2},
-                                                   {2, 2}, {3, 2}, {4, 2}, {6, 2}, {8, 2},
-                                                   {12, 2}, {2, 3}, {3, 3}, {4, 3}, {5, 3}};
-constexpr int kSweepMaxBands = 12;
+constexpr int kSweepClasses = 17;
+constexpr SweepClass kSweepClass[kSweepClasses] = {
+    {1, 1}, {2, 1}, {3, 1}, {4, 1}, {1, 2}, {2, 2}, {3, 2}, {4, 2}, {6, 2},
+    {8, 2}, {12, 2}, {2, 3}, {3, 3}, {4, 3}, {5, 3}, {7, 2}, {14, 2}};
 // Class index of (nb, ph), or -1.
 inline int sweep_class_of(int nb, int ph) {
     for (int c = 0; c < kSweepClasses; ++c)
         if (kSweepClass[c].nb == nb && kSweepClass[c].ph == ph) return c;
     return -1;
 }
-// Kernels: the square bound for the one- and two-phase classes (index =
-// class, classes 0..10); the union bound for those of at most
-// kUnionBoundMaxBands bands (index 11 + class, classes 0..9) and for the
-// three-phase classes, which have no square-bound kernel (index 21.., classes
-// 11..14).
-constexpr int kSweepSquareClasses = 11;
-constexpr int kSweepUnionClasses = 10;
-constexpr int kSweepThreeFirst = 11;
-constexpr int kSweepKernels =
-    kSweepSquareClasses + kSweepUnionClasses + (kSweepClasses - kSweepThreeFirst);
-static_assert(kSweepClass[kSweepUnionClasses - 1].nb <= kUnionBoundMaxBands &&
-                  kSweepClass[kSweepUnionClasses].nb > kUnionBoundMaxBands &&
-                  kSweepClass[kSweepThreeFirst].ph == 3 &&
-                  kSweepClass[kSweepThreeFirst - 1].ph == 2,
-              "sweep kernels: union bound within the band limit, three phases last");
+// The kernels, each a class with the square bound or the union bound: the
+// square bound for the one- and two-phase classes up to 12 bands; the union
+// bound for every class but (12, 2) before the wide classes (7, 12 and 14
+// bands, two phases: a served full-world request takes them as separate
+// calls do) were added; three-phase classes have the union bound only.
+struct SweepKernelDef {
+    int cls;
+    bool ub;
+};
+constexpr int kSweepKernels = 28;
+constexpr SweepKernelDef kSweepKernel[kSweepKernels] = {
+    {0, false},  {1, false},  {2, false}, {3, false}, {4, false}, {5, false}, {6, false},
+    {7, false},  {8, false},  {9, false}, {10, false}, {0, true}, {1, true},  {2, true},
+    {3, true},   {4, true},   {5, true},  {6, true},   {7, true}, {8, true},  {9, true},
+    {11, true},  {12, true},  {13, true}, {14, true},  {10, true}, {15, true}, {16, true}};
+constexpr int sweep_class_of_kernel(int k) { return kSweepKernel[k].cls; }
+constexpr bool sweep_kernel_union(int k) { return kSweepKernel[k].ub; }
+constexpr int sweep_kernel_find(int cls, bool ub) {
+    for (int k = 0; k < kSweepKernels; ++k)
+        if (kSweepKernel[k].cls == cls && kSweepKernel[k].ub == ub) return k;
+    return -1;
+}
 // Whether class c has a kernel with the union bound (ub) or the square bound.
-constexpr bool sweep_class_has(int c, bool ub) {
-    return ub ? (c < kSweepUnionClasses || c >= kSweepThreeFirst) : c < kSweepSquareClasses;
-}
-// The kernel of class c: with the union bound where the class has it, else
-// the square bound (-1 for a three-phase class without the union bound).
+constexpr bool sweep_class_has(int c, bool ub) { return sweep_kernel_find(c, ub) >= 0; }
+// The kernel of class c: with the union bound where asked and the class has
+// it, else the square bound (-1 for a class without either).
 inline int sweep_kernel_of(int cls, bool union_bound) {
-    if (cls >= kSweepThreeFirst)
-        return union_bound ? kSweepSquareClasses + kSweepUnionClasses + (cls - kSweepThreeFirst)
-                           : -1;
-    return union_bound && cls < kSweepUnionClasses ? kSweepSquareClasses + cls : cls;
+    const int k = union_bound ? sweep_kernel_find(cls, true) : -1;
+    return k >= 0 ? k : sweep_kernel_find(cls, false);
 }
-constexpr int sweep_class_of_kernel(int k) {
-    return k < kSweepSquareClasses                        ? k
-           : k < kSweepSquareClasses + kSweepUnionClasses ? k - kSweepSquareClasses
-                                                          : k - kSweepSquareClasses -
-                                                                kSweepUnionClasses +
-                                                                kSweepThreeFirst;
-}
-constexpr bool sweep_kernel_union(int k) { return k >= kSweepSquareClasses; }
 
 constexpr int32_t kReqActive = 1;  // request status: tiles may be claimed
 

@@ -57,13 +57,14 @@ __device__ bool claim_tile(const SweepArgs& a, int& pos, int& job, int& tile) {
 // (a runtime switch over all classes in one kernel spills).
 // Waves per SIMD each class is built for: the occupancy its per-search kernel
 // reaches with the same tile code (fewer bands need fewer registers).
-// The union-bound kernels (index kSweepSquareClasses + class) keep their
-// class's occupancy where the extra rolling state fits. The 2-band, two-phase
-// class runs at four waves in both forms (at five, with the claim loop around
-// the tile and, in the union-bound form, the matrix-core window sums, it
-// spilled). The three-phase kernels (union bound only) run at four.
-constexpr int kSweepWaves[kSweepKernels] = {6, 5, 5, 5, 5, 4, 4, 4, 4, 3, 3, 6, 5,
-                                            4, 4, 5, 4, 4, 4, 3, 3, 4, 4, 4, 4};
+// The union-bound kernels keep their class's occupancy where the extra
+// rolling state fits. The 2-band, two-phase class runs at four waves in both
+// forms (at five, with the claim loop around the tile and, in the union-bound
+// form, the matrix-core window sums, it spilled). The three-phase kernels
+// (union bound only) run at four, the 12- and 14-band union-bound kernels at
+// two (their one-tile form needs up to 167 VGPRs before the claim loop).
+constexpr int kSweepWaves[kSweepKernels] = {6, 5, 5, 5, 5, 4, 4, 4, 4, 3, 3, 6, 5, 4,
+                                            4, 5, 4, 4, 4, 3, 3, 4, 4, 4, 4, 2, 3, 2};
 
 template <int K>
 __global__ __launch_bounds__(kThreads, kSweepWaves[K]) void sweep_kernel(SweepArgs a) {
@@ -189,6 +190,9 @@ int sweep_blocks_per_cu(int kernel) {
         case 22: return blocks_per_cu<22>();
         case 23: return blocks_per_cu<23>();
         case 24: return blocks_per_cu<24>();
+        case 25: return blocks_per_cu<25>();
+        case 26: return blocks_per_cu<26>();
+        case 27: return blocks_per_cu<27>();
         default: return 1;
     }
 }
@@ -221,10 +225,13 @@ hipError_t launch_sweep(const SweepArgs& a, int kernel, int grid, hipStream_t st
         case 22: return launch_class<22>(a, grid, stream);
         case 23: return launch_class<23>(a, grid, stream);
         case 24: return launch_class<24>(a, grid, stream);
+        case 25: return launch_class<25>(a, grid, stream);
+        case 26: return launch_class<26>(a, grid, stream);
+        case 27: return launch_class<27>(a, grid, stream);
         default: return hipErrorInvalidValue;
     }
 }
-static_assert(kSweepKernels == 25, "one case per kernel");
+static_assert(kSweepKernels == 28, "one case per kernel");
 
 hipError_t launch_batch(const BatchArgs& a, int kernel, int64_t blocks, hipStream_t stream) {
     if (a.n <= 0 || blocks <= 0) return hipSuccess;
@@ -255,6 +262,9 @@ hipError_t launch_batch(const BatchArgs& a, int kernel, int64_t blocks, hipStrea
         case 22: return launch_batch_kernel<22>(a, blocks, stream);
         case 23: return launch_batch_kernel<23>(a, blocks, stream);
         case 24: return launch_batch_kernel<24>(a, blocks, stream);
+        case 25: return launch_batch_kernel<25>(a, blocks, stream);
+        case 26: return launch_batch_kernel<26>(a, blocks, stream);
+        case 27: return launch_batch_kernel<27>(a, blocks, stream);
         default: return hipErrorInvalidValue;
     }
 }
