@@ -18,9 +18,11 @@ workload on this host's cores.
 Two more legs ride in the same line:
   batched  256 searches from a splitmix64 stream (radius log-uniform in
            [256, 200000], centers within +-1.8M chunks, thresholds 50..62),
-           run as one sweep (persistent kernels, optionally replayed from a
-           hipGraph) against 256 separate calls of the same library and 256
-           calls of its plain per-chunk kernel; all results must be identical.
+           run as one sweep (one launch per kernel class, optionally replayed
+           from a hipGraph, or the persistent kernels) against 256 separate
+           calls of the same library and 256 calls of its plain per-chunk
+           kernel; all results must be identical. A secondary leg,
+           batched_small, does the same for 256 searches of radius 256..4096.
            Their top sites (up to 1000 per search) then go through the biome
            filter (Deep Dark / Mushroom Fields under the sites' slime chunks,
            Minecraft 26.2) in one launch, against one context per world, and
@@ -31,7 +33,9 @@ Two more legs ride in the same line:
            radius 50,000, 5% full world; thresholds 50..65; 10% pause then
            resume, 5% cancel) into one session, against the same arrivals
            served one search at a time in arrival order; p50/p99 latency and
-           aggregate T/s of both, results identical.
+           aggregate T/s of both, results identical. A secondary leg,
+           served_small, serves 2000 requests of the small kind alone at
+           20,000 arrivals per second, with no pause or cancel.
 """
 import argparse
 import ctypes as C
@@ -60,13 +64,13 @@ VALU_PEAK_T = 157.3 / 2
 # VALU lane instructions per center of the full-world search's banded kernels,
 # counted by rocprofv3 (SQ_INSTS_VALU x 64 / centers over all launches;
 # profiles/scan_band_kernel_full_world_thr60.md).
-VALU_INSTS_PER_CENTER = 4.61
+VALU_INSTS_PER_CENTER = 4.59
 CPU_SAMPLE_RADIUS = 131072
 # HBM bytes per scan_band_kernel launch (rocprofv3 FETCH_SIZE + WRITE_SIZE of
 # the full-world run, scaled by the same counters on a copy of known size;
-# profiles/scan_band_kernel_full_world_thr60.md): 105.0 KiB x 2.0 + 0.02 KiB x 1.0
-# per launch, averaged over the search's 31 launches.
-TRAFFIC_BYTES = 215000
+# profiles/scan_band_kernel_full_world_thr60.md): 96.01 KiB x 2.0 + 0.02 KiB x 1.0
+# per launch, averaged over the search's 32 launches.
+TRAFFIC_BYTES = 196660
 
 
 def host_cpu():
@@ -279,6 +283,39 @@ def batched_leg(repeats):
     return out
 
 
+def batched_small_leg():
+    """A batch of small searches only (radius log-uniform in [256, 4096], the
+    same stream and thresholds): one sweep against separate calls, where the
+    separate calls leave most of the device idle."""
+    import slime_amd
+    from slime_amd.workloads import batched_searches
+    searches = batched_searches(256, r_hi=4096)
+    cols = list(zip(*searches))
+    k = 1000
+    centers = sum((2 * s[1] + 1) ** 2 for s in searches)
+    res = slime_amd.search_batch(*cols, max_results=k)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        got = slime_amd.search_batch(*cols, max_results=k)
+        times.append(time.perf_counter() - t0)
+        if got != res:
+            raise SystemExit("batched results differ between runs")
+    dt = min(times)
+    st = slime_amd.batch_stats()
+    t0 = time.perf_counter()
+    per = [slime_amd.search_slime_clusters_gpu_v1_centered(*s, max_results=k) for s in searches]
+    dp = time.perf_counter() - t0
+    print(f"batched small: {dt:.4f} s, per-search {dp:.4f} s", file=sys.stderr, flush=True)
+    return {"workload": "256 searches, radius log-uniform [256, 4096], thr 50..62",
+            "centers": centers,
+            "sweep": {"seconds": dt, "t_centers_per_s": centers / dt / 1e12,
+                      "kernel_s": st["kernel_ns"] / 1e9, "prepare_s": st["prepare_ns"] / 1e9,
+                      "sweep_s": st["sweep_ns"] / 1e9, "wall_s": st["wall_ns"] / 1e9},
+            "per_search": {"seconds": dp, "t_centers_per_s": centers / dp / 1e12},
+            "speedup_vs_per_search": dp / dt, "identical": per == res}
+
+
 FP64_PEAK_TF = 78.6          # MI355X dense FP64 vector peak
 FLOPS_PER_COLUMN = 52 * 60   # 52 double Perlin octave samples of ~60 FP64 ops
 # FP64 add and multiply wave instructions of one filter launch over these
@@ -456,20 +493,25 @@ def latency_stats(lat, reqs, idx):
             "small_p50_s": percentile(small, 50), "small_p99_s": percentile(small, 99)}
 
 
-def served_leg(n, rate, with_plain=False):
+def served_leg(n, rate, with_plain=False, small_only=False):
     """The served configuration: one session vs one search at a time through
     the same library's per-search engine (optionally also through the plain
     per-chunk kernel). At the default rate requests arrive faster than the
-    device finishes them, so both forms are device-bound."""
+    device finishes them, so both forms are device-bound. small_only: every
+    request of the mix's small kind (radius 256..4,096)."""
     import threading
     import slime_amd
     from slime_amd.workloads import served_requests, SERVED_STATE
-    reqs = served_requests(n, rate)
+    reqs = served_requests(n, rate, small_only=small_only)
+    if small_only:  # batching alone: no pauses or cancels
+        for r in reqs:
+            r["action"] = None
     k = 1000
     pause_s, cancel_s = 0.005, 0.005
     # served: continuous batching
     lat, results = {}, {}
-    with slime_amd.Session(hbm_budget=16 << 30, max_active=64, round_ms=20) as sess:
+    with slime_amd.Session(hbm_budget=16 << 30, max_active=256 if small_only else 64,
+                           round_ms=20) as sess:
         ids = [None] * n
         t_start = time.perf_counter()
 
@@ -543,7 +585,9 @@ def served_leg(n, rate, with_plain=False):
               "kernel_s": stats["kernel_ns"] / 1e9, "rounds": stats["rounds"]}
     served.update(latency_stats(lat, reqs, list(lat)))
     out = {
-        "workload": "Poisson arrivals: 70% radius 256..4096, 25% radius 50000, 5% full world; "
+        "workload": ("Poisson arrivals: radius 256..4096 only (the served mix's small kind); "
+                     "thr 50..65; no pause or cancel") if small_only else
+                    "Poisson arrivals: 70% radius 256..4096, 25% radius 50000, 5% full world; "
                     "thr 50..65; 10% pause+resume, 5% cancel",
         "state": hex(SERVED_STATE), "requests": n, "rate_per_s": rate, "completed": len(done),
         "centers": centers, "last_arrival_s": reqs[-1]["arrival"],
@@ -773,8 +817,13 @@ def main():
         out["c1"] = c1_leg()
     if not args.no_batched:
         out["batched"] = batched_leg(2)
+        # secondary: where separate calls cannot fill the device
+        out["batched_small"] = batched_small_leg()
     if not args.no_served:
         out["served"] = served_leg(args.served_requests, args.served_rate, args.served_plain)
+        # secondary: the mix's small requests alone, arriving faster than
+        # either form serves them
+        out["served_small"] = served_leg(2000, 20000.0, small_only=True)
     print(json.dumps(out), flush=True)
     return 0
 

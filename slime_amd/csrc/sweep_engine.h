@@ -99,10 +99,28 @@ class Sweeper {
     bool round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& pol, RoundStats* st);
     double progress(const SweepRequest& r) const;
     bool finalize(SweepRequest& r, bool partial);
+    // Many finished requests: their counts in one copy, their stored hits in
+    // one more round trip (device top-K selections one request at a time).
+    bool finalize_all(const std::vector<SweepRequest*>& rs, bool partial);
 
    private:
     bool ensure_fix_cap(size_t n);
     bool prepare_fixes(const std::vector<SweepRequest*>& reqs);
+    // Ranking of a finished request, in three steps: the corrected histogram,
+    // total and threshold from its counts (or the rerun alone of an
+    // overflowed region); its stored hits (or the device selection of those
+    // that can reach the top K); the ranked rows.
+    struct RankState {
+        bool done = false;  // rerun alone: rows final
+        size_t stored = 0;
+        int32_t t = 0;
+        bool select = false;
+        std::vector<Hit> cand;
+    };
+    bool rank_begin(SweepRequest& r, bool partial, const unsigned long long* hist,
+                    unsigned long long emitted, RankState& rs);
+    bool rank_hits(SweepRequest& r, const RankState& rs, std::vector<Hit>& hits);
+    void rank_end(SweepRequest& r, RankState& rs, const std::vector<Hit>& hits);
 
     int device_ = -1;
     int cus_ = 256;

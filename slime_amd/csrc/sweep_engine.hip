@@ -547,9 +547,34 @@ bool Sweeper::prepare_fixes(const std::vector<SweepRequest*>& reqs) {
 }
 
 bool Sweeper::upload(const std::vector<SweepRequest*>& reqs) {
+    // Jobs, cleared cursors and outputs and the active status of the new
+    // requests, written per run of adjacent job ranges and of adjacent slots
+    // (a batch's requests form one run each way): a few copies and fill
+    // kernels instead of several calls per request.
+    std::vector<SweepRequest*> by_job(reqs), by_slot(reqs);
+    std::sort(by_job.begin(), by_job.end(),
+              [](const SweepRequest* a, const SweepRequest* b) { return a->job_off < b->job_off; });
+    std::sort(by_slot.begin(), by_slot.end(),
+              [](const SweepRequest* a, const SweepRequest* b) { return a->slot < b->slot; });
     std::vector<SweepJob> staged;
-    for (SweepRequest* r : reqs) {
+    int64_t run_off = 0;
+    auto flush_jobs = [&]() -> bool {
+        const size_t nj = staged.size();
+        if (!nj) return true;
+        const bool ok =
+            hipMemcpyAsync(d_jobs_ + run_off, staged.data(), sizeof(SweepJob) * nj,
+                           hipMemcpyHostToDevice, stream_) == hipSuccess &&
+            launch_fill_u32(d_cursor_ + run_off, nj, 0u, stream_) == hipSuccess &&
+            launch_fill_u32(d_done_ + run_off, nj, 0u, stream_) == hipSuccess &&
+            hipStreamSynchronize(stream_) == hipSuccess;  // `staged` is reused
         staged.clear();
+        return ok;
+    };
+    for (SweepRequest* r : by_job) {
+        if (r->jobs.empty()) continue;
+        if (!staged.empty() && r->job_off != run_off + (int64_t)staged.size() && !flush_jobs())
+            return false;
+        if (staged.empty()) run_off = r->job_off;
         for (size_t j = 0; j < r->jobs.size(); ++j) {
             SweepJob sj{};
             sj.p = r->jobs[j].p;
@@ -562,25 +587,33 @@ bool Sweeper::upload(const std::vector<SweepRequest*>& reqs) {
             sj.p.hist = d_hist_ + (size_t)r->slot * kHistBins;
             staged.push_back(sj);
         }
-        const size_t nj = staged.size();
-        if ((nj && (hipMemcpyAsync(d_jobs_ + r->job_off, staged.data(), sizeof(SweepJob) * nj,
-                                   hipMemcpyHostToDevice, stream_) != hipSuccess ||
-                    hipMemsetAsync(d_cursor_ + r->job_off, 0, sizeof(uint32_t) * nj, stream_) !=
-                        hipSuccess ||
-                    hipMemsetAsync(d_done_ + r->job_off, 0, sizeof(uint32_t) * nj, stream_) !=
-                        hipSuccess)) ||
-            hipMemsetAsync(d_hist_ + (size_t)r->slot * kHistBins, 0,
-                           sizeof(unsigned long long) * kHistBins, stream_) != hipSuccess ||
-            hipMemsetAsync(d_emitted_ + r->slot, 0, sizeof(unsigned long long), stream_) !=
-                hipSuccess)
-            return false;
-        r->status = kReqActive;
-        if (hipMemcpyAsync(d_status_ + r->slot, &r->status, sizeof(int32_t),
-                           hipMemcpyHostToDevice, stream_) != hipSuccess)
-            return false;
-        r->remaining_ns = r->est_ns;
     }
-    return hipStreamSynchronize(stream_) == hipSuccess;
+    if (!flush_jobs()) return false;
+    std::vector<int32_t> status;
+    int32_t slot0 = 0;
+    auto flush_slots = [&]() -> bool {
+        const size_t ns = status.size();
+        if (!ns) return true;
+        const bool ok =
+            launch_fill_u32(reinterpret_cast<uint32_t*>(d_hist_ + (size_t)slot0 * kHistBins),
+                            2 * (size_t)kHistBins * ns, 0u, stream_) == hipSuccess &&
+            launch_fill_u32(reinterpret_cast<uint32_t*>(d_emitted_ + slot0), 2 * ns, 0u,
+                            stream_) == hipSuccess &&
+            hipMemcpyAsync(d_status_ + slot0, status.data(), sizeof(int32_t) * ns,
+                           hipMemcpyHostToDevice, stream_) == hipSuccess &&
+            hipStreamSynchronize(stream_) == hipSuccess;  // `status` is reused
+        status.clear();
+        return ok;
+    };
+    for (SweepRequest* r : by_slot) {
+        if (!status.empty() && r->slot != slot0 + (int32_t)status.size() && !flush_slots())
+            return false;
+        if (status.empty()) slot0 = r->slot;
+        r->status = kReqActive;
+        r->remaining_ns = r->est_ns;
+        status.push_back(kReqActive);
+    }
+    return flush_slots();
 }
 
 bool Sweeper::clear_status() {
@@ -802,31 +835,24 @@ double Sweeper::progress(const SweepRequest& r) const {
     return total ? (double)done / (double)total : 1.0;
 }
 
-bool Sweeper::finalize(SweepRequest& r, bool partial) {
-    unsigned long long hist[kHistBins];
-    unsigned long long emitted = 0;
-    if (hipMemcpyAsync(hist, d_hist_ + (size_t)r.slot * kHistBins, sizeof(hist),
-                       hipMemcpyDeviceToHost, stream_) != hipSuccess ||
-        hipMemcpyAsync(&emitted, d_emitted_ + r.slot, sizeof(emitted), hipMemcpyDeviceToHost,
-                       stream_) != hipSuccess ||
-        hipStreamSynchronize(stream_) != hipSuccess)
-        return false;
+bool Sweeper::rank_begin(SweepRequest& r, bool partial, const unsigned long long* hist,
+                         unsigned long long emitted, RankState& rs) {
     if ((int64_t)emitted > r.hit_cap && !partial) {
         // more hits than the reserved region: rerun alone with a rising gate
         r.rows.assign((size_t)r.keep, ResultRow{});
         r.total = run_search(r.spec, r.rows.data(), r.keep);
         r.rows.resize((size_t)std::max<int64_t>(0, std::min<int64_t>(r.total, r.keep)));
         r.fallback = true;
+        rs.done = true;
         return r.total >= 0;
     }
     // A cancelled request keeps what its finished tiles found: the total
     // counts every decided center; the rows rank the stored hits (the first
     // hit_cap emitted if the region overflowed: then r.truncated).
-    const size_t stored = (size_t)std::min<int64_t>((int64_t)emitted, r.hit_cap);
+    rs.stored = (size_t)std::min<int64_t>((int64_t)emitted, r.hit_cap);
     r.truncated = (int64_t)emitted > r.hit_cap;
     int64_t h[kHistBins];
     for (int s = 0; s < kHistBins; ++s) h[s] = (int64_t)hist[s];
-    std::vector<Hit> cand;
     // Replace the provisional first-draw counts of recounted centers: all of
     // them, or for a cancelled request those in finished tiles (every job's
     // tiles finish in index order, so its first done[j] tiles).
@@ -843,7 +869,7 @@ bool Sweeper::finalize(SweepRequest& r, bool partial) {
         if (f.first_draw >= r.spec.min_size && f.first_draw <= r.spec.max_size) h[f.first_draw]--;
         if (f.exact >= r.spec.min_size && f.exact <= r.spec.max_size) {
             h[f.exact]++;
-            cand.push_back(Hit{f.exact, f.cx, f.cz, 0});
+            rs.cand.push_back(Hit{f.exact, f.cx, f.cz, 0});
         }
     }
     int64_t total = 0;
@@ -855,48 +881,107 @@ bool Sweeper::finalize(SweepRequest& r, bool partial) {
     // Java-rejected chunk are dropped below and replaced by the fixes, which
     // the histogram already counts). A truncated region, or more such hits
     // than the staging buffer holds, brings every stored hit back.
-    int32_t t = r.spec.min_size;
+    rs.t = r.spec.min_size;
     for (int64_t s = kHistBins - 1, acc = 0; s > r.spec.min_size; --s) {
         acc += h[s];
         if (acc >= (int64_t)r.keep) {
-            t = (int32_t)s;
+            rs.t = (int32_t)s;
             break;
         }
     }
-    std::vector<Hit> hits;
-    bool selected = false;
-    if (!r.truncated && t > r.spec.min_size && stored > (size_t)r.keep) {
+    rs.select = !r.truncated && rs.t > r.spec.min_size && rs.stored > (size_t)r.keep;
+    return true;
+}
+
+bool Sweeper::rank_hits(SweepRequest& r, const RankState& rs, std::vector<Hit>& hits) {
+    if (rs.select) {
         unsigned int n_sel = 0;
         if (launch_fill_u32(d_select_count_, 1, 0u, stream_) != hipSuccess ||
-            launch_select_hits(d_hits_ + r.hit_off, (int64_t)stored, t, d_select_, kSelectCap,
-                               d_select_count_, stream_) != hipSuccess ||
+            launch_select_hits(d_hits_ + r.hit_off, (int64_t)rs.stored, rs.t, d_select_,
+                               kSelectCap, d_select_count_, stream_) != hipSuccess ||
             hipMemcpyAsync(&n_sel, d_select_count_, sizeof(n_sel), hipMemcpyDeviceToHost,
                            stream_) != hipSuccess ||
             hipStreamSynchronize(stream_) != hipSuccess)
             return false;
         if (n_sel <= kSelectCap) {
             hits.resize(n_sel);
-            if (n_sel && (hipMemcpyAsync(hits.data(), d_select_, sizeof(Hit) * n_sel,
-                                         hipMemcpyDeviceToHost, stream_) != hipSuccess ||
-                          hipStreamSynchronize(stream_) != hipSuccess))
-                return false;
-            selected = true;
+            return !n_sel || (hipMemcpyAsync(hits.data(), d_select_, sizeof(Hit) * n_sel,
+                                             hipMemcpyDeviceToHost, stream_) == hipSuccess &&
+                              hipStreamSynchronize(stream_) == hipSuccess);
         }
     }
-    if (!selected) {
-        hits.resize(stored);
-        if (stored && (hipMemcpyAsync(hits.data(), d_hits_ + r.hit_off, sizeof(Hit) * stored,
-                                      hipMemcpyDeviceToHost, stream_) != hipSuccess ||
-                       hipStreamSynchronize(stream_) != hipSuccess))
-            return false;
-    }
-    r.hits_stored = (int64_t)stored;
+    hits.resize(rs.stored);
+    return !rs.stored || (hipMemcpyAsync(hits.data(), d_hits_ + r.hit_off, sizeof(Hit) * rs.stored,
+                                         hipMemcpyDeviceToHost, stream_) == hipSuccess &&
+                          hipStreamSynchronize(stream_) == hipSuccess);
+}
+
+void Sweeper::rank_end(SweepRequest& r, RankState& rs, const std::vector<Hit>& hits) {
+    r.hits_stored = (int64_t)rs.stored;
     r.hits_copied = (int64_t)hits.size();
+    std::vector<Hit>& cand = rs.cand;
     cand.reserve(cand.size() + hits.size());
     for (const Hit& x : hits)
         if (!r.mm.affects(x.cx, x.cz)) cand.push_back(Hit{x.size, x.cx, x.cz, 0});
     r.rows.assign((size_t)std::min<int64_t>((int64_t)cand.size(), r.keep), ResultRow{});
     write_rows(cand, (size_t)r.keep, r.rows.data());
+}
+
+bool Sweeper::finalize(SweepRequest& r, bool partial) {
+    unsigned long long hist[kHistBins];
+    unsigned long long emitted = 0;
+    if (hipMemcpyAsync(hist, d_hist_ + (size_t)r.slot * kHistBins, sizeof(hist),
+                       hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+        hipMemcpyAsync(&emitted, d_emitted_ + r.slot, sizeof(emitted), hipMemcpyDeviceToHost,
+                       stream_) != hipSuccess ||
+        hipStreamSynchronize(stream_) != hipSuccess)
+        return false;
+    RankState rs;
+    if (!rank_begin(r, partial, hist, emitted, rs)) return false;
+    if (rs.done) return true;
+    std::vector<Hit> hits;
+    if (!rank_hits(r, rs, hits)) return false;
+    rank_end(r, rs, hits);
+    return true;
+}
+
+bool Sweeper::finalize_all(const std::vector<SweepRequest*>& rs_in, bool partial) {
+    if (rs_in.size() <= 1) {
+        for (SweepRequest* r : rs_in)
+            if (!finalize(*r, partial)) return false;
+        return true;
+    }
+    // every slot's counts in two copies, one synchronisation
+    std::vector<unsigned long long> hist((size_t)req_cap_ * kHistBins), emitted((size_t)req_cap_);
+    if (hipMemcpyAsync(hist.data(), d_hist_, sizeof(unsigned long long) * hist.size(),
+                       hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+        hipMemcpyAsync(emitted.data(), d_emitted_, sizeof(unsigned long long) * emitted.size(),
+                       hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+        hipStreamSynchronize(stream_) != hipSuccess)
+        return false;
+    std::vector<RankState> st(rs_in.size());
+    std::vector<std::vector<Hit>> hits(rs_in.size());
+    bool copies = false;
+    for (size_t i = 0; i < rs_in.size(); ++i) {
+        SweepRequest& r = *rs_in[i];
+        if (!rank_begin(r, partial, hist.data() + (size_t)r.slot * kHistBins, emitted[r.slot],
+                        st[i]))
+            return false;
+        if (st[i].done) continue;
+        if (st[i].select) {
+            if (!rank_hits(r, st[i], hits[i])) return false;  // selections one at a time
+        } else if (st[i].stored) {
+            // the stored hits of every other request in one more round trip
+            hits[i].resize(st[i].stored);
+            if (hipMemcpyAsync(hits[i].data(), d_hits_ + r.hit_off, sizeof(Hit) * st[i].stored,
+                               hipMemcpyDeviceToHost, stream_) != hipSuccess)
+                return false;
+            copies = true;
+        }
+    }
+    if (copies && hipStreamSynchronize(stream_) != hipSuccess) return false;
+    for (size_t i = 0; i < rs_in.size(); ++i)
+        if (!st[i].done) rank_end(*rs_in[i], st[i], hits[i]);
     return true;
 }
 
@@ -970,12 +1055,17 @@ int run_batch(const SearchSpec* specs, int32_t n, int32_t max_results, ResultRow
         cleanup();
         return fail();
     }
-    for (int32_t i = 0; i < n; ++i) {
-        SweepRequest& r = reqs[i];
-        if (!r.complete || !sw.finalize(r, false)) {
+    for (SweepRequest* r : ptrs)
+        if (!r->complete) {
             cleanup();
             return fail();
         }
+    if (!sw.finalize_all(ptrs, false)) {
+        cleanup();
+        return fail();
+    }
+    for (int32_t i = 0; i < n; ++i) {
+        SweepRequest& r = reqs[i];
         totals[i] = r.total;
         std::memcpy(out + (size_t)i * max_results, r.rows.data(),
                     sizeof(ResultRow) * r.rows.size());
@@ -1306,10 +1396,10 @@ void Session::worker() {
             ok = sw_.round(running, pol, &rs);
         }
         // finish completed and cancelled requests
-        for (int64_t k = 0; ok && k < (int64_t)running.size(); ++k) {
-            SweepRequest* r = running[k];
-            if (r->complete) ok = sw_.finalize(*r, false);
-        }
+        std::vector<SweepRequest*> complete;
+        for (SweepRequest* r : running)
+            if (r->complete) complete.push_back(r);
+        if (ok) ok = sw_.finalize_all(complete, false);
         for (Entry* e : cancelled) ok = ok && sw_.finalize(e->req, true);
 
         lk.lock();

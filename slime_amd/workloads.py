@@ -54,20 +54,20 @@ def batched_searches(n=256, state=BATCH_STATE, r_lo=256, r_hi=200_000, center=1_
     return out
 
 
-def served_requests(n, rate_per_s, state=SERVED_STATE, full_world=True):
+def served_requests(n, rate_per_s, state=SERVED_STATE, full_world=True, small_only=False):
     """n requests of the served mix with Poisson arrivals: dicts with the
     search, its arrival time (s) and whether it pauses (then resumes) or
     cancels. Mix: 70% radius uniform in [256, 4096], 25% radius 50,000,
     5% full world (radius 1,875,000 around 0,0; with full_world False the 5%
-    are radius 200,000); thresholds uniform in [50, 65]; 10% pause/resume,
-    5% cancel."""
+    are radius 200,000; with small_only every request is of the first kind);
+    thresholds uniform in [50, 65]; 10% pause/resume, 5% cancel."""
     g = SplitMix64(state)
     t = 0.0
     out = []
     for i in range(n):
         t += -math.log(1.0 - g.uniform()) / rate_per_s
         u = g.uniform()
-        if u < 0.70:
+        if u < 0.70 or small_only:
             rd = g.randint(256, 4096)
             cx, cz = g.randint(-1_800_000, 1_800_000), g.randint(-1_800_000, 1_800_000)
         elif u < 0.95:
