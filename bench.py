@@ -35,7 +35,7 @@ Two more legs ride in the same line:
            served one search at a time in arrival order; p50/p99 latency and
            aggregate T/s of both, results identical. A secondary leg,
            served_small, serves 2000 requests of the small kind alone at
-           20,000 arrivals per second, with no pause or cancel.
+           200,000 arrivals per second, with no pause or cancel.
 """
 import argparse
 import ctypes as C
@@ -823,7 +823,7 @@ def main():
         out["served"] = served_leg(args.served_requests, args.served_rate, args.served_plain)
         # secondary: the mix's small requests alone, arriving faster than
         # either form serves them
-        out["served_small"] = served_leg(2000, 20000.0, small_only=True)
+        out["served_small"] = served_leg(2000, 200000.0, small_only=True)
     print(json.dumps(out), flush=True)
     return 0
 
