@@ -117,6 +117,10 @@ struct Buffers {
     int device = -1;
     int cus = 256;  // compute units of the device
     hipStream_t stream = nullptr;
+    // the first-draw mismatch search and recount, while the first scan
+    // launch runs on `stream` (default priority, like `stream`)
+    hipStream_t aux = nullptr;
+    unsigned int* h_mm_count = nullptr;  // pinned
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     Hit* d_hits = nullptr;
     unsigned long long* d_counters = nullptr;  // [0] emitted, [1..222] histogram
@@ -151,6 +155,8 @@ void free_buffers() {
     if (g_buf.ev_start) (void)hipEventDestroy(g_buf.ev_start);
     if (g_buf.ev_stop) (void)hipEventDestroy(g_buf.ev_stop);
     if (g_buf.stream) (void)hipStreamDestroy(g_buf.stream);
+    if (g_buf.aux) (void)hipStreamDestroy(g_buf.aux);
+    if (g_buf.h_mm_count) (void)hipHostFree(g_buf.h_mm_count);
     g_fixes.release();
     g_buf = Buffers{};
 }
@@ -168,6 +174,9 @@ bool ensure_buffers() {
     const std::vector<uint64_t>& table = mismatch_table();
     g_buf.table_n = (int32_t)table.size();
     if (hipStreamCreateWithFlags(&g_buf.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&g_buf.aux, hipStreamNonBlocking) != hipSuccess ||
+        hipHostMalloc(&g_buf.h_mm_count, sizeof(unsigned int), hipHostMallocDefault) !=
+            hipSuccess ||
         hipEventCreate(&g_buf.ev_start) != hipSuccess ||
         hipEventCreate(&g_buf.ev_stop) != hipSuccess ||
         hipMalloc(&g_buf.d_hits, sizeof(Hit) * (size_t)kHitCap) != hipSuccess ||
@@ -253,7 +262,10 @@ struct Work {
 
 enum class RunResult { kOk, kSplit, kError };
 
-RunResult run_work(Work& wk, Ranker& rk) {
+// while_running (if set) runs once, after the first launch is queued and
+// before the host waits for it (host work that the launch hides).
+RunResult run_work(Work& wk, Ranker& rk, const std::function<bool()>& while_running = {}) {
+    bool ran = false;
     // The first hits come back with the counters (no second round trip when a
     // launch emits at most this many); the rest are copied after the count is
     // known.
@@ -288,9 +300,13 @@ RunResult run_work(Work& wk, Ranker& rk) {
                            sizeof(unsigned long long) * (1 + kHistBins), hipMemcpyDeviceToHost,
                            g_buf.stream) != hipSuccess ||
             hipMemcpyAsync(g_buf.h_hits, g_buf.d_hits, sizeof(Hit) * eager,
-                           hipMemcpyDeviceToHost, g_buf.stream) != hipSuccess ||
-            hipStreamSynchronize(g_buf.stream) != hipSuccess)
+                           hipMemcpyDeviceToHost, g_buf.stream) != hipSuccess)
             return RunResult::kError;
+        if (while_running && !ran) {
+            ran = true;
+            if (!while_running()) return RunResult::kError;
+        }
+        if (hipStreamSynchronize(g_buf.stream) != hipSuccess) return RunResult::kError;
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, g_buf.ev_start, g_buf.ev_stop) == hipSuccess) {
             std::atomic<int64_t>* st = g_kstat[wk.banded ? 1 : 0];
@@ -709,27 +725,30 @@ void plan_banded(const SearchSpec& spec, std::deque<Work>& out) {
 }
 
 // Finds the first-draw mismatches of the region and recounts every affected
-// center exactly. Returns false on a device error.
-bool prepare_fixes(const SearchSpec& spec, MismatchIndex& index, Ranker& rk, FixList& fixes) {
-    if (launch_fill_u32(g_buf.d_mismatch_count, 1, 0u, g_buf.stream) != hipSuccess ||
-        launch_find_mismatches(spec.seed, (int32_t)((int64_t)spec.x0 - 8), spec.w + 16,
-                               (int32_t)((int64_t)spec.z0 - 8), spec.h + 16, g_buf.d_table,
-                               g_buf.table_n, g_buf.d_mismatch, kMismatchCap,
-                               g_buf.d_mismatch_count, g_buf.stream) != hipSuccess)
-        return false;
-    // the count comes back through the pinned counter buffer (free until the
-    // first scan launch)
-    unsigned int count = 0;
-    if (hipMemcpyAsync(g_buf.h_counters, g_buf.d_mismatch_count, sizeof(count),
-                       hipMemcpyDeviceToHost, g_buf.stream) != hipSuccess ||
-        hipStreamSynchronize(g_buf.stream) != hipSuccess)
-        return false;
-    std::memcpy(&count, g_buf.h_counters, sizeof(count));
+// center exactly, in two halves on the aux stream: start_fixes queues the
+// mismatch search, finish_fixes waits for it, collects the affected centers
+// on the host, recounts them on the device and waits. The scan launches do
+// not need them (they count first draws), so the first one runs meanwhile.
+// Returns false on a device error.
+bool start_fixes(const SearchSpec& spec) {
+    return launch_fill_u32(g_buf.d_mismatch_count, 1, 0u, g_buf.aux) == hipSuccess &&
+           launch_find_mismatches(spec.seed, (int32_t)((int64_t)spec.x0 - 8), spec.w + 16,
+                                  (int32_t)((int64_t)spec.z0 - 8), spec.h + 16, g_buf.d_table,
+                                  g_buf.table_n, g_buf.d_mismatch, kMismatchCap,
+                                  g_buf.d_mismatch_count, g_buf.aux) == hipSuccess &&
+           hipMemcpyAsync(g_buf.h_mm_count, g_buf.d_mismatch_count, sizeof(unsigned int),
+                          hipMemcpyDeviceToHost, g_buf.aux) == hipSuccess;
+}
+
+bool finish_fixes(const SearchSpec& spec, MismatchIndex& index, Ranker& rk, FixList& fixes) {
+    if (hipStreamSynchronize(g_buf.aux) != hipSuccess) return false;
+    const unsigned int count = *g_buf.h_mm_count;
     if (count > kMismatchCap) return false;
     std::vector<MismatchChunk>& mm = index.mm;
     mm.resize(count);
-    if (count && (hipMemcpy(mm.data(), g_buf.d_mismatch, sizeof(MismatchChunk) * count,
-                            hipMemcpyDeviceToHost) != hipSuccess))
+    if (count && (hipMemcpyAsync(mm.data(), g_buf.d_mismatch, sizeof(MismatchChunk) * count,
+                                 hipMemcpyDeviceToHost, g_buf.aux) != hipSuccess ||
+                  hipStreamSynchronize(g_buf.aux) != hipSuccess))
         return false;
     index.sort();
     fixes.clear();
@@ -740,11 +759,11 @@ bool prepare_fixes(const SearchSpec& spec, MismatchIndex& index, Ranker& rk, Fix
     if (fixes.empty()) return true;
     if (!ensure_fixes(fixes.size()) ||
         hipMemcpyAsync(g_buf.d_fixes, fixes.data(), sizeof(CenterFix) * fixes.size(),
-                       hipMemcpyHostToDevice, g_buf.stream) != hipSuccess ||
-        launch_recount(g_buf.d_fixes, (int32_t)fixes.size(), g_buf.stream) != hipSuccess ||
+                       hipMemcpyHostToDevice, g_buf.aux) != hipSuccess ||
+        launch_recount(g_buf.d_fixes, (int32_t)fixes.size(), g_buf.aux) != hipSuccess ||
         hipMemcpyAsync(fixes.data(), g_buf.d_fixes, sizeof(CenterFix) * fixes.size(),
-                       hipMemcpyDeviceToHost, g_buf.stream) != hipSuccess ||
-        hipStreamSynchronize(g_buf.stream) != hipSuccess)
+                       hipMemcpyDeviceToHost, g_buf.aux) != hipSuccess ||
+        hipStreamSynchronize(g_buf.aux) != hipSuccess)
         return false;
     for (const CenterFix& f : fixes)
         if (f.first_draw >= spec.min_size && f.first_draw <= spec.max_size) rk.pending[f.first_draw]++;
@@ -808,7 +827,12 @@ int64_t run_search(const SearchSpec& spec, ResultRow* out, int32_t max_results,
     std::deque<Work> works;
     FixList& fixes = g_fixes;
     MismatchIndex mismatches;
-    if (!prepare_fixes(spec, mismatches, rk, fixes)) return fail();
+    if (!start_fixes(spec)) return fail();
+    bool fixes_done = false;
+    const std::function<bool()> finish = [&]() {
+        fixes_done = true;
+        return finish_fixes(spec, mismatches, rk, fixes);
+    };
     if (banded)
         plan_banded(spec, works);
     else
@@ -834,7 +858,8 @@ int64_t run_search(const SearchSpec& spec, ResultRow* out, int32_t max_results,
         Work wk = works.front();
         works.pop_front();
         const auto t0 = std::chrono::steady_clock::now();
-        const RunResult r = run_work(wk, rk);
+        // the fixes are finished while the first launch runs
+        const RunResult r = fixes_done ? run_work(wk, rk) : run_work(wk, rk, finish);
         if (r == RunResult::kError) return fail();
         if (r == RunResult::kSplit) {
             Work a, b;
@@ -853,6 +878,7 @@ int64_t run_search(const SearchSpec& spec, ResultRow* out, int32_t max_results,
         ctl.progress.store(1 + (int32_t)((done_centers * 79) / std::max<int64_t>(1, total_centers)),
                            std::memory_order_relaxed);
     }
+    if (!fixes_done && !finish()) return fail();
     // Replace the provisional first-draw counts of recounted centers: all of
     // them, or after a cancel those the finished launches decided (the others
     // were never counted).
