@@ -182,6 +182,8 @@ struct BatchArgs {
 hipError_t launch_batch(const BatchArgs& a, int kernel, int64_t blocks, hipStream_t stream);
 // Workgroups of class cls's sweep kernel that fit one CU at once.
 int sweep_blocks_per_cu(int kernel);
+// Workgroups of the one-tile batch kernel of a kernel index resident per CU.
+int batch_blocks_per_cu(int kernel);
 // The persistent kernel of class cls over a.order (jobs of that class only).
 hipError_t launch_sweep(const SweepArgs& a, int kernel, int grid, hipStream_t stream);
 // Sets n words to v with a kernel (usable inside a captured graph).

@@ -9,6 +9,8 @@
 // launch's claim budget is spent). There is one kernel per class (bands,
 // x-phases); its tiles run the same code as the per-search kernels
 // (band_tile.h).
+#include <utility>
+
 #include "band_tile.h"
 
 namespace slime {
@@ -195,6 +197,22 @@ int sweep_blocks_per_cu(int kernel) {
         case 27: return blocks_per_cu<27>();
         default: return 1;
     }
+}
+
+template <int... K>
+int batch_blocks_per_cu_of(int kernel, std::integer_sequence<int, K...>) {
+    int n = 0;
+    hipError_t e = hipErrorInvalidValue;
+    (void)((kernel == K
+                ? (e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel<K>, kThreads, 0),
+                   true)
+                : false) ||
+           ...);
+    return e == hipSuccess && n >= 1 ? n : 1;
+}
+
+int batch_blocks_per_cu(int kernel) {
+    return batch_blocks_per_cu_of(kernel, std::make_integer_sequence<int, kSweepKernels>{});
 }
 
 hipError_t launch_sweep(const SweepArgs& a, int kernel, int grid, hipStream_t stream) {

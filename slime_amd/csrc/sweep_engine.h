@@ -124,7 +124,8 @@ class Sweeper {
 
     int device_ = -1;
     int cus_ = 256;
-    int grid_[kSweepKernels] = {};
+    int grid_[kSweepKernels] = {};         // persistent launches: resident workgroups
+    int direct_grid_[kSweepKernels] = {};  // one-tile launches: resident workgroups
     hipStream_t stream_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     int32_t req_cap_ = 0;

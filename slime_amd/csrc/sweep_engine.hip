@@ -197,7 +197,10 @@ bool Sweeper::init(int32_t req_cap, int64_t job_cap, int64_t hit_cap) {
     hits_alloc_.reset(hit_cap);
     free_slots_.clear();
     for (int32_t s = req_cap - 1; s >= 0; --s) free_slots_.push_back(s);
-    for (int k = 0; k < kSweepKernels; ++k) grid_[k] = cus_ * sweep_blocks_per_cu(k);
+    for (int k = 0; k < kSweepKernels; ++k) {
+        grid_[k] = cus_ * sweep_blocks_per_cu(k);
+        direct_grid_[k] = cus_ * batch_blocks_per_cu(k);
+    }
     return true;
 }
 
@@ -710,11 +713,16 @@ bool Sweeper::round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& p
         l.min_key = cw.min_key;
         int64_t want = INT64_MAX;
         if (pol.round_ns > 0) {
+            // a direct launch takes whole waves of its resident workgroups, so
+            // that its tiles end together (a last partial wave would hold the
+            // device for a whole tile at part occupancy)
+            const double g = (double)(direct ? direct_grid_[c] : grid_[c]);
             const double per_tile = std::max(1e-3, cw.per_tile);
-            const double give =
-                std::max(std::min(cw.need_ns, left_ns), per_tile * (double)grid_[c]);
-            left_ns -= std::min(give, cw.need_ns);
-            want = (int64_t)std::min<double>(4e9, std::max<double>((double)grid_[c], give / per_tile));
+            const double give = std::max(std::min(cw.need_ns, left_ns), per_tile * g);
+            double tiles = give / per_tile;
+            if (direct) tiles = std::max(1.0, std::round(tiles / g)) * g;
+            left_ns -= std::min(tiles * per_tile, cw.need_ns);
+            want = (int64_t)std::min<double>(4e9, std::max(g, tiles));
         }
         l.budget = pol.round_ns > 0 ? (uint32_t)want : 0xFFFFFFFFu;
         l.first_off = (int32_t)firsts.size();
