@@ -212,10 +212,26 @@ int64_t Sweeper::footprint(const SweepRequest& r) {
 
 // ---- autotuner ----
 
+namespace {
+// The measured table of each device, shared by every sweeper of the process
+// (batches and sessions plan with the same costs, and a new session does not
+// measure again).
+std::mutex g_cost_mutex;
+std::map<int, CostTable> g_cost_tables;
+}  // namespace
+
 bool Sweeper::calibrate() {
     if (costs_.ready) return true;
-    // One synthetic job per class and tile height; min_size above any count so
-    // nothing is emitted. Slot 0 holds the outputs; its status is set active.
+    std::lock_guard<std::mutex> lock(g_cost_mutex);
+    const auto known = g_cost_tables.find(device_);
+    if (known != g_cost_tables.end()) {
+        costs_ = known->second;
+        return true;
+    }
+    // One synthetic job per class and tile height, timed in the one-tile
+    // launches that batches and direct rounds run; min_size above any count
+    // so nothing is emitted. Slot 0 holds the outputs; its status is set
+    // active.
     const int32_t slot = 0;
     const int32_t active = kReqActive;
     if (hipMemcpyAsync(d_status_ + slot, &active, sizeof(int32_t), hipMemcpyHostToDevice,
@@ -236,7 +252,7 @@ bool Sweeper::calibrate() {
             spec.z0 = 12345;
             spec.w = ph * kPeriod;
             const int64_t tiles_x = kTilesPerPhase;
-            const int64_t want = (k < 0 ? 1 : (th == 64 ? 24 : 6)) * (int64_t)grid_[kern];
+            const int64_t want = (k < 0 ? 1 : (th == 64 ? 24 : 6)) * (int64_t)direct_grid_[kern];
             const int64_t row_tiles = std::max<int64_t>(1, (want + tiles_x - 1) / tiles_x);
             const int64_t slot_rows = std::min<int64_t>(kPeriod / th * th, row_tiles * th);
             spec.h = (int64_t)(nb - 1) * kPeriod + slot_rows;
@@ -255,21 +271,22 @@ bool Sweeper::calibrate() {
             sj.p.hit_cap = 0;
             sj.p.emitted = d_emitted_ + slot;
             sj.p.hist = d_hist_ + (size_t)slot * kHistBins;
-            const int32_t order0 = 0;
+            const int32_t order0 = 0, start0 = 0;
+            const int64_t first[2] = {0, (int64_t)sj.p.tiles_total};
             float ms = 0.f;
             if (hipMemcpyAsync(d_jobs_, &sj, sizeof(sj), hipMemcpyHostToDevice, stream_) !=
                     hipSuccess ||
-                hipMemsetAsync(d_cursor_, 0, sizeof(uint32_t), stream_) != hipSuccess ||
                 hipMemsetAsync(d_done_, 0, sizeof(uint32_t), stream_) != hipSuccess ||
-                hipMemsetAsync(d_claims_, 0, sizeof(uint32_t) * kSweepKernels, stream_) !=
-                    hipSuccess ||
                 hipMemcpyAsync(d_order_, &order0, sizeof(int32_t), hipMemcpyHostToDevice,
                                stream_) != hipSuccess ||
+                hipMemcpyAsync(d_start_, &start0, sizeof(int32_t), hipMemcpyHostToDevice,
+                               stream_) != hipSuccess ||
+                hipMemcpyAsync(d_first_, first, sizeof(first), hipMemcpyHostToDevice, stream_) !=
+                    hipSuccess ||
                 hipEventRecord(ev0_, stream_) != hipSuccess)
                 return false;
-            SweepArgs args{d_jobs_, d_order_, 1, d_cursor_, d_done_, d_status_,
-                           d_claims_ + kern, 0xFFFFFFFFu};
-            if (launch_sweep(args, kern, grid_[kern], stream_) != hipSuccess ||
+            const BatchArgs args{d_jobs_, d_order_, d_first_, d_start_, 1, d_done_};
+            if (launch_batch(args, kern, first[1], stream_) != hipSuccess ||
                 hipEventRecord(ev1_, stream_) != hipSuccess ||
                 hipStreamSynchronize(stream_) != hipSuccess ||
                 hipEventElapsedTime(&ms, ev0_, ev1_) != hipSuccess)
@@ -292,6 +309,7 @@ bool Sweeper::calibrate() {
         hipStreamSynchronize(stream_) != hipSuccess)
         return false;
     costs_.ready = true;
+    g_cost_tables[device_] = costs_;
     return true;
 }
 
