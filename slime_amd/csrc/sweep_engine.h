@@ -128,6 +128,10 @@ class Sweeper {
     int direct_grid_[kSweepKernels] = {};  // one-tile launches: resident workgroups
     hipStream_t stream_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    // streams beside stream_ for the class launches of a direct round
+    static constexpr int kLanes = 3;
+    hipStream_t lane_[kLanes - 1] = {};
+    hipEvent_t ev_fork_ = nullptr, ev_join_[kLanes - 1] = {};
     int32_t req_cap_ = 0;
     int64_t job_cap_ = 0, hit_cap_ = 0, fix_cap_ = 0;
     SweepJob* d_jobs_ = nullptr;

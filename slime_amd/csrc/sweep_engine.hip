@@ -123,6 +123,14 @@ void Sweeper::release() {
     if (h_done_) (void)hipHostFree(h_done_);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
+    if (ev_fork_) (void)hipEventDestroy(ev_fork_);
+    for (int i = 0; i < kLanes - 1; ++i) {
+        if (ev_join_[i]) (void)hipEventDestroy(ev_join_[i]);
+        if (lane_[i]) (void)hipStreamDestroy(lane_[i]);
+        ev_join_[i] = nullptr;
+        lane_[i] = nullptr;
+    }
+    ev_fork_ = nullptr;
     if (stream_) (void)hipStreamDestroy(stream_);
     d_jobs_ = nullptr;
     d_cursor_ = d_done_ = d_claims_ = nullptr;
@@ -165,6 +173,7 @@ bool Sweeper::init(int32_t req_cap, int64_t job_cap, int64_t hit_cap) {
     bool ok =
         hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) == hipSuccess &&
         hipEventCreate(&ev0_) == hipSuccess && hipEventCreate(&ev1_) == hipSuccess &&
+        hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming) == hipSuccess &&
         hipMalloc(&d_jobs_, sizeof(SweepJob) * job_cap) == hipSuccess &&
         hipMalloc(&d_cursor_, sizeof(uint32_t) * job_cap) == hipSuccess &&
         hipMalloc(&d_done_, sizeof(uint32_t) * job_cap) == hipSuccess &&
@@ -188,6 +197,9 @@ bool Sweeper::init(int32_t req_cap, int64_t job_cap, int64_t hit_cap) {
         hipMalloc(&d_select_count_, sizeof(unsigned int)) == hipSuccess &&
         hipMemsetAsync(d_status_, 0, sizeof(int32_t) * req_cap, stream_) == hipSuccess &&
         hipStreamSynchronize(stream_) == hipSuccess;
+    for (int i = 0; ok && i < kLanes - 1; ++i)
+        ok = hipStreamCreateWithFlags(&lane_[i], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&ev_join_[i], hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         (void)hipGetLastError();
         release();
@@ -773,19 +785,35 @@ bool Sweeper::round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& p
                                    hipMemcpyHostToDevice, stream_) != hipSuccess)))
         return false;
 
+    // Direct launches of several classes run side by side on the lane
+    // streams (they touch disjoint jobs; a request's counters are atomics), so
+    // that one launch's last workgroups overlap the next one's; the round
+    // ends when all have ended. A captured graph keeps one stream.
+    const int lanes = direct && !pol.use_graph ? std::min<int>(kLanes, (int)launches.size()) : 1;
     auto enqueue = [&]() -> bool {
         if (launch_fill_u32(d_claims_, kSweepKernels, 0u, stream_) != hipSuccess) return false;
-        for (const Launch& l : launches) {
+        if (lanes > 1 && hipEventRecord(ev_fork_, stream_) != hipSuccess) return false;
+        for (int i = 1; i < lanes; ++i)
+            if (hipStreamWaitEvent(lane_[i - 1], ev_fork_, 0) != hipSuccess) return false;
+        for (size_t li = 0; li < launches.size(); ++li) {
+            const Launch& l = launches[li];
             if (direct) {
+                const int lane = (int)(li % (size_t)lanes);
                 BatchArgs a{d_jobs_, d_order_ + l.off, d_first_ + l.first_off, d_start_ + l.off,
                             l.n, d_done_};
-                if (launch_batch(a, l.cls, l.blocks, stream_) != hipSuccess) return false;
+                if (launch_batch(a, l.cls, l.blocks, lane ? lane_[lane - 1] : stream_) !=
+                    hipSuccess)
+                    return false;
                 continue;
             }
             SweepArgs a{d_jobs_, d_order_ + l.off, l.n, d_cursor_, d_done_, d_status_,
                         d_claims_ + l.cls, l.budget};
             if (launch_sweep(a, l.cls, grid_[l.cls], stream_) != hipSuccess) return false;
         }
+        for (int i = 1; i < lanes; ++i)
+            if (hipEventRecord(ev_join_[i - 1], lane_[i - 1]) != hipSuccess ||
+                hipStreamWaitEvent(stream_, ev_join_[i - 1], 0) != hipSuccess)
+                return false;
         return true;
     };
     if (hipEventRecord(ev0_, stream_) != hipSuccess) return false;
