@@ -342,14 +342,14 @@ double Sweeper::tile_ns(int kernel, int32_t tile_h) const {
 //    phase elsewhere (x_parts_one_phase_tail); or, with the union bound and
 //    at most 5 bands, three phases past the last whole 2P block;
 //  * z, per x-part: the bands grouped into kernel classes (cheapest cover),
-//    either all over P slot rows, or the full bands so and the last, partial
-//    band as its own job over just its rows.
+//    either all over P slot rows, or with the last full bands split by slot
+//    rows around the partial band (best_z below).
 bool Sweeper::plan(SweepRequest& r) const {
     const SearchSpec& spec = r.spec;
     struct Group {
         int64_t b_first;
         int nb;
-        int32_t s_rows;
+        int32_t s_lo, s_hi;  // slot rows [s_lo, s_hi)
     };
     auto tile_for = [](int32_t s_rows) { return std::min(kMaxTileH, (s_rows + 3) / 4 * 4); };
     auto group_cost = [&](const XPart& xp, int nb, int32_t s_rows) {
@@ -363,9 +363,10 @@ bool Sweeper::plan(SweepRequest& r) const {
     // kUnionBoundMaxBands bands and the three-phase classes: where it
     // applies, only those cover the bands
     const bool ub = union_bound_for(spec.mode, spec.min_size);
-    // cheapest cover of bands [b0, b0 + n) over s_rows slot rows
-    auto cover = [&](const XPart& xp, int64_t b0, int64_t n, int32_t s_rows,
+    // cheapest cover of bands [b0, b0 + n) over slot rows [s_lo, s_hi)
+    auto cover = [&](const XPart& xp, int64_t b0, int64_t n, int32_t s_lo, int32_t s_hi,
                      std::vector<Group>& out) {
+        const int32_t s_rows = s_hi - s_lo;
         std::vector<double> cost(n + 1, -1.0);
         std::vector<int> pick(n + 1, 0);
         cost[0] = 0;
@@ -387,29 +388,37 @@ bool Sweeper::plan(SweepRequest& r) const {
         std::sort(sizes.begin(), sizes.end(), std::greater<int>());
         int64_t b = b0;
         for (int nb : sizes) {
-            out.push_back(Group{b, nb, s_rows});
+            out.push_back(Group{b, nb, s_lo, s_hi});
             b += nb;
         }
         return cost[n];
     };
+    // (a) every band over all slot rows; or, with a last, partial band of rem
+    // rows after nf full ones, for m = 0..nf: the first nf - m bands over all
+    // P rows, the last m full bands and the partial one over rows [0, rem),
+    // those m bands alone over rows [rem, P) (m = 0: the partial band as a
+    // short group of its own). No tile then scans the partial band's missing
+    // rows, and the groups keep high band counts.
     auto best_z = [&](const XPart& xp, std::vector<Group>& out) {
         const int32_t slot_rows = slot_rows_of(spec);
         const int64_t nbz = bands_of(spec);
         std::vector<Group> a;
-        double ca = cover(xp, 0, nbz, slot_rows, a);
+        double best = cover(xp, 0, nbz, 0, slot_rows, a);
+        out = a;
         const int64_t nf = spec.h / kPeriod, rem = spec.h % kPeriod;
-        if (spec.band_rows == 0 && nf >= 1 && rem > 0) {
-            std::vector<Group> b;
-            double cb = cover(xp, 0, nf, kPeriod, b);
-            const double cp = group_cost(xp, 1, (int32_t)rem);
-            if (cb >= 0 && cp >= 0 && (ca < 0 || cb + cp < ca)) {
-                b.push_back(Group{nf, 1, (int32_t)rem});
-                out = b;
-                return cb + cp;
+        if (spec.band_rows != 0 || nf < 1 || rem <= 0) return best;
+        for (int64_t m = 0; m <= nf; ++m) {
+            std::vector<Group> g;
+            const double c0 = cover(xp, 0, nf - m, 0, kPeriod, g);
+            const double c1 = cover(xp, nf - m, m + 1, 0, (int32_t)rem, g);
+            const double c2 = cover(xp, nf - m, m, (int32_t)rem, kPeriod, g);
+            if (c0 < 0 || c1 < 0 || c2 < 0) continue;
+            if (best < 0 || c0 + c1 + c2 < best) {
+                best = c0 + c1 + c2;
+                out = g;
             }
         }
-        out = a;
-        return ca;
+        return best;
     };
     std::vector<std::vector<XPart>> xopts;
     xopts.push_back(x_parts(spec, 1));
@@ -442,8 +451,8 @@ bool Sweeper::plan(SweepRequest& r) const {
     r.jobs.clear();
     for (size_t i = 0; i < best_parts.size(); ++i)
         for (const Group& g : best_groups[i])
-            band_jobs_part(spec, best_parts[i], g.b_first, g.nb, g.s_rows, tile_for(g.s_rows),
-                           int64_t(1) << 62, r.jobs);
+            band_jobs_part(spec, best_parts[i], g.b_first, g.nb, g.s_hi,
+                           tile_for(g.s_hi - g.s_lo), int64_t(1) << 62, r.jobs, g.s_lo);
     r.job_cls.clear();
     r.centers = 0;
     for (const BandJob& j : r.jobs) {
