@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <chrono>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -42,8 +43,10 @@ struct SweepRequest {
     int64_t job_off = -1, hit_off = -1;
     // progress
     int32_t status = 0;  // mirror of the device status word
-    std::vector<uint32_t> done;
-    double remaining_ns = 0;
+    std::vector<uint32_t> done;     // per job: tiles finished (the first done[j] of the job)
+    std::vector<uint32_t> planned;  // per job: tiles launched (direct rounds)
+    int32_t inflight = 0;           // launched rounds holding tiles of it
+    double remaining_ns = 0;        // modelled device time of its tiles not yet launched
     bool complete = false;
     // first-draw fixes
     MismatchIndex mm;
@@ -96,7 +99,24 @@ class Sweeper {
     bool upload(const std::vector<SweepRequest*>& reqs);   // jobs, cleared outputs, active
     bool set_status(SweepRequest& r, int32_t status);
     bool clear_status();  // every request slot inactive
+    // One round, waited for (after any rounds still in flight).
     bool round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& pol, RoundStats* st);
+    // A round launched without waiting for it, planned from the tiles already
+    // launched (direct rounds: up to two in flight, the next one queued while
+    // the last one runs): 1 launched, 0 nothing to launch, -1 error.
+    int launch_round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& pol);
+    // Waits for the oldest round in flight and takes in its done counts; its
+    // requests with every tile done are complete.
+    bool wait_round(RoundStats* st);
+    int in_flight() const { return pend_n_; }
+    // whether the oldest round in flight has ended (its done counts copied)
+    bool oldest_ended() const {
+        return pend_n_ > 0 && hipEventQuery(ev_copied_[pend_first_]) == hipSuccess;
+    }
+    // host time by which the newest round in flight is modelled to end
+    std::chrono::steady_clock::time_point expected_end() const {
+        return pend_[(pend_first_ + pend_n_ + kRing - 1) % kRing].t_end;
+    }
     double progress(const SweepRequest& r) const;
     bool finalize(SweepRequest& r, bool partial);
     // Many finished requests: their counts in one copy, their stored hits in
@@ -126,21 +146,45 @@ class Sweeper {
     int cus_ = 256;
     int grid_[kSweepKernels] = {};         // persistent launches: resident workgroups
     int direct_grid_[kSweepKernels] = {};  // one-tile launches: resident workgroups
-    hipStream_t stream_ = nullptr;
+    hipStream_t stream_ = nullptr;  // set-up, uploads, status words, ranking
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
-    // streams beside stream_ for the class launches of a direct round
+    // Rounds run on the lane streams (a round's class launches side by side)
+    // from a ring of kRing slots, each with its own launch arrays, done-count
+    // copy and events; at most kRing - 1 rounds are in flight, so a slot is
+    // rewritten only after its round has been waited for and the round before
+    // the one being waited for keeps its events (the device time overlap).
     static constexpr int kLanes = 3;
-    hipStream_t lane_[kLanes - 1] = {};
-    hipEvent_t ev_fork_ = nullptr, ev_join_[kLanes - 1] = {};
+    static constexpr int kRing = 3;
+    hipStream_t lane_[kLanes] = {};
+    struct Expect {
+        SweepRequest* req;
+        int32_t j;
+        uint32_t count;  // the job's done count once the round has ended
+    };
+    struct Pending {
+        bool direct = true;
+        int32_t launches = 0;
+        double est_ns = 0;
+        std::chrono::steady_clock::time_point t_launch, t_end;
+        std::vector<int32_t> order, starts;  // host copies of the launch arrays
+        std::vector<int64_t> firsts;
+        std::vector<Expect> expect;
+        std::vector<SweepRequest*> reqs;
+    };
+    Pending pend_[kRing];
+    int pend_first_ = 0, pend_n_ = 0, last_slot_ = -1;
+    int big_lane_ = 0;  // lane of the last round's longest launch
+    hipEvent_t rev0_[kRing] = {}, rev1_[kRing] = {}, ev_copied_[kRing] = {}, ev_fork_[kRing] = {};
+    hipEvent_t ev_join_[kRing][kLanes - 1] = {};
     int32_t req_cap_ = 0;
     int64_t job_cap_ = 0, hit_cap_ = 0, fix_cap_ = 0;
     SweepJob* d_jobs_ = nullptr;
     uint32_t* d_cursor_ = nullptr;
     uint32_t* d_done_ = nullptr;
-    uint32_t* h_done_ = nullptr;
-    int32_t* d_order_ = nullptr;
-    int64_t* d_first_ = nullptr;  // direct launches: first block of each job (job_cap + kernels)
-    int32_t* d_start_ = nullptr;  // direct launches: first tile of each job
+    uint32_t* h_done_[kRing] = {};
+    int32_t* d_order_[kRing] = {};
+    int64_t* d_first_[kRing] = {};  // direct launches: first block of each job (job_cap + kernels)
+    int32_t* d_start_[kRing] = {};  // direct launches: first tile of each job
     int32_t* d_status_ = nullptr;
     unsigned long long* d_hist_ = nullptr;
     unsigned long long* d_emitted_ = nullptr;

@@ -96,18 +96,27 @@ void Sweeper::Ranges::release(int64_t off, int64_t n) {
 Sweeper::~Sweeper() { release(); }
 
 void Sweeper::release() {
+    if (device_ >= 0) (void)hipSetDevice(device_);
+    // rounds still in flight end first
+    while (pend_n_ > 0) {
+        (void)hipEventSynchronize(ev_copied_[pend_first_]);
+        pend_first_ = (pend_first_ + 1) % kRing;
+        --pend_n_;
+    }
+    pend_first_ = 0;
+    last_slot_ = -1;
     for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
     graphs_.clear();
     auto f = [](void* p) {
         if (p) (void)hipFree(p);
     };
-    if (device_ >= 0) (void)hipSetDevice(device_);
+    auto ev = [](hipEvent_t& e) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    };
     f(d_jobs_);
     f(d_cursor_);
     f(d_done_);
-    f(d_order_);
-    f(d_first_);
-    f(d_start_);
     f(d_status_);
     f(d_hist_);
     f(d_emitted_);
@@ -120,23 +129,31 @@ void Sweeper::release() {
     f(d_fixes_);
     f(d_select_);
     f(d_select_count_);
-    if (h_done_) (void)hipHostFree(h_done_);
-    if (ev0_) (void)hipEventDestroy(ev0_);
-    if (ev1_) (void)hipEventDestroy(ev1_);
-    if (ev_fork_) (void)hipEventDestroy(ev_fork_);
-    for (int i = 0; i < kLanes - 1; ++i) {
-        if (ev_join_[i]) (void)hipEventDestroy(ev_join_[i]);
+    for (int k = 0; k < kRing; ++k) {
+        f(d_order_[k]);
+        f(d_first_[k]);
+        f(d_start_[k]);
+        if (h_done_[k]) (void)hipHostFree(h_done_[k]);
+        d_order_[k] = nullptr;
+        d_first_[k] = nullptr;
+        d_start_[k] = nullptr;
+        h_done_[k] = nullptr;
+        ev(rev0_[k]);
+        ev(rev1_[k]);
+        ev(ev_copied_[k]);
+        ev(ev_fork_[k]);
+        for (int i = 0; i < kLanes - 1; ++i) ev(ev_join_[k][i]);
+    }
+    ev(ev0_);
+    ev(ev1_);
+    for (int i = 0; i < kLanes; ++i) {
         if (lane_[i]) (void)hipStreamDestroy(lane_[i]);
-        ev_join_[i] = nullptr;
         lane_[i] = nullptr;
     }
-    ev_fork_ = nullptr;
     if (stream_) (void)hipStreamDestroy(stream_);
     d_jobs_ = nullptr;
     d_cursor_ = d_done_ = d_claims_ = nullptr;
-    d_order_ = d_status_ = nullptr;
-    d_first_ = nullptr;
-    d_start_ = nullptr;
+    d_status_ = nullptr;
     d_hist_ = d_emitted_ = nullptr;
     d_hits_ = nullptr;
     d_table_ = nullptr;
@@ -146,8 +163,6 @@ void Sweeper::release() {
     d_fixes_ = nullptr;
     d_select_ = nullptr;
     d_select_count_ = nullptr;
-    h_done_ = nullptr;
-    ev0_ = ev1_ = nullptr;
     stream_ = nullptr;
     device_ = -1;
     req_cap_ = 0;
@@ -173,14 +188,10 @@ bool Sweeper::init(int32_t req_cap, int64_t job_cap, int64_t hit_cap) {
     bool ok =
         hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) == hipSuccess &&
         hipEventCreate(&ev0_) == hipSuccess && hipEventCreate(&ev1_) == hipSuccess &&
-        hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming) == hipSuccess &&
         hipMalloc(&d_jobs_, sizeof(SweepJob) * job_cap) == hipSuccess &&
         hipMalloc(&d_cursor_, sizeof(uint32_t) * job_cap) == hipSuccess &&
         hipMalloc(&d_done_, sizeof(uint32_t) * job_cap) == hipSuccess &&
-        hipMalloc(&d_order_, sizeof(int32_t) * job_cap) == hipSuccess &&
-        hipMalloc(&d_first_, sizeof(int64_t) * (job_cap + kSweepKernels)) == hipSuccess &&
-        hipMalloc(&d_start_, sizeof(int32_t) * job_cap) == hipSuccess &&
-        hipHostMalloc(&h_done_, sizeof(uint32_t) * job_cap, hipHostMallocDefault) == hipSuccess &&
+
         hipMalloc(&d_status_, sizeof(int32_t) * req_cap) == hipSuccess &&
         hipMalloc(&d_hist_, sizeof(unsigned long long) * kHistBins * req_cap) == hipSuccess &&
         hipMalloc(&d_emitted_, sizeof(unsigned long long) * req_cap) == hipSuccess &&
@@ -197,9 +208,20 @@ bool Sweeper::init(int32_t req_cap, int64_t job_cap, int64_t hit_cap) {
         hipMalloc(&d_select_count_, sizeof(unsigned int)) == hipSuccess &&
         hipMemsetAsync(d_status_, 0, sizeof(int32_t) * req_cap, stream_) == hipSuccess &&
         hipStreamSynchronize(stream_) == hipSuccess;
-    for (int i = 0; ok && i < kLanes - 1; ++i)
-        ok = hipStreamCreateWithFlags(&lane_[i], hipStreamNonBlocking) == hipSuccess &&
-             hipEventCreateWithFlags(&ev_join_[i], hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; ok && i < kLanes; ++i)
+        ok = hipStreamCreateWithFlags(&lane_[i], hipStreamNonBlocking) == hipSuccess;
+    for (int k = 0; ok && k < kRing; ++k) {
+        ok = hipMalloc(&d_order_[k], sizeof(int32_t) * job_cap) == hipSuccess &&
+             hipMalloc(&d_first_[k], sizeof(int64_t) * (job_cap + kSweepKernels)) == hipSuccess &&
+             hipMalloc(&d_start_[k], sizeof(int32_t) * job_cap) == hipSuccess &&
+             hipHostMalloc(&h_done_[k], sizeof(uint32_t) * job_cap, hipHostMallocDefault) ==
+                 hipSuccess &&
+             hipEventCreate(&rev0_[k]) == hipSuccess && hipEventCreate(&rev1_[k]) == hipSuccess &&
+             hipEventCreateWithFlags(&ev_copied_[k], hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&ev_fork_[k], hipEventDisableTiming) == hipSuccess;
+        for (int i = 0; ok && i < kLanes - 1; ++i)
+            ok = hipEventCreateWithFlags(&ev_join_[k][i], hipEventDisableTiming) == hipSuccess;
+    }
     if (!ok) {
         (void)hipGetLastError();
         release();
@@ -289,15 +311,15 @@ bool Sweeper::calibrate() {
             if (hipMemcpyAsync(d_jobs_, &sj, sizeof(sj), hipMemcpyHostToDevice, stream_) !=
                     hipSuccess ||
                 hipMemsetAsync(d_done_, 0, sizeof(uint32_t), stream_) != hipSuccess ||
-                hipMemcpyAsync(d_order_, &order0, sizeof(int32_t), hipMemcpyHostToDevice,
+                hipMemcpyAsync(d_order_[0], &order0, sizeof(int32_t), hipMemcpyHostToDevice,
                                stream_) != hipSuccess ||
-                hipMemcpyAsync(d_start_, &start0, sizeof(int32_t), hipMemcpyHostToDevice,
+                hipMemcpyAsync(d_start_[0], &start0, sizeof(int32_t), hipMemcpyHostToDevice,
                                stream_) != hipSuccess ||
-                hipMemcpyAsync(d_first_, first, sizeof(first), hipMemcpyHostToDevice, stream_) !=
+                hipMemcpyAsync(d_first_[0], first, sizeof(first), hipMemcpyHostToDevice, stream_) !=
                     hipSuccess ||
                 hipEventRecord(ev0_, stream_) != hipSuccess)
                 return false;
-            const BatchArgs args{d_jobs_, d_order_, d_first_, d_start_, 1, d_done_};
+            const BatchArgs args{d_jobs_, d_order_[0], d_first_[0], d_start_[0], 1, d_done_};
             if (launch_batch(args, kern, first[1], stream_) != hipSuccess ||
                 hipEventRecord(ev1_, stream_) != hipSuccess ||
                 hipStreamSynchronize(stream_) != hipSuccess ||
@@ -482,6 +504,8 @@ bool Sweeper::reserve(SweepRequest& r) {
     r.job_off = jo;
     r.hit_off = ho;
     r.done.assign(r.jobs.size(), 0);
+    r.planned.assign(r.jobs.size(), 0);
+    r.inflight = 0;
     r.complete = false;
     return true;
 }
@@ -673,39 +697,71 @@ bool Sweeper::set_status(SweepRequest& r, int32_t status) {
 
 bool Sweeper::round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& pol,
                     RoundStats* st) {
-    // per-class claim order over the jobs of active requests with tiles left
+    while (pend_n_ > 0)
+        if (!wait_round(st)) return false;
+    const int launched = launch_round(reqs, pol);
+    return launched >= 0 && (launched == 0 || wait_round(st));
+}
+
+namespace {
+bool all_tiles(const SweepRequest& r, const std::vector<uint32_t>& count) {
+    for (size_t j = 0; j < r.jobs.size(); ++j)
+        if ((int64_t)count[j] < (int64_t)r.jobs[j].p.tiles_total) return false;
+    return true;
+}
+}  // namespace
+
+int Sweeper::launch_round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& pol) {
+    if (pend_n_ >= kRing - 1) return -1;  // at most kRing - 1 rounds in flight
+    const bool direct = pol.direct;
+    if (!direct && pend_n_ > 0) return -1;  // persistent rounds read the device's counts
+    // a request with every tile launched and none in flight is complete
+    for (SweepRequest* r : reqs)
+        if (r->slot >= 0 && r->inflight == 0 && all_tiles(*r, r->done)) r->complete = true;
+    // per-class claim order over the jobs of active requests with tiles not
+    // yet launched
     struct Item {
         double key;
         int64_t seq;
         int32_t job;
         int64_t left;
         int32_t tile_h;
-        int64_t start;  // tiles of the job finished before this round
+        int64_t start;  // tiles of the job launched before this round
+        SweepRequest* req;
+        int32_t j;
     };
     std::vector<Item> items[kSweepKernels];
     int64_t seq = 0;
-    for (const SweepRequest* r : reqs) {
-        if (r->status != kReqActive || r->complete) continue;
+    for (SweepRequest* r : reqs) {
+        if (r->slot < 0 || r->status != kReqActive || r->complete) continue;
         for (size_t j = 0; j < r->jobs.size(); ++j) {
-            const int64_t left = (int64_t)r->jobs[j].p.tiles_total - r->done[j];
+            const int64_t left = (int64_t)r->jobs[j].p.tiles_total - r->planned[j];
             const int c = r->job_cls[j];
             // largest tiles first (batch), or the request with the least work left
             const double key = pol.shortest_first ? r->remaining_ns : -tile_ns(c, r->jobs[j].p.tile_h);
             if (left > 0)
                 items[c].push_back(Item{key, seq, (int32_t)(r->job_off + (int64_t)j), left,
-                                        r->jobs[j].p.tile_h, (int64_t)r->done[j]});
+                                        r->jobs[j].p.tile_h, (int64_t)r->planned[j], r,
+                                        (int32_t)j});
             ++seq;
         }
     }
-    std::vector<int32_t> order;
+    const int slot = (pend_first_ + pend_n_) % kRing;
+    Pending& pr = pend_[slot];
+    pr.order.clear();
+    pr.firsts.clear();
+    pr.starts.clear();
+    pr.expect.clear();
+    pr.reqs.clear();
+    pr.est_ns = 0;
+    std::vector<int32_t>& order = pr.order;
     // Direct rounds: one workgroup per tile of a range of tiles per job (all
     // its tiles left, or as many as the round's device time allows, most
     // urgent jobs first), as the per-search kernels do (their register
     // budget, no claim loop); pause, resume and cancel act between rounds.
     // Otherwise the persistent kernels claim tiles under a claim budget.
-    const bool direct = pol.direct;
-    std::vector<int64_t> firsts;  // per launch: n + 1 block offsets
-    std::vector<int32_t> starts;  // per direct item: its first tile this round
+    std::vector<int64_t>& firsts = pr.firsts;  // per launch: n + 1 block offsets
+    std::vector<int32_t>& starts = pr.starts;  // per direct item: its first tile this round
     struct Launch {
         int cls;
         int32_t off, n;
@@ -713,6 +769,7 @@ bool Sweeper::round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& p
         double min_key;
         int32_t first_off;
         int64_t blocks;
+        double est_ns;  // modelled device time
     };
     std::vector<Launch> launches;
     // classes holding the most urgent request go first, and the round's device
@@ -766,6 +823,7 @@ bool Sweeper::round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& p
         l.budget = pol.round_ns > 0 ? (uint32_t)want : 0xFFFFFFFFu;
         l.first_off = (int32_t)firsts.size();
         l.blocks = 0;
+        const double est_before = pr.est_ns;
         if (direct) {
             // a launch's grid holds at most 2^31 - 1 workgroups; the rest of
             // the tiles wait for the next round
@@ -778,59 +836,67 @@ bool Sweeper::round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& p
                 starts.push_back((int32_t)it.start);
                 l.blocks += n;
                 firsts.push_back(l.blocks);
+                pr.expect.push_back(Expect{it.req, it.j, (uint32_t)(it.start + n)});
+                pr.est_ns += (double)n * tile_ns(c, it.tile_h);
             }
         } else {
             for (const Item& it : v) order.push_back(it.job);
+            pr.est_ns += std::min(cw.need_ns, (double)want * cw.per_tile);
         }
         l.n = (int32_t)order.size() - l.off;
+        l.est_ns = pr.est_ns - est_before;
         launches.push_back(l);
     }
-    if (launches.empty()) return true;
-    if (hipMemcpyAsync(d_order_, order.data(), sizeof(int32_t) * order.size(),
-                       hipMemcpyHostToDevice, stream_) != hipSuccess ||
-        (direct && (hipMemcpyAsync(d_first_, firsts.data(), sizeof(int64_t) * firsts.size(),
-                                   hipMemcpyHostToDevice, stream_) != hipSuccess ||
-                    hipMemcpyAsync(d_start_, starts.data(), sizeof(int32_t) * starts.size(),
-                                   hipMemcpyHostToDevice, stream_) != hipSuccess)))
-        return false;
-
+    if (launches.empty()) return 0;
     // Direct launches of several classes run side by side on the lane
     // streams (they touch disjoint jobs; a request's counters are atomics), so
     // that one launch's last workgroups overlap the next one's; the round
     // ends when all have ended. A captured graph keeps one stream.
+    // A round launched while the previous one runs starts on the lane after
+    // the one holding the previous round's longest launch, so that its most
+    // urgent launch does not queue behind that one.
     const int lanes = direct && !pol.use_graph ? std::min<int>(kLanes, (int)launches.size()) : 1;
+    const int rot = direct && !pol.use_graph ? (big_lane_ + 1) % kLanes : 0;
+    hipStream_t join = lane_[rot];  // the round's copies, events and the join run there
+    if (hipMemcpyAsync(d_order_[slot], order.data(), sizeof(int32_t) * order.size(),
+                       hipMemcpyHostToDevice, join) != hipSuccess ||
+        (direct && (hipMemcpyAsync(d_first_[slot], firsts.data(), sizeof(int64_t) * firsts.size(),
+                                   hipMemcpyHostToDevice, join) != hipSuccess ||
+                    hipMemcpyAsync(d_start_[slot], starts.data(), sizeof(int32_t) * starts.size(),
+                                   hipMemcpyHostToDevice, join) != hipSuccess)))
+        return -1;
+    auto lane_of = [&](size_t li) { return lane_[(rot + (int)(li % (size_t)lanes)) % kLanes]; };
     auto enqueue = [&]() -> bool {
-        if (launch_fill_u32(d_claims_, kSweepKernels, 0u, stream_) != hipSuccess) return false;
-        if (lanes > 1 && hipEventRecord(ev_fork_, stream_) != hipSuccess) return false;
+        if (launch_fill_u32(d_claims_, kSweepKernels, 0u, join) != hipSuccess) return false;
+        if (lanes > 1 && hipEventRecord(ev_fork_[slot], join) != hipSuccess) return false;
         for (int i = 1; i < lanes; ++i)
-            if (hipStreamWaitEvent(lane_[i - 1], ev_fork_, 0) != hipSuccess) return false;
+            if (hipStreamWaitEvent(lane_of((size_t)i), ev_fork_[slot], 0) != hipSuccess)
+                return false;
         for (size_t li = 0; li < launches.size(); ++li) {
             const Launch& l = launches[li];
             if (direct) {
-                const int lane = (int)(li % (size_t)lanes);
-                BatchArgs a{d_jobs_, d_order_ + l.off, d_first_ + l.first_off, d_start_ + l.off,
-                            l.n, d_done_};
-                if (launch_batch(a, l.cls, l.blocks, lane ? lane_[lane - 1] : stream_) !=
-                    hipSuccess)
-                    return false;
+                BatchArgs a{d_jobs_, d_order_[slot] + l.off, d_first_[slot] + l.first_off,
+                            d_start_[slot] + l.off, l.n, d_done_};
+                if (launch_batch(a, l.cls, l.blocks, lane_of(li)) != hipSuccess) return false;
                 continue;
             }
-            SweepArgs a{d_jobs_, d_order_ + l.off, l.n, d_cursor_, d_done_, d_status_,
+            SweepArgs a{d_jobs_, d_order_[slot] + l.off, l.n, d_cursor_, d_done_, d_status_,
                         d_claims_ + l.cls, l.budget};
-            if (launch_sweep(a, l.cls, grid_[l.cls], stream_) != hipSuccess) return false;
+            if (launch_sweep(a, l.cls, grid_[l.cls], join) != hipSuccess) return false;
         }
         for (int i = 1; i < lanes; ++i)
-            if (hipEventRecord(ev_join_[i - 1], lane_[i - 1]) != hipSuccess ||
-                hipStreamWaitEvent(stream_, ev_join_[i - 1], 0) != hipSuccess)
+            if (hipEventRecord(ev_join_[slot][i - 1], lane_of((size_t)i)) != hipSuccess ||
+                hipStreamWaitEvent(join, ev_join_[slot][i - 1], 0) != hipSuccess)
                 return false;
         return true;
     };
-    if (hipEventRecord(ev0_, stream_) != hipSuccess) return false;
+    if (hipEventRecord(rev0_[slot], join) != hipSuccess) return -1;
     if (pol.use_graph) {
         // the launch sequence, keyed by its arguments; replays reuse the device
         // arrays (jobs, orders) whose contents were written above
         std::vector<int64_t> key;
         key.push_back(direct ? 1 : 0);
+        key.push_back(slot);
         for (const Launch& l : launches) {
             key.push_back(l.cls);
             key.push_back(l.off);
@@ -843,48 +909,115 @@ bool Sweeper::round(const std::vector<SweepRequest*>& reqs, const RoundPolicy& p
         if (it == graphs_.end()) {
             hipGraph_t g = nullptr;
             hipGraphExec_t ge = nullptr;
-            if (hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal) != hipSuccess)
-                return false;
+            if (hipStreamBeginCapture(join, hipStreamCaptureModeThreadLocal) != hipSuccess)
+                return -1;
             const bool ok = enqueue();
-            if (hipStreamEndCapture(stream_, &g) != hipSuccess || !ok) {
+            if (hipStreamEndCapture(join, &g) != hipSuccess || !ok) {
                 if (g) (void)hipGraphDestroy(g);
-                return false;
+                return -1;
             }
             const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
-            if (e != hipSuccess) return false;
+            if (e != hipSuccess) return -1;
             it = graphs_.emplace(key, ge).first;
         }
-        if (hipGraphLaunch(it->second, stream_) != hipSuccess) return false;
+        if (hipGraphLaunch(it->second, join) != hipSuccess) return -1;
     } else if (!enqueue()) {
-        return false;
+        return -1;
     }
-    float ms = 0.f;
-    if (hipEventRecord(ev1_, stream_) != hipSuccess ||
-        hipMemcpyAsync(h_done_, d_done_, sizeof(uint32_t) * (size_t)job_cap_,
-                       hipMemcpyDeviceToHost, stream_) != hipSuccess ||
-        hipStreamSynchronize(stream_) != hipSuccess ||
-        hipEventElapsedTime(&ms, ev0_, ev1_) != hipSuccess)
-        return false;
-    if (st) {
-        st->kernel_ns += (int64_t)((double)ms * 1e6);
-        st->launches += (int32_t)launches.size();
-        st->rounds += 1;
+    if (hipEventRecord(rev1_[slot], join) != hipSuccess ||
+        hipMemcpyAsync(h_done_[slot], d_done_, sizeof(uint32_t) * (size_t)job_cap_,
+                       hipMemcpyDeviceToHost, join) != hipSuccess ||
+        hipEventRecord(ev_copied_[slot], join) != hipSuccess)
+        return -1;
+    size_t big = 0;
+    for (size_t li = 1; li < launches.size(); ++li)
+        if (launches[li].est_ns > launches[big].est_ns) big = li;
+    big_lane_ = (rot + (int)(big % (size_t)lanes)) % kLanes;
+    pr.direct = direct;
+    pr.launches = (int32_t)launches.size();
+    for (const Expect& e : pr.expect) {
+        SweepRequest& r = *e.req;
+        r.planned[e.j] = e.count;
     }
     for (SweepRequest* r : reqs) {
-        if (r->slot < 0 || r->complete) continue;
-        bool all = true;
+        bool in = !direct && r->slot >= 0 && r->status == kReqActive && !r->complete;
+        if (direct)
+            for (const Expect& e : pr.expect)
+                if (e.req == r) {
+                    in = true;
+                    break;
+                }
+        if (!in) continue;
+        ++r->inflight;
+        pr.reqs.push_back(r);
         double left_ns = 0;
         for (size_t j = 0; j < r->jobs.size(); ++j) {
-            r->done[j] = h_done_[r->job_off + (int64_t)j];
-            const int64_t left = (int64_t)r->jobs[j].p.tiles_total - r->done[j];
-            if (left > 0) {
-                all = false;
-                left_ns += (double)left * tile_ns(r->job_cls[j], r->jobs[j].p.tile_h);
-            }
+            const int64_t left = (int64_t)r->jobs[j].p.tiles_total - r->planned[j];
+            if (left > 0) left_ns += (double)left * tile_ns(r->job_cls[j], r->jobs[j].p.tile_h);
         }
         r->remaining_ns = left_ns;
-        r->complete = all;
+    }
+    const Clock::time_point now = Clock::now();
+    Clock::time_point begin = now;
+    if (pend_n_ > 0) begin = std::max(begin, pend_[(pend_first_ + pend_n_ - 1) % kRing].t_end);
+    pr.t_launch = now;
+    pr.t_end = begin + std::chrono::nanoseconds((int64_t)pr.est_ns);
+    ++pend_n_;
+    return 1;
+}
+
+bool Sweeper::wait_round(RoundStats* st) {
+    if (pend_n_ == 0) return true;
+    const int slot = pend_first_;
+    Pending& pr = pend_[slot];
+    if (hipEventSynchronize(ev_copied_[slot]) != hipSuccess) return false;
+    pend_first_ = (pend_first_ + 1) % kRing;
+    --pend_n_;
+    const Clock::time_point now = Clock::now();
+    if (pend_n_ > 0) {  // the next round runs from about now
+        Pending& nx = pend_[pend_first_];
+        nx.t_end = std::max(now, nx.t_launch) + std::chrono::nanoseconds((int64_t)nx.est_ns);
+    }
+    // device time: the round's span less what overlapped the previous round
+    float ms = 0.f, before = 0.f;
+    if (hipEventElapsedTime(&ms, rev0_[slot], rev1_[slot]) != hipSuccess) return false;
+    if (last_slot_ >= 0 && hipEventElapsedTime(&before, rev0_[slot], rev1_[last_slot_]) == hipSuccess)
+        ms -= std::min(ms, std::max(0.f, before));
+    else
+        (void)hipGetLastError();
+    last_slot_ = slot;
+    if (st) {
+        st->kernel_ns += (int64_t)((double)ms * 1e6);
+        st->launches += pr.launches;
+        st->rounds += 1;
+    }
+    const uint32_t* h = h_done_[slot];
+    if (pr.direct) {
+        // the tiles of this round are done (later rounds may have added more)
+        for (const Expect& e : pr.expect) {
+            SweepRequest& r = *e.req;
+            if (h[r.job_off + e.j] < e.count) return false;
+            r.done[e.j] = std::max(r.done[e.j], e.count);
+        }
+    } else {
+        for (SweepRequest* r : pr.reqs)
+            for (size_t j = 0; j < r->jobs.size(); ++j) {
+                r->done[j] = h[r->job_off + (int64_t)j];
+                r->planned[j] = r->done[j];
+            }
+    }
+    for (SweepRequest* r : pr.reqs) {
+        --r->inflight;
+        if (!pr.direct) {
+            double left_ns = 0;
+            for (size_t j = 0; j < r->jobs.size(); ++j) {
+                const int64_t left = (int64_t)r->jobs[j].p.tiles_total - r->done[j];
+                if (left > 0) left_ns += (double)left * tile_ns(r->job_cls[j], r->jobs[j].p.tile_h);
+            }
+            r->remaining_ns = left_ns;
+        }
+        if (r->inflight == 0 && all_tiles(*r, r->done)) r->complete = true;
     }
     return true;
 }
@@ -1334,6 +1467,51 @@ SessionStats Session::stats() {
 
 void Session::worker() {
     std::unique_lock<std::mutex> lk(mu_);
+    // Ranks the finished requests (every tile done) and the cancelled ones,
+    // then releases them; called with the lock held, drops it for the device
+    // work. ok false: a device error fails every admitted request (queued
+    // ones wait).
+    auto finish = [&](bool ok, const std::vector<Entry*>& cancelled, const RoundStats& rs) {
+        auto is_cancelled = [&](const Entry& e) {
+            return std::find(cancelled.begin(), cancelled.end(), &e) != cancelled.end();
+        };
+        std::vector<SweepRequest*> complete;
+        for (int64_t id : admitted_) {
+            Entry& e = *entries_[id];
+            if (e.state == ReqState::kRunning && e.req.complete && !is_cancelled(e))
+                complete.push_back(&e.req);
+        }
+        lk.unlock();
+        if (ok) ok = sw_.finalize_all(complete, false);
+        for (Entry* e : cancelled) ok = ok && sw_.finalize(e->req, true);
+        if (!ok)  // nothing stays in flight over the requests about to be released
+            while (sw_.in_flight() > 0 && sw_.wait_round(nullptr)) {
+            }
+        lk.lock();
+        st_.kernel_ns += rs.kernel_ns;
+        st_.rounds += rs.rounds;
+        const auto now = Clock::now();
+        for (auto it = admitted_.begin(); it != admitted_.end();) {
+            Entry& e = *entries_[*it];
+            e.progress = sw_.progress(e.req);
+            const bool was_cancelled = is_cancelled(e);
+            const bool done = e.state == ReqState::kRunning && e.req.complete;
+            if (!ok || done || was_cancelled) {
+                e.state = !ok ? ReqState::kFailed
+                              : was_cancelled ? ReqState::kCancelled : ReqState::kDone;
+                e.t_done = now;
+                if (done && ok) st_.centers_done += e.req.centers;
+                sw_.unreserve(e.req);
+                in_use_ -= e.footprint;
+                admit_blocked_ = false;
+                it = admitted_.erase(it);
+                continue;
+            }
+            ++it;
+        }
+        done_cv_.notify_all();
+        if (!ok) (void)hipGetLastError();
+    };
     for (;;) {
         auto has_work = [&] {
             if (stop_) return true;
@@ -1349,13 +1527,37 @@ void Session::worker() {
                     e.want_pause)
                     return true;
             }
-            return false;
+            return sw_.in_flight() > 0;
         };
         cv_.wait(lk, has_work);
         if (stop_) break;
+        // Direct rounds are pipelined: with one round in flight the worker
+        // waits until shortly before that round is modelled to end, then
+        // admits, applies controls and launches the next round, which the
+        // device starts as the current one drains (the host's work between
+        // rounds and the last launch's tail are hidden), and only then waits
+        // for the current round. The model's end can be late (a round that
+        // ends early would leave the device idle): the worker looks at the
+        // round every half millisecond, and a round found ended has its
+        // requests finished first (before new requests are prepared).
+        if (!persistent_ && sw_.in_flight() == 1) {
+            const auto margin =
+                std::chrono::nanoseconds(std::clamp<int64_t>(round_ns_ / 8, 1000000, 3000000));
+            const auto due = sw_.expected_end() - margin;
+            while (!stop_ && Clock::now() < due && !sw_.oldest_ended())
+                cv_.wait_until(lk, std::min(due, Clock::now() + std::chrono::microseconds(500)),
+                               [&] { return stop_; });
+            if (stop_) break;
+            if (sw_.oldest_ended()) {
+                RoundStats rs0{};
+                lk.unlock();
+                const bool ok0 = sw_.wait_round(&rs0);
+                lk.lock();
+                finish(ok0, {}, rs0);
+            }
+        }
 
         // 1. control edits of queued requests, then admission in arrival order
-        std::vector<Entry*> finished;
         for (auto qi = queue_.begin(); qi != queue_.end();) {
             Entry& e = *entries_[*qi];
             if (e.want_cancel) {
@@ -1451,48 +1653,33 @@ void Session::worker() {
                 running.push_back(&e.req);
         }
         lk.unlock();
+
+        // 3. the next round; the older round once a newer one is queued (or
+        // nothing more was)
+        int launched = 0;
         if (ok && !running.empty()) {
             RoundPolicy pol;
             pol.round_ns = round_ns_;
             pol.shortest_first = true;
             pol.direct = !persistent_;
-            ok = sw_.round(running, pol, &rs);
-        }
-        // finish completed and cancelled requests
-        std::vector<SweepRequest*> complete;
-        for (SweepRequest* r : running)
-            if (r->complete) complete.push_back(r);
-        if (ok) ok = sw_.finalize_all(complete, false);
-        for (Entry* e : cancelled) ok = ok && sw_.finalize(e->req, true);
-
-        lk.lock();
-        st_.kernel_ns += rs.kernel_ns;
-        st_.rounds += rs.rounds;
-        const auto now = Clock::now();
-        for (auto it = admitted_.begin(); it != admitted_.end();) {
-            Entry& e = *entries_[*it];
-            e.progress = sw_.progress(e.req);
-            const bool was_cancelled =
-                std::find(cancelled.begin(), cancelled.end(), &e) != cancelled.end();
-            const bool done = e.state == ReqState::kRunning && e.req.complete;
-            if (!ok || done || was_cancelled) {
-                e.state = !ok ? ReqState::kFailed
-                              : was_cancelled ? ReqState::kCancelled : ReqState::kDone;
-                e.t_done = now;
-                if (done && ok) st_.centers_done += e.req.centers;
-                sw_.unreserve(e.req);
-                in_use_ -= e.footprint;
-                admit_blocked_ = false;
-                it = admitted_.erase(it);
-                continue;
+            if (persistent_) {
+                ok = sw_.round(running, pol, &rs);
+            } else {
+                launched = sw_.launch_round(running, pol);
+                ok = launched >= 0;
             }
-            ++it;
         }
-        done_cv_.notify_all();
-        if (!ok) {
-            // a device error fails every admitted request; queued ones wait
-            (void)hipGetLastError();
-        }
+        if (ok && sw_.in_flight() > 0 && (sw_.in_flight() > 1 || launched == 0))
+            ok = sw_.wait_round(&rs);
+        // a cancelled request is not in the round just launched; its earlier
+        // rounds have ended before its partial result is ranked
+        for (Entry* e : cancelled)
+            while (ok && e->req.inflight > 0 && sw_.in_flight() > 0) ok = sw_.wait_round(&rs);
+        lk.lock();
+        finish(ok, cancelled, rs);
+    }
+    // rounds still in flight end before the session's device memory goes
+    while (sw_.in_flight() > 0 && sw_.wait_round(nullptr)) {
     }
 }
 
