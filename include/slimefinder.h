@@ -288,7 +288,10 @@ int32_t slime_sweep_costs(double* ns_row, double* extra_rows, int32_t* nb, int32
  * centers: radius 8,388,607), over the budget alone, or with more jobs or
  * hits than the session holds is refused at submit (-1).
  * slime_session_create runs direct rounds (one workgroup per tile, the
- * per-search kernels' register budget; pause and cancel act between rounds);
+ * per-search kernels' register budget; pause and cancel act between rounds),
+ * pipelined: the next round is queued shortly before the current one ends, so
+ * a pause or cancel takes effect after at most the round in flight and short
+ * rounds (10 ms) cost little throughput;
  * slime_session_create_ex with flags bit 0 runs the persistent work-queue
  * kernels instead, which read each request's status word before every tile
  * claim. NULL on bad arguments or no device. */
