@@ -240,3 +240,45 @@ def test_served_full_world_paused_equals_alone(gpu):
         got = s.result(rid, k)
     assert got[0] == 12
     assert got == alone(FULL_WORLD, k)
+
+
+def test_pipelined_short_rounds(gpu, oracle):
+    """1 ms rounds, so that the next round is nearly always queued while the
+    current one runs: requests admitted between rounds, one paused and
+    resumed, one cancelled while its tiles are in flight. Finished results
+    equal the same search alone; the cancelled one holds exact rows of the
+    tiles that finished."""
+    import slime_amd
+    lib = oracle.oracle_lib()
+    reqs = searches(0x51DE, 12, [300, 3000, 40_000, 150_000])
+    k = 300
+    with slime_amd.Session(hbm_budget=2 << 30, max_active=8, round_ms=1) as s:
+        ids = []
+        for i, r in enumerate(reqs):
+            ids.append(s.submit(*r, max_results=k))
+            time.sleep(0.002)
+            if i == 5:
+                s.pause(ids[2])
+        victim = ids[11]
+        deadline = time.time() + 30
+        while s.poll(victim)[1] == 0 and time.time() < deadline:
+            time.sleep(0.0005)
+        assert s.cancel(victim)
+        assert s.resume(ids[2]) or s.poll(ids[2])[0] == slime_amd.DONE
+        for r, rid in zip(reqs, ids):
+            st = s.wait(rid, 120000)
+            total, rows = s.result(rid, k)
+            if rid == victim and st == slime_amd.CANCELLED:
+                full = alone(r, k)
+                print("cancelled at", total, "of", full[0], flush=True)
+                assert total <= full[0] and len(rows) <= min(total, k)
+                for row in rows:
+                    assert lib.oracle_disk_count(r[0], (row[1] - 8) // 16,
+                                                 (row[2] - 8) // 16) == row[0]
+                continue
+            assert st == slime_amd.DONE, (r, st)
+            assert (total, rows) == alone(r, k), r
+        rounds = s.stats()["rounds"]
+        print("rounds", rounds, flush=True)
+        # a round is at least one wave of each class's tiles (a few ms here)
+        assert rounds > 8
