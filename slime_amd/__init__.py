@@ -452,9 +452,11 @@ QUEUED, RUNNING, PAUSED, DONE, CANCELLED, FAILED = range(6)
 class Session:
     """Served searches with continuous batching (slime_session_*): requests
     are admitted by device footprint, scanned together in rounds, and can be
-    paused, resumed and cancelled while others run."""
+    paused, resumed and cancelled while others run. round_ms is the device
+    time of one round; rounds are pipelined, so short rounds (the default 10
+    ms) cost little throughput."""
 
-    def __init__(self, hbm_budget=4 << 30, max_active=64, round_ms=20, persistent=False):
+    def __init__(self, hbm_budget=4 << 30, max_active=64, round_ms=10, persistent=False):
         self._h = lib().slime_session_create_ex(hbm_budget, max_active, round_ms,
                                                 1 if persistent else 0)
         if not self._h:
