@@ -55,6 +55,7 @@ double center_tail(int t) {
 
 constexpr int64_t kMaxHitCap = int64_t(1) << 24;
 constexpr int32_t kMaxTileH = 2048;
+constexpr int32_t kMinTileH = 16;
 constexpr int32_t kTileHalo = 16;  // generated rows beyond the tile's slot rows
 // Hits a finishing request may bring back after the device selection (4 MiB).
 constexpr uint32_t kSelectCap = 1u << 18;
@@ -373,11 +374,20 @@ bool Sweeper::plan(SweepRequest& r) const {
         int nb;
         int32_t s_lo, s_hi;  // slot rows [s_lo, s_hi)
     };
-    auto tile_for = [](int32_t s_rows) { return std::min(kMaxTileH, (s_rows + 3) / 4 * 4); };
+    // the largest tile that fits the rows, but short enough that a small
+    // region still has about one tile per CU (a tile's rows are generated one
+    // step after another, so few tall tiles bound a small search's time)
+    auto tile_for = [&](const XPart& xp, int32_t s_rows) {
+        const int64_t col_tiles = (int64_t)xp.nblocks * xp.tiles_per_block();
+        const int64_t fit = ((int64_t)s_rows * col_tiles + cus_ - 1) / cus_;
+        return (int32_t)std::max<int64_t>(
+            kMinTileH, std::min<int64_t>(std::min<int64_t>(kMaxTileH, (s_rows + 3) / 4 * 4),
+                                         (fit + 3) / 4 * 4));
+    };
     auto group_cost = [&](const XPart& xp, int nb, int32_t s_rows) {
         const int c = sweep_class_of(nb, xp.ph);
         if (c < 0) return -1.0;
-        const int32_t th = tile_for(s_rows);
+        const int32_t th = tile_for(xp, s_rows);
         const int64_t tiles = (int64_t)xp.nblocks * xp.tiles_per_block() * ((s_rows + th - 1) / th);
         return (double)tiles * class_tile_ns(c, th);
     };
@@ -474,7 +484,8 @@ bool Sweeper::plan(SweepRequest& r) const {
     for (size_t i = 0; i < best_parts.size(); ++i)
         for (const Group& g : best_groups[i])
             band_jobs_part(spec, best_parts[i], g.b_first, g.nb, g.s_hi,
-                           tile_for(g.s_hi - g.s_lo), int64_t(1) << 62, r.jobs, g.s_lo);
+                           tile_for(best_parts[i], g.s_hi - g.s_lo), int64_t(1) << 62, r.jobs,
+                           g.s_lo);
     r.job_cls.clear();
     r.centers = 0;
     for (const BandJob& j : r.jobs) {
