@@ -282,3 +282,26 @@ def test_pipelined_short_rounds(gpu, oracle):
         print("rounds", rounds, flush=True)
         # a round is at least one wave of each class's tiles (a few ms here)
         assert rounds > 8
+
+
+def test_session_closed_while_rounds_in_flight(gpu):
+    """Closing a session while a large request runs: the worker lets the
+    rounds in flight end and stops (no hang, memory released), and a new
+    session on the same device serves requests as before."""
+    import slime_amd
+    big = (0x5E55C105E, 1_200_000, 60, 221, 0, 12345, -54321)
+    small = (0xC105ED, 3000, 50, 221, 0, -999, 777)
+    t0 = time.time()
+    with slime_amd.Session(hbm_budget=4 << 30, max_active=4, round_ms=10) as s:
+        rid = s.submit(*big, max_results=100)
+        deadline = time.time() + 30
+        while s.poll(rid)[1] == 0 and time.time() < deadline:
+            time.sleep(0.002)
+        assert s.poll(rid)[0] == slime_amd.RUNNING
+    closed = time.time() - t0
+    print("closed after", closed, "s", flush=True)
+    assert closed < 10
+    with slime_amd.Session(hbm_budget=1 << 30, max_active=4, round_ms=10) as s:
+        rid = s.submit(*small, max_results=100)
+        assert s.wait(rid, 60000) == slime_amd.DONE
+        assert s.result(rid, 100) == alone(small, 100)
