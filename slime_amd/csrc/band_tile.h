@@ -647,12 +647,18 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
                     // keeps it where it is even)
                     constexpr int kPairs = (NB + 1) / 2;
                     const uint32_t xs = PH == 1 ? xh0[j] : xh2[PH == 3 ? j : 0];
+                    // the slot's bias as a value of its own: otherwise the
+                    // compiler folds its product into each test's sum (two
+                    // multiplies and a three-operand add where a multiply-add
+                    // does)
+                    uint32_t bias = sl.bias;
+                    if constexpr (PH == 1) asm("" : "+v"(bias));
                     uint32_t v2[kPairs];
                     static_for<0, kPairs>([&](auto kc) {
                         constexpr int k = decltype(kc)::value;
                         uint32_t zs = zhi[2 * k];
                         if constexpr (2 * k + 1 < NB) zs = sl.e ? zhi[2 * k + 1] : zhi[2 * k];
-                        v2[k] = ((xs + zs + sl.carry) ^ kHighScramble) * kMulInv5 + sl.bias;
+                        v2[k] = ((xs + zs + sl.carry) ^ kHighScramble) * kMulInv5 + bias;
                     });
                     static_for<0, kPairs>([&](auto kc) {
                         constexpr int k = decltype(kc)::value;
