@@ -628,9 +628,13 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
                         v[b] = ((((b & 1) ? xb : xa) + zv[b]) ^ kHighScramble) * kMulInv5 + sl.bias;
                     });
                 } else {
+                    // with one band, the bias as a value of its own (as for
+                    // one phase below: one multiply-add per test)
+                    BandSlot sb = sl;
+                    if constexpr (NB == 1) asm("" : "+v"(sb.bias));
                     static_for<0, NB>([&](auto bc) {
                         constexpr int b = decltype(bc)::value;
-                        v[b] = band_value(sl, b, zhi[b]);
+                        v[b] = band_value(sb, b, zhi[b]);
                     });
                 }
                 // shift in the complement of each hit: [a*K + bias > limit]
