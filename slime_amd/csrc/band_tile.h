@@ -606,6 +606,18 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
 #pragma unroll
                 for (int b = 0; b < NB; ++b) asm("v_mov_b32 %0, %1" : "=v"(zv[b]) : "s"(zhi[b]));
             }
+            // The z words of the band pairs tested as one (the phase without a
+            // partner) copied to VGPRs once per row, so that each slot selects
+            // its word with one v_cndmask: selected from the scalar words, the
+            // choice of two neighbouring array elements became an indexed LDS
+            // read per slot.
+            constexpr int kZPairs = PH != 2 ? NB / 2 : 0;
+            uint32_t zpe[kZPairs > 0 ? kZPairs : 1], zpo[kZPairs > 0 ? kZPairs : 1];
+#pragma unroll
+            for (int k = 0; k < kZPairs; ++k) {
+                asm("v_mov_b32 %0, %1" : "=v"(zpe[k]) : "s"(zhi[2 * k]));
+                asm("v_mov_b32 %0, %1" : "=v"(zpo[k]) : "s"(zhi[2 * k + 1]));
+            }
             // The bit of chunk kChunksPerLane*l + j lands in bit j of each plane's
             // register, so the 64 lanes' bytes tile the 512-bit row.
             uint32_t acc[R::kPlanes];
@@ -661,7 +673,7 @@ __device__ __forceinline__ void band_tile(const BandParams& p, int32_t emit_min,
                     static_for<0, kPairs>([&](auto kc) {
                         constexpr int k = decltype(kc)::value;
                         uint32_t zs = zhi[2 * k];
-                        if constexpr (2 * k + 1 < NB) zs = sl.e ? zhi[2 * k + 1] : zhi[2 * k];
+                        if constexpr (2 * k + 1 < NB) zs = sl.e ? zpo[k] : zpe[k];
                         v2[k] = ((xs + zs + sl.carry) ^ kHighScramble) * kMulInv5 + bias;
                     });
                     static_for<0, kPairs>([&](auto kc) {
