@@ -63,10 +63,11 @@ __device__ bool claim_tile(const SweepArgs& a, int& pos, int& job, int& tile) {
 // rolling state fits. The 2-band, two-phase class runs at four waves in both
 // forms (at five, with the claim loop around the tile and, in the union-bound
 // form, the matrix-core window sums, it spilled). The three-phase kernels
-// (union bound only) run at four, the 12- and 14-band union-bound kernels at
-// two (their one-tile form needs up to 167 VGPRs before the claim loop).
+// (union bound only) run at four (the 5-band one at three: at four it spilled
+// 16 bytes), the 12- and 14-band union-bound kernels at two (their one-tile
+// form needs up to 167 VGPRs before the claim loop).
 constexpr int kSweepWaves[kSweepKernels] = {6, 5, 5, 5, 5, 4, 4, 4, 4, 3, 3, 6, 5, 4,
-                                            4, 5, 4, 4, 4, 3, 3, 4, 4, 4, 4, 2, 3, 2};
+                                            4, 5, 4, 4, 4, 3, 3, 4, 4, 4, 3, 2, 3, 2};
 
 template <int K>
 __global__ __launch_bounds__(kThreads, kSweepWaves[K]) void sweep_kernel(SweepArgs a) {
