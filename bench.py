@@ -505,8 +505,8 @@ def latency_stats(lat, reqs, idx):
 
 # Device time per served round. Rounds are pipelined (the next one is queued
 # while the current one runs), so short rounds cost little throughput: on the
-# served mix 10 ms rounds gave small requests a p50 of 25 ms at the aggregate
-# that 20 ms rounds had before the pipelining (p50 37 ms).
+# served mix 10 ms rounds keep small requests near a p50 of 25-30 ms at about
+# the aggregate of one-at-a-time calls.
 SERVED_ROUND_MS = 10
 
 
