@@ -68,9 +68,11 @@ VALU_INSTS_PER_CENTER = 4.59
 CPU_SAMPLE_RADIUS = 131072
 # HBM bytes per scan_band_kernel launch (rocprofv3 FETCH_SIZE + WRITE_SIZE of
 # the full-world run, scaled by the same counters on a copy of known size;
-# profiles/scan_band_kernel_full_world_thr60.md): 96.01 KiB x 2.0 + 0.02 KiB x 1.0
-# per launch, averaged over the search's 32 launches.
-TRAFFIC_BYTES = 196660
+# profiles/scan_band_kernel_full_world_thr60.md): 81.4 KiB x 2.0 + 0.05 KiB x 1.0
+# per launch, averaged over the search's launches after the first two (those
+# two share their counter windows with the first-draw fix work running on the
+# second stream).
+TRAFFIC_BYTES = 166754
 
 
 def host_cpu():

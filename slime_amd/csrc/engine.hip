@@ -43,7 +43,11 @@ namespace {
 constexpr uint32_t kHitCap = 1u << 24;                       // hits per launch (256 MiB)
 constexpr size_t kEagerHits = 4096;  // hits copied back with every launch's counters (64 KB)
 constexpr int64_t kPlainCentersPerLaunch = int64_t(1) << 37;
-constexpr int64_t kBandCentersPerLaunch = int64_t(1) << 39;
+// Centers per banded launch at most: each launch ends in a partial last wave
+// of tiles, so fewer, longer launches waste less (full world: 2^39 9.90, 2^40
+// 9.99-10.02, 2^41 10.06-10.07 T centers/s end to end); pause and cancel act
+// between launches (about 0.2 s apart at this size).
+constexpr int64_t kBandCentersPerLaunch = int64_t(1) << 41;
 constexpr int kBandTileH = 2048;
 // Small regions get short tiles so that their few tiles spread over the
 // device: each tile's rows are generated one step after another, so a search
