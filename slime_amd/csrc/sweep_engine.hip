@@ -1663,6 +1663,12 @@ void Session::worker() {
                 std::find(cancelled.begin(), cancelled.end(), &e) == cancelled.end())
                 running.push_back(&e.req);
         }
+        // With nothing queued and every running request more than a few
+        // rounds from its end, no latency is at stake: rounds twice as long
+        // (fewer launch tails); a new arrival brings short rounds back.
+        bool long_rounds = queue_.empty();
+        for (const SweepRequest* r : running)
+            long_rounds = long_rounds && r->remaining_ns > 4.0 * (double)round_ns_;
         lk.unlock();
 
         // 3. the next round; the older round once a newer one is queued (or
@@ -1670,7 +1676,7 @@ void Session::worker() {
         int launched = 0;
         if (ok && !running.empty()) {
             RoundPolicy pol;
-            pol.round_ns = round_ns_;
+            pol.round_ns = long_rounds ? 2 * round_ns_ : round_ns_;
             pol.shortest_first = true;
             pol.direct = !persistent_;
             if (persistent_) {
